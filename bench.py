@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark of the nonbonded + PME real-space hot path on one MI355X.
+
+One step = the hot path of one MD force evaluation on the 96,000-atom
+solvated box (26,000 TIP3P + 60 chains x 300 beads, 98 A cube, cutoff 12 A,
+switchdist 10 A, pairlistdist 13.5 A, PME 108^3 order 4):
+
+    PME fractional coordinates -> grid clear -> charge spread -> force gather
+    -> nonbonded pair kernel (fast + slow forces) -> force transpose
+
+Pairlists are rebuilt (tile-list build + refinement in the pair kernel) every
+20 steps and energies + virial are reduced on those steps too; the other
+steps are force-only, as in NAMD's GPU-resident defaults.  The PME k-space
+solve (FFT + convolution), bonded terms and integration are not part of this
+path and are not timed.  Inputs (positions, charges, tables, tile lists) are
+resident in HBM when the timed region starts.
+
+Prints one JSON line.  --gpus above 1 exits: this build measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "namd-3.0_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "ns/day + nonbonded Gpairs/s at 96k atoms; force RMS err vs FP64 oracle"
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFLOPS = 157.3
+FLOPS_PER_PAIR = 72  # force-only direct math, see DESIGN.md
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--pairlist-every", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mi355nb import compute as mc
+    from mi355nb import systems
+
+    if not torch.cuda.is_available():
+        log("no GPU visible")
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    log("building solvated96k system")
+    s = systems.solvated_96k()
+    p = s.params
+    c = mc.ComputeNonbonded.from_system(s, stream=sp)
+    info = c.info()
+    box = np.array(info["box"])
+    recip = np.diag(1.0 / box)
+    K = p.pme_grid
+    order = p.pme_order
+    n = s.natoms
+    pos_d = torch.from_numpy(np.ascontiguousarray(s.pos)).to(dev)
+    q_d = torch.from_numpy(np.ascontiguousarray(s.charge)).to(dev)
+    atoms = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    grid = torch.zeros(K[0] * K[1] * K[2], dtype=torch.float32, device=dev)
+    fpme = torch.empty((n, 4), dtype=torch.float32, device=dev)
+
+    ev = {}
+
+    def step(k, timed=False):
+        pl = (k % args.pairlist_every) == 0
+        mc.frac_coords(pos_d, q_d, recip, out=atoms, stream=sp)
+        grid.zero_()
+        if timed:
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e[0].record(stream)
+        mc.spread_charge(atoms, grid, K, order, stream=sp)
+        if timed:
+            e[1].record(stream)
+        mc.gather_force(atoms, grid, K, order, recip=recip, out=fpme, stream=sp)
+        if timed:
+            e[2].record(stream)
+            ev.setdefault("pme", []).append(e)
+        c.launch_work(do_energy=pl, do_virial=pl, do_slow=True, do_pairlist=pl)
+
+    log(f"warmup {args.warmup} steps")
+    for k in range(max(1, args.warmup)):
+        step(k)
+    torch.cuda.synchronize()
+    r0 = c.finish(forces=False)
+    npairs = int(r0["stat"]["numPairsCutoff"])
+    nlists = int(r0["stat"]["numTileLists"])
+    njt = int(r0["stat"]["numJtiles"])
+
+    log(f"timing {args.steps} steps")
+    c.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(k, timed=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_pl, n_pl, ms_f, n_f = c.kernel_time()
+    c.set_timing(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    ns_day = 2e-6 * 86400.0 / (ms_step * 1e-3)
+    spread_ms = np.mean([a.elapsed_time(b) for a, b, _ in ev["pme"]])
+    gather_ms = np.mean([b.elapsed_time(cc) for _, b, cc in ev["pme"]])
+    t_force_ms = ms_f / max(n_f, 1)
+    t_pl_ms = ms_pl / max(n_pl, 1)
+    gpairs = npairs / (t_force_ms * 1e-3) / 1e9
+
+    # algorithmic bytes of one force-only pair-kernel launch
+    nslots = info["atomStorageSize"]
+    bytes_compulsory = nslots * (16 + 4 + 24) + nlists * 44 + njt * (4 + 128)
+    bytes_pervisit = bytes_compulsory + njt * 32 * (20 + 2 * 24) + nlists * 32 * 20
+    achieved = bytes_compulsory / (t_force_ms * 1e-3) / 1e9
+    valu_tflops = FLOPS_PER_PAIR * npairs / (t_force_ms * 1e-3) / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": round(ns_day, 2),
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "solvated96k: 26,000 TIP3P + 60 chains x 300 beads, 98 A cube, rc 12 A, switch 10 A, "
+                        "pairlist 13.5 A, PME 108^3 order 4; step = PME frac coords + spread + gather + "
+                        "nonbonded fast+slow forces, pairlist + energy/virial every 20 steps, dt 2 fs",
+            "natoms": n,
+            "pairlist_every": args.pairlist_every,
+            "dt_fs": 2.0,
+        },
+        "gpairs_per_s": round(gpairs, 2),
+        "pairs_in_cutoff": npairs,
+        "kernel_ms": {"nonbonded_force": round(t_force_ms, 4), "nonbonded_pairlist": round(t_pl_ms, 4),
+                      "pme_spread": round(float(spread_ms), 4), "pme_gather": round(float(gather_ms), 4)},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "kernel": "nonbondedForceKernel<doEnergy=0,doVirial=0,doSlow=1,doPairlist=0>",
+            "bytes_model": "compulsory",
+            "bytes_per_launch": int(bytes_compulsory),
+        },
+        "roofline_pervisit_hbm": {"bytes_per_launch": int(bytes_pervisit),
+                                  "achieved": round(bytes_pervisit / (t_force_ms * 1e-3) / 1e9, 2),
+                                  "frac": round(bytes_pervisit / (t_force_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "roofline_valu": {"achieved": round(valu_tflops, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(valu_tflops / VALU_PEAK_TFLOPS, 4), "flops_per_pair": FLOPS_PER_PAIR},
+    }
+
+    if not args.no_cpu_baseline:
+        out.update(cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args, ms_step))
+    print(json.dumps(out), flush=True)
+    c.close()
+
+
+def cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args, gpu_ms_step):
+    """The oracle timed on the host cores on the full 96k system (one force
+    evaluation + one PME spread/gather), and the GPU force error against it."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as orc
+    from mi355nb import lib as L
+    from mi355nb import systems
+
+    p = s.params
+    ncores = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", 0) or os.cpu_count() or 1)
+    pos = c.effective_positions()
+    beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+    A, B = L.lj_mix(s.lj_sigma, s.lj_eps)
+    q = s.charge * math.sqrt(332.0636)
+    log(f"cpu baseline: oracle on {ncores} threads")
+    t0 = time.perf_counter()
+    o = orc.nb_fp64(pos, q, s.vdw_type, A, B, s.excl_ptr, s.excl_idx, box, p.cutoff, p.switchdist, beta,
+                    nthreads=ncores)
+    t1 = time.perf_counter()
+    a = atoms.cpu().numpy()
+    g = orc.pme_spread_gpu(a, K, order)
+    orc.pme_gather_gpu(a, K, order, g, recip)
+    t2 = time.perf_counter()
+    cpu_step = (t1 - t0) + (t2 - t1)
+    cpu_ns_day = 2e-6 * 86400.0 / cpu_step
+
+    # GPU forces of one full evaluation vs the FP64 oracle
+    c.launch_work(True, True, True, True)
+    r = c.finish()
+
+    def rel(x, y):
+        return float(np.sqrt(((x - y) ** 2).sum() / (y ** 2).sum()))
+
+    err = {"fast": rel(r["f"], o["f"]), "slow": rel(r["fs"], o["fs"]),
+           "total": rel(r["f"] + r["fs"], o["f"] + o["fs"]),
+           "e_vdw": abs(r["e_vdw"] - o["e_vdw"]) / abs(o["e_vdw"]),
+           "e_elec": abs(r["e_elec"] - o["e_elec"]) / abs(o["e_elec"]),
+           "e_slow": abs(r["e_slow"] - o["e_slow"]) / abs(o["e_slow"]),
+           "virial": float(np.linalg.norm(r["virial"] + r["virial_slow"] - o["virial"] - o["virial_slow"])
+                           / np.linalg.norm(o["virial"] + o["virial_slow"]))}
+    return {
+        "force_rel_rms_err": {k: float(f"{v:.3e}") for k, v in err.items()},
+        "cpu_baseline": {
+            "value": round(cpu_ns_day, 4),
+            "unit": "ns/day",
+            "cores": ncores,
+            "kind": "port",
+            "sample": "one full evaluation of the 96k system: FP64 oracle nonbonded (all pairs < 12 A, cell list, "
+                      f"{ncores} OpenMP threads) + FP64 PME spread and gather (1 thread); "
+                      f"{cpu_step:.2f} s per step",
+        },
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,226 @@
+/*
+ * mi355nb - MI355X (gfx950) native NAMD short-range nonbonded force path and
+ * PME charge spread / force gather, behind a plain C ABI.
+ *
+ * Every entry point returns an int status: 0 on success, negative on error
+ * (-1 bad argument, -2 state / contract violation, -3 capacity, <= -1000 a HIP
+ * runtime error).  mi355nb_last_error() returns the message of the last
+ * failure on the calling thread.  Nothing in the library aborts the process;
+ * NAMD's convention is cudaCheck -> cudaDie -> NAMD_die (src/CudaUtils.h:214-221),
+ * so a NAMD-side shim maps a nonzero status to NAMD_die.
+ *
+ * Threading: one mi355nb_compute per (device, stream), driven from one host
+ * thread, as NAMD drives CudaComputeNonbonded from its masterPe
+ * (src/CudaComputeNonbonded.C:1090-1091, :1154-1155).  All device work is
+ * asynchronous on the stream given at creation; *_finish synchronises.
+ *
+ * Device pointers are plain pointers into HBM (e.g. from hipMalloc or a torch
+ * tensor's data_ptr()).  float4 arrays are passed as float* with 4 floats per
+ * element.
+ */
+#ifndef MI355NB_H
+#define MI355NB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI355NB_ABI_VERSION 1
+
+#define MI355NB_OK 0
+#define MI355NB_ERR_ARG -1
+#define MI355NB_ERR_STATE -2
+#define MI355NB_ERR_CAPACITY -3
+
+/* CudaNBConstants (src/CudaUtils.h:574-590), identical field order. */
+typedef struct {
+  float lj_0, lj_1, lj_2, lj_3, lj_4, lj_5;
+  float e_0, e_0_slow, e_1, e_2;
+  float ewald_0, ewald_1, ewald_2, ewald_3_slow;
+  float slowScale;
+} mi355nb_nb_constants;
+
+/* VirialEnergy (src/CudaTileListKernel.h:96-117), identical field order. */
+typedef struct {
+  double virial[9];
+  double virialSlow[9];
+  double energyVdw, energyElec, energySlow;
+  double energyVdw_s, energyElec_s, energySlow_s;
+  double energyVdw_ti_1, energyVdw_ti_2;
+  double energyElec_ti_1, energyElec_ti_2;
+  double energySlow_ti_1, energySlow_ti_2;
+  double energyGBIS;
+} mi355nb_virial_energy;
+
+/* TileListStat (src/CudaTileListKernel.h:62-70) plus pair checksums. */
+typedef struct {
+  int32_t numTileLists;         /* lists with at least one interacting j-tile */
+  int32_t numJtiles;            /* j-tiles kept after refinement */
+  int32_t numExcluded;          /* excluded pairs within the pairlist cutoff */
+  int32_t maxDepth;             /* most j-tiles in one list */
+  uint64_t numPairsCutoff;      /* non-excluded pairs with r^2 < cutoff^2 */
+  uint64_t numPairsList;        /* non-excluded pairs with r^2 < pairlistdist^2 */
+} mi355nb_tile_list_stat;
+
+/* The SimParameters values the path reads (cutoff src/SimParameters.C:548,
+ * switchdist :578, pairlistdist :589, margin :642; ewaldcof from
+ * mi355nb_ewald_coefficient; slowScale = fullElectFrequency/nonbondedFrequency). */
+typedef struct {
+  double cutoff;
+  double switchdist;
+  double pairlistdist;
+  double ewaldcof;
+  double slowScale;
+  double margin;
+} mi355nb_nb_params;
+
+typedef struct {
+  int32_t natoms;
+  int32_t atomStorageSize;      /* padded slots, multiple of 32 */
+  int32_t numPatches;
+  int32_t numComputes;
+  int32_t numCandidateLists;    /* (compute, i-tile) lists before refinement */
+  int32_t patchDims[3];
+  int64_t jtileCapacity;
+  double box[3];                /* edge lengths after quantisation to 2^-17 A */
+} mi355nb_layout_info;
+
+/* Device pointers owned by a compute object (valid until the next
+ * mi355nb_compute_assign_patches or destroy). */
+typedef struct {
+  float* xyzq;                  /* float4[atomStorageSize] patch-centred x,y,z, scaled q */
+  float* force;                 /* float4[atomStorageSize] fast force (after launch) */
+  float* forceSlow;             /* float4[atomStorageSize] slow force */
+  int32_t* atomIndex;           /* int[atomStorageSize] global atom id of each slot */
+} mi355nb_device_buffers;
+
+typedef struct mi355nb_compute mi355nb_compute;
+
+const char* mi355nb_last_error(void);
+int mi355nb_abi_version(void);
+
+/* PMEEwaldCoefficient bisection; replaces src/SimParameters.C:4243-4256. */
+double mi355nb_ewald_coefficient(double cutoff, double tolerance);
+
+/* CudaComputeNonbonded::getNonbondedCoef; replaces src/CudaComputeNonbonded.C:2318-2349. */
+int mi355nb_get_nonbonded_coef(double cutoff, double switchdist, double ewaldcof, double slowScale,
+                               mi355nb_nb_constants* out);
+
+/* CudaComputeNonbonded::getDoTable; replaces src/CudaComputeNonbonded.C:2351-2366.
+ * Always 0: this library evaluates LJ + Ewald by direct math on every step. */
+int mi355nb_get_do_table(int useCUDANonbondedForceTable, int doSlow, int doVirial);
+
+/* LJTable::compute_vdw_params mixing; replaces src/LJTable.C:118-135.
+ * A and B are ntypes*ntypes row-major. */
+int mi355nb_lj_mix(int ntypes, const double* sigma, const double* epsilon, int geometric,
+                   double* A, double* B);
+
+/* CudaComputeNonbonded::buildExclusions; replaces src/CudaComputeNonbonded.C:2191-2316.
+ * excl_ptr/excl_idx: CSR full exclusion lists (no self).  Call with bits == NULL
+ * to get *nwords; then call again with bits of *nwords words.  start_maxdiff
+ * receives 2 ints per atom: bit index of offset 0, max |offset|. */
+int mi355nb_build_exclusions(int natoms, const int64_t* excl_ptr, const int32_t* excl_idx,
+                             int32_t* start_maxdiff, uint32_t* bits, int64_t* nwords, int32_t* nlists);
+
+/* Converts natural-order tile masks (bit j of word i = pair (i, j)) to NAMD's
+ * rotated order (bit t of word i = pair (i, (i+t) mod 32);
+ * src/CudaComputeNonbondedKernel.cu:1129-1149). */
+int mi355nb_tile_excl_to_namd(const uint32_t* natural, uint32_t* rotated, int64_t ntiles);
+
+/* ---- CudaComputeNonbonded (src/CudaComputeNonbonded.h:279-309) ---- */
+
+/* Constructor + initialize (src/CudaComputeNonbonded.C:627-682).  stream may be NULL. */
+int mi355nb_compute_create(const mi355nb_nb_params* params, int device, void* stream,
+                           mi355nb_compute** out);
+int mi355nb_compute_destroy(mi355nb_compute* c);
+
+/* Molecule data: vdw types, charges (e; scaled by sqrt(COULOMB) internally,
+ * src/HomePatch.C:1965-1995), LJ A/B table (CudaNonbondedTables::buildVdwCoefTable,
+ * src/CudaNonbondedTables.C:110-148) and exclusions (buildExclusions). */
+int mi355nb_compute_set_molecule(mi355nb_compute* c, int natoms, const int32_t* vdw_type,
+                                 const double* charge, int ntypes, const double* lj_A,
+                                 const double* lj_B, const int64_t* excl_ptr, const int32_t* excl_idx);
+
+/* assignPatches + registerComputeSelf/Pair + updatePatches + atom migration
+ * (src/CudaComputeNonbonded.C:173-197, :691-758; src/SortAtoms.C:45-132):
+ * builds the patch grid for an orthorhombic box, the self + 13 half-shell
+ * computes, ORB-sorted padded storage, and uploads.  pos: natoms*3 doubles. */
+int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], const double* pos);
+
+/* Per-step xyzq staging for the current assignment (openBoxesOnPe /
+ * copyAtomsSubset, src/CudaComputeNonbonded.C:903-1052) and upload. */
+int mi355nb_compute_update_positions(mi355nb_compute* c, const double* pos);
+
+/* launchWork -> doForce -> CudaComputeNonbondedKernel::nonbondedForce
+ * (src/CudaComputeNonbonded.C:1153-1562; src/CudaComputeNonbondedKernel.cu:2076-3590).
+ * doPairlist rebuilds tile lists (CudaTileListKernel::buildTileLists,
+ * src/CudaTileListKernel.cu:1022-1195) and refines them in the same pass. */
+int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow, int doPairlist);
+
+/* finishPatch + finishReductions (src/CudaComputeNonbonded.C:1606-1855):
+ * waits for the stream, returns forces in global atom order (natoms*3
+ * doubles; either pointer may be NULL), the FP64 virial/energy and the last
+ * pairlist statistics (either may be NULL). */
+int mi355nb_compute_finish(mi355nb_compute* c, double* force, double* forceSlow,
+                           mi355nb_virial_energy* ve, mi355nb_tile_list_stat* stat);
+
+/* Kernel timing with HIP events recorded on the compute's stream around every
+ * pair-kernel launch (no counterpart in the reference; NAMD brackets the
+ * kernel with Projections events, src/CudaComputeNonbonded.C:1957-1973).
+ * get_kernel_time synchronises and returns the summed milliseconds and
+ * launch counts since the last reset, split by pairlist / non-pairlist. */
+int mi355nb_compute_set_timing(mi355nb_compute* c, int enable);
+int mi355nb_compute_get_kernel_time(mi355nb_compute* c, double* msPairlist, int32_t* nPairlist, double* msForce,
+                                    int32_t* nForce);
+
+int mi355nb_compute_get_info(const mi355nb_compute* c, mi355nb_layout_info* info);
+int mi355nb_compute_get_device_buffers(const mi355nb_compute* c, mi355nb_device_buffers* out);
+int mi355nb_compute_get_nonbonded_coef(const mi355nb_compute* c, mi355nb_nb_constants* out);
+
+/* Positions the kernels actually see, in global atom order (natoms*3
+ * doubles): patch centre + patch-centred position rounded to 2^-17 A. */
+int mi355nb_compute_get_effective_positions(const mi355nb_compute* c, double* pos);
+
+/* Active (refined) tile lists after the last pairlist launch: for each list
+ * its i-atom slot, j-tile range and shift; per j-tile its j-atom slot and 32
+ * natural-order mask words.  Pass NULL arrays to query the counts. */
+int mi355nb_compute_get_tile_lists(mi355nb_compute* c, int32_t* numLists, int32_t* numJtiles,
+                                   int32_t* iatomStart, int32_t* jtileStart, int32_t* jtileEnd,
+                                   float* shift /*3 per list*/, int32_t* jatomStart,
+                                   uint32_t* excl /*32 per j-tile*/, int32_t* slotAtomIndex);
+
+/* ---- PME real space (src/CudaPmeSolverUtilKernel.h:6-51) ----
+ * atoms: float4 per atom (fractional x,y,z in [0,1) as produced by
+ * mi355nb_pme_frac_coords, charge scaled by sqrt(COULOMB)).  Grid layout
+ * data[x + xdim*(y + nffty*z)], xdim >= nfftx.  order 4, 6 or 8. */
+
+/* spread_charge (CudaPmeSolverUtilKernel.cu:426-645); accumulates into grid. */
+int mi355nb_spread_charge(const float* atoms, int natoms, int nfftx, int nffty, int nfftz, int xdim,
+                          float* grid, int order, void* stream);
+
+/* gather_force (CudaPmeSolverUtilKernel.cu:707-915).  recip == NULL: writes the
+ * fractional-gradient force -q*(K1*sum dtx ty tz Q, ...) as NAMD does;
+ * otherwise recip holds the reciprocal lattice vectors a*,b*,c* (row-major
+ * 3x3) and the Cartesian force f1*a* + f2*b* + f3*c* is written
+ * (src/ComputePmeCUDA.C:643-652).  force: float4 per atom (w = 0). */
+int mi355nb_gather_force(const float* atoms, int natoms, int nfftx, int nffty, int nfftz, int xdim,
+                         const float* grid, int order, const double* recip, float* force, void* stream);
+
+/* selfEnergyKernel (CudaPmeSolverUtilKernel.cu:1191-1236): *energy = -beta/sqrt(pi) * sum q^2
+ * (device double, overwritten). */
+int mi355nb_self_energy(const float* atoms, int natoms, double ewaldcof, double* energy, void* stream);
+
+/* Fractional PME coordinates (src/ComputePmeCUDA.C:283-306): w = r.(a*,b*,c*),
+ * w - (floor(w + 0.5) - 0.5), wrapped below 1, charge * sqrt(COULOMB).
+ * pos, charge: device doubles (natoms*3, natoms). */
+int mi355nb_pme_frac_coords(const double* pos, const double* charge, int natoms, const double recip[9],
+                            const double origin[3], float* atoms, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MI355NB_H */

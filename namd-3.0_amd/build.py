@@ -1,0 +1,66 @@
+"""Build recipe for libmi355nb.so (hipcc, gfx950).
+
+    python namd-3.0_amd/build.py            # build in place
+    python namd-3.0_amd/build.py --check    # print the commands only
+
+Every source is compiled with packed FP32 instructions disabled
+(-Xclang -target-feature -Xclang -packed-fp32-ops); hipcc's host pass warns
+that it does not know that feature, which is harmless.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libmi355nb.so")
+
+SOURCES = ["layout.cpp", "nb_kernels.hip", "compute.hip", "pme.hip"]
+ARCH = os.environ.get("MI355NB_ARCH", "gfx950")
+
+
+def hipcc() -> str:
+    for cand in (os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc"), shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found (ROCm is required to build libmi355nb.so)")
+
+
+def flags() -> list[str]:
+    return ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-munsafe-fp-atomics", "-Wno-unused-command-line-argument",
+            "-I", os.path.join(ROOT, "include"), "-I", CSRC]
+
+
+def build(verbose: bool = False, dry: bool = False) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    cc = hipcc()
+    objs = []
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+        cmd = [cc] + flags() + ["-c", path, "-o", obj]
+        if src.endswith(".cpp"):
+            cmd = [cc, "-x", "hip"] + flags() + ["-c", path, "-o", obj]
+        objs.append(obj)
+        if verbose or dry:
+            print(" ".join(cmd))
+        if not dry:
+            subprocess.run(cmd, check=True)
+    link = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", LIB + ".tmp"] + objs
+    if verbose or dry:
+        print(" ".join(link))
+    if not dry:
+        subprocess.run(link, check=True)
+        os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True, dry="--check" in sys.argv)

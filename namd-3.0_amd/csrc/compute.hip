@@ -1,0 +1,579 @@
+// The compute object behind the C ABI: the MI355X counterpart of NAMD's
+// CudaComputeNonbonded (src/CudaComputeNonbonded.{C,h}) together with the
+// buffer ownership of CudaComputeNonbondedKernel and CudaTileListKernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mi355nb.h"
+#include "devutil.h"
+#include "layout.h"
+#include "nb_kernels.h"
+
+using namespace mi355nb;
+
+namespace mi355nb {
+static thread_local std::string g_lastError;
+void setLastError(const std::string& msg) { g_lastError = msg; }
+}  // namespace mi355nb
+
+#define CHECK_ARG(cond, msg)                 \
+  do {                                       \
+    if (!(cond)) {                           \
+      setLastError(msg);                     \
+      return MI355NB_ERR_ARG;                \
+    }                                        \
+  } while (0)
+
+#define CHECK_STATE(cond, msg)               \
+  do {                                       \
+    if (!(cond)) {                           \
+      setLastError(msg);                     \
+      return MI355NB_ERR_STATE;              \
+    }                                        \
+  } while (0)
+
+struct mi355nb_compute {
+  mi355nb_nb_params p;
+  NbConstants c;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // molecule
+  int natoms = 0;
+  int ntypes = 0;
+  std::vector<int> vdwType;
+  std::vector<double> qScaled;
+  std::vector<I2> exclByAtom;
+  std::vector<uint32_t> exclBitsHost;
+  std::vector<float> vdwCoefHost;  // 2 * ntypes * ntypes
+  bool haveMolecule = false;
+  bool haveLayout = false;
+  bool haveLists = false;
+
+  PatchLayout L;
+  int numCandLists = 0;
+
+  // device buffers
+  DevBuf<F4> xyzq, force, forceSlow;
+  DevBuf<int> vdwTypes, atomIndex;
+  DevBuf<I2> excl;
+  DevBuf<uint32_t> bits;
+  DevBuf<float> vdwCoef;
+  DevBuf<PatchRecord> patches;
+  DevBuf<BoundingBox> bbox;
+  DevBuf<TileList> listsCand, listsAct;
+  DevBuf<int> jCand, jAct;
+  DevBuf<TileExcl> exCand, exAct;
+  DevBuf<int> counts, offsets, jtileFlags, listDepth;
+  DevBuf<int> sortKeys, sortKeysOut, sortVals, sortValsOut, newStart;
+  DevBuf<char> temp;
+  size_t tempBytes = 0;
+  DevBuf<float> fsoa;
+  DevBuf<TileListVirialEnergy> tlve;
+  DevBuf<double> partials;
+  DevBuf<VirialEnergy> ve;
+  DevBuf<TileListStat> stat;
+  DevBuf<int> numCand;
+
+  // host staging
+  std::vector<F4> hForce, hForceSlow;
+  bool lastSlow = false;
+
+  // optional kernel timing: event pairs around each pair-kernel launch
+  bool timing = false;
+  std::vector<hipEvent_t> evPool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> evPairlist, evForce;
+  hipEvent_t takeEvent() {
+    if (!evPool.empty()) {
+      hipEvent_t e = evPool.back();
+      evPool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  void recycleEvents() {
+    for (auto& p : evPairlist) { evPool.push_back(p.first); evPool.push_back(p.second); }
+    for (auto& p : evForce) { evPool.push_back(p.first); evPool.push_back(p.second); }
+    evPairlist.clear();
+    evForce.clear();
+  }
+
+  void releaseAll() {
+    xyzq.release(); force.release(); forceSlow.release();
+    vdwTypes.release(); atomIndex.release(); excl.release(); bits.release(); vdwCoef.release();
+    patches.release(); bbox.release(); listsCand.release(); listsAct.release();
+    jCand.release(); jAct.release(); exCand.release(); exAct.release();
+    counts.release(); offsets.release(); jtileFlags.release(); listDepth.release();
+    sortKeys.release(); sortKeysOut.release(); sortVals.release(); sortValsOut.release(); newStart.release();
+    temp.release(); fsoa.release(); tlve.release(); partials.release(); ve.release(); stat.release();
+    numCand.release();
+    recycleEvents();
+    for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
+    evPool.clear();
+  }
+};
+
+extern "C" {
+
+const char* mi355nb_last_error(void) { return g_lastError.c_str(); }
+
+int mi355nb_abi_version(void) { return MI355NB_ABI_VERSION; }
+
+double mi355nb_ewald_coefficient(double cutoff, double tolerance) {
+  if (!(cutoff > 0) || !(tolerance > 0)) return 0.0;
+  return ewaldCoefficient(cutoff, tolerance);
+}
+
+int mi355nb_get_nonbonded_coef(double cutoff, double switchdist, double ewaldcof, double slowScale,
+                               mi355nb_nb_constants* out) {
+  CHECK_ARG(out && cutoff > 0 && switchdist > 0 && switchdist <= cutoff, "bad nonbonded parameters");
+  NbConstants c = getNonbondedCoef(cutoff, switchdist, ewaldcof, slowScale);
+  static_assert(sizeof(NbConstants) == sizeof(mi355nb_nb_constants), "NbConstants layout");
+  std::memcpy(out, &c, sizeof(c));
+  return MI355NB_OK;
+}
+
+int mi355nb_get_do_table(int, int, int) { return 0; }
+
+int mi355nb_lj_mix(int ntypes, const double* sigma, const double* epsilon, int geometric, double* A,
+                   double* B) {
+  CHECK_ARG(ntypes > 0 && sigma && epsilon && A && B, "bad LJ arguments");
+  ljMix(ntypes, sigma, epsilon, geometric, A, B);
+  return MI355NB_OK;
+}
+
+int mi355nb_build_exclusions(int natoms, const int64_t* excl_ptr, const int32_t* excl_idx,
+                             int32_t* start_maxdiff, uint32_t* bitsOut, int64_t* nwords, int32_t* nlists) {
+  CHECK_ARG(natoms >= 0 && excl_ptr && (excl_idx || excl_ptr[natoms] == 0) && nwords, "bad exclusion arguments");
+  std::vector<I2> sm;
+  std::vector<uint32_t> b;
+  int nl = 0;
+  std::string err;
+  if (!buildExclusions(natoms, excl_ptr, excl_idx, sm, b, nl, err)) {
+    setLastError(err);
+    return MI355NB_ERR_CAPACITY;
+  }
+  if (bitsOut) {
+    CHECK_ARG(*nwords >= (int64_t)b.size(), "exclusion bit buffer too small");
+    std::memcpy(bitsOut, b.data(), b.size() * sizeof(uint32_t));
+  }
+  *nwords = (int64_t)b.size();
+  if (nlists) *nlists = nl;
+  if (start_maxdiff)
+    for (int i = 0; i < natoms; ++i) {
+      start_maxdiff[2 * i] = sm[i].x;
+      start_maxdiff[2 * i + 1] = sm[i].y;
+    }
+  return MI355NB_OK;
+}
+
+int mi355nb_tile_excl_to_namd(const uint32_t* natural, uint32_t* rotated, int64_t ntiles) {
+  CHECK_ARG(ntiles >= 0 && (ntiles == 0 || (natural && rotated)), "bad mask arguments");
+  for (int64_t t = 0; t < ntiles; ++t)
+    for (int i = 0; i < TILE; ++i) {
+      uint32_t m = natural[t * TILE + i], r = 0;
+      for (int k = 0; k < TILE; ++k)
+        if ((m >> ((i + k) & 31)) & 1u) r |= 1u << k;
+      rotated[t * TILE + i] = r;
+    }
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_create(const mi355nb_nb_params* params, int device, void* stream, mi355nb_compute** out) {
+  CHECK_ARG(params && out, "null argument");
+  CHECK_ARG(params->cutoff > 0 && params->switchdist > 0 && params->switchdist <= params->cutoff &&
+                params->pairlistdist >= params->cutoff && params->ewaldcof >= 0,
+            "inconsistent cutoff / switchdist / pairlistdist / ewaldcof");
+  int ndev = 0;
+  MI355NB_HIP(hipGetDeviceCount(&ndev));
+  CHECK_ARG(device >= 0 && device < ndev, "device index out of range");
+  MI355NB_HIP(hipSetDevice(device));
+  mi355nb_compute* c = new mi355nb_compute();
+  c->p = *params;
+  c->c = getNonbondedCoef(params->cutoff, params->switchdist, params->ewaldcof, params->slowScale);
+  c->device = device;
+  c->stream = (hipStream_t)stream;
+  *out = c;
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_destroy(mi355nb_compute* c) {
+  if (!c) return MI355NB_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream); else (void)hipDeviceSynchronize();
+  c->releaseAll();
+  delete c;
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_set_molecule(mi355nb_compute* c, int natoms, const int32_t* vdw_type, const double* charge,
+                                 int ntypes, const double* lj_A, const double* lj_B, const int64_t* excl_ptr,
+                                 const int32_t* excl_idx) {
+  CHECK_ARG(c && natoms > 0 && vdw_type && charge && ntypes > 0 && lj_A && lj_B && excl_ptr,
+            "bad molecule arguments");
+  for (int i = 0; i < natoms; ++i)
+    CHECK_ARG(vdw_type[i] >= 0 && vdw_type[i] < ntypes, "vdw type out of range");
+  for (int i = 0; i < natoms; ++i)
+    for (int64_t k = excl_ptr[i]; k < excl_ptr[i + 1]; ++k)
+      CHECK_ARG(excl_idx[k] >= 0 && excl_idx[k] < natoms && excl_idx[k] != i, "bad exclusion index");
+  std::string err;
+  int nl = 0;
+  if (!buildExclusions(natoms, excl_ptr, excl_idx, c->exclByAtom, c->exclBitsHost, nl, err)) {
+    setLastError(err);
+    return MI355NB_ERR_CAPACITY;
+  }
+  c->natoms = natoms;
+  c->ntypes = ntypes;
+  c->vdwType.assign(vdw_type, vdw_type + natoms);
+  c->qScaled.resize(natoms);
+  const double qs = std::sqrt(COULOMB);  // nonbondedScaling = 1, dielectric = 1
+  for (int i = 0; i < natoms; ++i) c->qScaled[i] = charge[i] * qs;
+  // CudaNonbondedTables::buildVdwCoefTable: float2 (A*scaling, B*scaling)
+  c->vdwCoefHost.resize(2 * (size_t)ntypes * ntypes);
+  for (int k = 0; k < ntypes * ntypes; ++k) {
+    c->vdwCoefHost[2 * k] = (float)lj_A[k];
+    c->vdwCoefHost[2 * k + 1] = (float)lj_B[k];
+  }
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(c->bits.reserve(std::max<size_t>(1, c->exclBitsHost.size()), 1.0f));
+  MI355NB_HIP(hipMemcpy(c->bits.ptr, c->exclBitsHost.data(), c->exclBitsHost.size() * sizeof(uint32_t),
+                        hipMemcpyHostToDevice));
+  MI355NB_HIP(c->vdwCoef.reserve(c->vdwCoefHost.size(), 1.0f));
+  MI355NB_HIP(hipMemcpy(c->vdwCoef.ptr, c->vdwCoefHost.data(), c->vdwCoefHost.size() * sizeof(float),
+                        hipMemcpyHostToDevice));
+  c->haveMolecule = true;
+  c->haveLayout = false;
+  c->haveLists = false;
+  return MI355NB_OK;
+}
+
+static int uploadSlots(mi355nb_compute* c) {
+  const int n = c->L.atomStorageSize;
+  MI355NB_HIP(hipMemcpyAsync(c->xyzq.ptr, c->L.xyzq.data(), n * sizeof(F4), hipMemcpyHostToDevice, c->stream));
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], const double* pos) {
+  CHECK_ARG(c && box && pos, "null argument");
+  CHECK_STATE(c->haveMolecule, "set_molecule must precede assign_patches");
+  for (int d = 0; d < 3; ++d)
+    CHECK_ARG(box[d] >= 2.0 * c->p.pairlistdist, "box edge must be at least twice pairlistdist");
+  for (int i = 0; i < 3 * c->natoms; ++i) CHECK_ARG(std::isfinite(pos[i]), "non-finite position");
+  std::string err;
+  if (!c->L.build(c->natoms, pos, box, c->p.pairlistdist + c->p.margin, err)) {
+    setLastError(err);
+    return MI355NB_ERR_ARG;
+  }
+  c->L.pack(pos, c->qScaled.data(), c->vdwType.data(), c->exclByAtom.data());
+  const int n = c->L.atomStorageSize;
+  const int nl = (int)c->L.tileListSkeleton.size();
+  const long long cap = std::max<long long>(1, c->L.jtileCapacity);
+  c->numCandLists = nl;
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(c->xyzq.reserve(n));
+  MI355NB_HIP(c->force.reserve(n));
+  MI355NB_HIP(c->forceSlow.reserve(n));
+  MI355NB_HIP(c->vdwTypes.reserve(n));
+  MI355NB_HIP(c->atomIndex.reserve(n));
+  MI355NB_HIP(c->excl.reserve(n));
+  MI355NB_HIP(c->patches.reserve(c->L.patches.size()));
+  MI355NB_HIP(c->bbox.reserve(n / TILE + 1));
+  MI355NB_HIP(c->listsCand.reserve(nl + 1));
+  MI355NB_HIP(c->listsAct.reserve(nl + 1));
+  MI355NB_HIP(c->jCand.reserve((size_t)cap));
+  MI355NB_HIP(c->jAct.reserve((size_t)cap));
+  MI355NB_HIP(c->exCand.reserve((size_t)cap));
+  MI355NB_HIP(c->exAct.reserve((size_t)cap));
+  MI355NB_HIP(c->jtileFlags.reserve((size_t)cap));
+  MI355NB_HIP(c->counts.reserve(nl + 1));
+  MI355NB_HIP(c->offsets.reserve(nl + 1));
+  MI355NB_HIP(c->listDepth.reserve(nl + 1));
+  MI355NB_HIP(c->sortKeys.reserve(nl + 1));
+  MI355NB_HIP(c->sortKeysOut.reserve(nl + 1));
+  MI355NB_HIP(c->sortVals.reserve(nl + 1));
+  MI355NB_HIP(c->sortValsOut.reserve(nl + 1));
+  MI355NB_HIP(c->newStart.reserve(nl + 1));
+  MI355NB_HIP(c->tlve.reserve(nl + 1));
+  MI355NB_HIP(c->partials.reserve((size_t)reducePartialBlocks(n, nl) * 21 + 21));
+  MI355NB_HIP(c->ve.reserve(1));
+  MI355NB_HIP(c->stat.reserve(1));
+  MI355NB_HIP(c->numCand.reserve(1));
+  const size_t fsoaOld = c->fsoa.cap;
+  MI355NB_HIP(c->fsoa.reserve(6 * (size_t)n));
+  // scratch for the scan and the radix sort
+  size_t b1 = 0, b2 = 0;
+  MI355NB_HIP(scanExclusive(nullptr, b1, c->counts.ptr, c->offsets.ptr, nl, c->stream));
+  MI355NB_HIP(sortDescending(nullptr, b2, c->sortKeys.ptr, c->sortKeysOut.ptr, c->sortVals.ptr,
+                             c->sortValsOut.ptr, nl, c->stream));
+  c->tempBytes = std::max(b1, b2);
+  MI355NB_HIP(c->temp.reserve(c->tempBytes + 16));
+
+  MI355NB_HIP(hipMemcpyAsync(c->vdwTypes.ptr, c->L.vdwTypes.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  MI355NB_HIP(hipMemcpyAsync(c->atomIndex.ptr, c->L.atomIndex.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  MI355NB_HIP(hipMemcpyAsync(c->excl.ptr, c->L.exclStartMaxdiff.data(), n * sizeof(I2), hipMemcpyHostToDevice, c->stream));
+  MI355NB_HIP(hipMemcpyAsync(c->patches.ptr, c->L.patches.data(), c->L.patches.size() * sizeof(PatchRecord),
+                             hipMemcpyHostToDevice, c->stream));
+  MI355NB_HIP(hipMemcpyAsync(c->listsCand.ptr, c->L.tileListSkeleton.data(), nl * sizeof(TileList),
+                             hipMemcpyHostToDevice, c->stream));
+  MI355NB_HIP(hipMemcpyAsync(c->numCand.ptr, &c->numCandLists, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  MI355NB_HIP(launchFill(c->fsoa.cap, c->fsoa.ptr, 0.0f, c->stream));
+  (void)fsoaOld;
+  int rc = uploadSlots(c);
+  if (rc) return rc;
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  c->haveLayout = true;
+  c->haveLists = false;
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_update_positions(mi355nb_compute* c, const double* pos) {
+  CHECK_ARG(c && pos, "null argument");
+  CHECK_STATE(c->haveLayout, "assign_patches must precede update_positions");
+  MI355NB_HIP(hipSetDevice(c->device));
+  // the host staging buffer may still be read by a previous async copy
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  c->L.packPositions(pos);
+  return uploadSlots(c);
+}
+
+int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow, int doPairlist) {
+  CHECK_ARG(c, "null compute");
+  CHECK_STATE(c->haveLayout, "assign_patches must precede launch");
+  CHECK_STATE(doPairlist || c->haveLists, "the first launch after assign_patches must build pairlists");
+  MI355NB_HIP(hipSetDevice(c->device));
+  const int n = c->L.atomStorageSize;
+  const int nl = c->numCandLists;
+  const float cutoff2 = (float)(c->p.cutoff * c->p.cutoff);
+  const float plcutoff2 = (float)(c->p.pairlistdist * c->p.pairlistdist);
+  hipStream_t s = c->stream;
+
+  ForceArgsHost a;
+  a.xyzq = c->xyzq.ptr;
+  a.vdwTypes = c->vdwTypes.ptr;
+  a.vdwWidth = c->ntypes;
+  a.vdwCoef = c->vdwCoef.ptr;
+  a.c = c->c;
+  a.cutoff2 = cutoff2;
+  a.plcutoff2 = plcutoff2;
+  a.patches = c->patches.ptr;
+  a.atomIndex = c->atomIndex.ptr;
+  a.exclStartMaxdiff = c->excl.ptr;
+  a.exclBits = c->bits.ptr;
+  a.jtileFlags = c->jtileFlags.ptr;
+  a.listDepth = c->listDepth.ptr;
+  a.stat = c->stat.ptr;
+  a.fsoa = c->fsoa.ptr;
+  a.stride = n;
+  a.virialEnergy = c->tlve.ptr;
+
+  const int* numListsForReduce;
+  if (doPairlist) {
+    const float cull = (float)(c->p.pairlistdist + 0.01);
+    MI355NB_HIP(launchBoundingBoxes(n / TILE, c->xyzq.ptr, c->bbox.ptr, s));
+    MI355NB_HIP(launchBuildJtiles(0, nl, c->listsCand.ptr, c->patches.ptr, c->bbox.ptr, cull * cull,
+                                  c->counts.ptr, nullptr, nullptr, s));
+    size_t tb = c->tempBytes;
+    MI355NB_HIP(scanExclusive(c->temp.ptr, tb, c->counts.ptr, c->offsets.ptr, nl, s));
+    MI355NB_HIP(launchBuildJtiles(1, nl, c->listsCand.ptr, c->patches.ptr, c->bbox.ptr, cull * cull,
+                                  c->counts.ptr, c->offsets.ptr, c->jCand.ptr, s));
+    MI355NB_HIP(launchClearStat(c->stat.ptr, s));
+    a.numLists = c->numCand.ptr;
+    a.tileLists = c->listsCand.ptr;
+    a.tileJatomStart = c->jCand.ptr;
+    a.tileExcls = c->exCand.ptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+      e0 = c->takeEvent(); e1 = c->takeEvent();
+      if (e0 && e1) MI355NB_HIP(hipEventRecord(e0, s));
+    }
+    MI355NB_HIP(launchNonbondedForce(nl, a, doEnergy, doVirial, doSlow, true, s));
+    if (c->timing && e0 && e1) {
+      MI355NB_HIP(hipEventRecord(e1, s));
+      c->evPairlist.emplace_back(e0, e1);
+    }
+    // refine: depth-ordered, compacted lists
+    MI355NB_HIP(launchSetupSort(nl, c->listDepth.ptr, c->sortKeys.ptr, c->sortVals.ptr, s));
+    tb = c->tempBytes;
+    MI355NB_HIP(sortDescending(c->temp.ptr, tb, c->sortKeys.ptr, c->sortKeysOut.ptr, c->sortVals.ptr,
+                               c->sortValsOut.ptr, nl, s));
+    tb = c->tempBytes;
+    MI355NB_HIP(scanExclusive(c->temp.ptr, tb, c->sortKeysOut.ptr, c->newStart.ptr, nl, s));
+    MI355NB_HIP(launchRepack(nl, &c->stat.ptr->numTileLists, c->sortValsOut.ptr, c->newStart.ptr,
+                             c->listsCand.ptr, c->jCand.ptr, c->exCand.ptr, c->jtileFlags.ptr, c->listsAct.ptr,
+                             c->jAct.ptr, c->exAct.ptr, s));
+    numListsForReduce = c->numCand.ptr;
+    c->haveLists = true;
+  } else {
+    a.numLists = &c->stat.ptr->numTileLists;
+    a.tileLists = c->listsAct.ptr;
+    a.tileJatomStart = c->jAct.ptr;
+    a.tileExcls = c->exAct.ptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+      e0 = c->takeEvent(); e1 = c->takeEvent();
+      if (e0 && e1) MI355NB_HIP(hipEventRecord(e0, s));
+    }
+    MI355NB_HIP(launchNonbondedForce(nl, a, doEnergy, doVirial, doSlow, false, s));
+    if (c->timing && e0 && e1) {
+      MI355NB_HIP(hipEventRecord(e1, s));
+      c->evForce.emplace_back(e0, e1);
+    }
+    numListsForReduce = &c->stat.ptr->numTileLists;
+  }
+  MI355NB_HIP(launchTransposeForces(n, n, c->fsoa.ptr, c->force.ptr, c->forceSlow.ptr, doSlow != 0, s));
+  if (doEnergy || doVirial) {
+    MI355NB_HIP(launchReduceVirialEnergy(n, nl, c->xyzq.ptr, c->force.ptr, c->forceSlow.ptr, numListsForReduce,
+                                         c->tlve.ptr, doEnergy != 0, doVirial != 0, doSlow != 0,
+                                         c->partials.ptr, c->ve.ptr, s));
+  }
+  c->lastSlow = doSlow != 0;
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_finish(mi355nb_compute* c, double* forceOut, double* forceSlowOut, mi355nb_virial_energy* veOut,
+                           mi355nb_tile_list_stat* statOut) {
+  CHECK_ARG(c, "null compute");
+  CHECK_STATE(c->haveLayout, "nothing launched");
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  const int n = c->L.atomStorageSize;
+  if (forceOut) {
+    c->hForce.resize(n);
+    MI355NB_HIP(hipMemcpy(c->hForce.data(), c->force.ptr, n * sizeof(F4), hipMemcpyDeviceToHost));
+    for (int a = 0; a < c->natoms; ++a) {
+      const F4 f = c->hForce[c->L.slotOfAtom[a]];
+      forceOut[3 * a] = f.x; forceOut[3 * a + 1] = f.y; forceOut[3 * a + 2] = f.z;
+    }
+  }
+  if (forceSlowOut) {
+    c->hForceSlow.resize(n);
+    MI355NB_HIP(hipMemcpy(c->hForceSlow.data(), c->forceSlow.ptr, n * sizeof(F4), hipMemcpyDeviceToHost));
+    for (int a = 0; a < c->natoms; ++a) {
+      const F4 f = c->hForceSlow[c->L.slotOfAtom[a]];
+      forceSlowOut[3 * a] = f.x; forceSlowOut[3 * a + 1] = f.y; forceSlowOut[3 * a + 2] = f.z;
+    }
+  }
+  if (veOut) {
+    static_assert(sizeof(VirialEnergy) == sizeof(mi355nb_virial_energy), "VirialEnergy layout");
+    MI355NB_HIP(hipMemcpy(veOut, c->ve.ptr, sizeof(VirialEnergy), hipMemcpyDeviceToHost));
+  }
+  if (statOut) {
+    static_assert(sizeof(TileListStat) == sizeof(mi355nb_tile_list_stat), "TileListStat layout");
+    MI355NB_HIP(hipMemcpy(statOut, c->stat.ptr, sizeof(TileListStat), hipMemcpyDeviceToHost));
+  }
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_set_timing(mi355nb_compute* c, int enable) {
+  CHECK_ARG(c, "null compute");
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  c->recycleEvents();
+  c->timing = enable != 0;
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_get_kernel_time(mi355nb_compute* c, double* msPairlist, int32_t* nPairlist, double* msForce,
+                                    int32_t* nForce) {
+  CHECK_ARG(c, "null compute");
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  double a = 0.0, b = 0.0;
+  for (auto& p : c->evPairlist) {
+    float ms = 0.f;
+    MI355NB_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+    a += ms;
+  }
+  for (auto& p : c->evForce) {
+    float ms = 0.f;
+    MI355NB_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+    b += ms;
+  }
+  if (msPairlist) *msPairlist = a;
+  if (nPairlist) *nPairlist = (int32_t)c->evPairlist.size();
+  if (msForce) *msForce = b;
+  if (nForce) *nForce = (int32_t)c->evForce.size();
+  c->recycleEvents();
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_get_info(const mi355nb_compute* c, mi355nb_layout_info* info) {
+  CHECK_ARG(c && info, "null argument");
+  std::memset(info, 0, sizeof(*info));
+  info->natoms = c->natoms;
+  if (!c->haveLayout) return MI355NB_OK;
+  info->atomStorageSize = c->L.atomStorageSize;
+  info->numPatches = (int)c->L.patches.size();
+  info->numComputes = (int)c->L.computes.size();
+  info->numCandidateLists = c->numCandLists;
+  for (int d = 0; d < 3; ++d) {
+    info->patchDims[d] = c->L.dims[d];
+    info->box[d] = c->L.box[d];
+  }
+  info->jtileCapacity = c->L.jtileCapacity;
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_get_device_buffers(const mi355nb_compute* c, mi355nb_device_buffers* out) {
+  CHECK_ARG(c && out, "null argument");
+  CHECK_STATE(c->haveLayout, "no layout yet");
+  out->xyzq = reinterpret_cast<float*>(c->xyzq.ptr);
+  out->force = reinterpret_cast<float*>(c->force.ptr);
+  out->forceSlow = reinterpret_cast<float*>(c->forceSlow.ptr);
+  out->atomIndex = c->atomIndex.ptr;
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_get_nonbonded_coef(const mi355nb_compute* c, mi355nb_nb_constants* out) {
+  CHECK_ARG(c && out, "null argument");
+  std::memcpy(out, &c->c, sizeof(c->c));
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_get_effective_positions(const mi355nb_compute* c, double* pos) {
+  CHECK_ARG(c && pos, "null argument");
+  CHECK_STATE(c->haveLayout, "no layout yet");
+  std::memcpy(pos, c->L.effPos.data(), c->L.effPos.size() * sizeof(double));
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_get_tile_lists(mi355nb_compute* c, int32_t* numLists, int32_t* numJtiles, int32_t* iatomStart,
+                                   int32_t* jtileStart, int32_t* jtileEnd, float* shift, int32_t* jatomStart,
+                                   uint32_t* exclOut, int32_t* slotAtomIndex) {
+  CHECK_ARG(c && numLists && numJtiles, "null argument");
+  CHECK_STATE(c->haveLists, "no pairlist has been built");
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  TileListStat st;
+  MI355NB_HIP(hipMemcpy(&st, c->stat.ptr, sizeof(st), hipMemcpyDeviceToHost));
+  const int nl = st.numTileLists;
+  std::vector<TileList> tl(std::max(nl, 1));
+  MI355NB_HIP(hipMemcpy(tl.data(), c->listsAct.ptr, nl * sizeof(TileList), hipMemcpyDeviceToHost));
+  int nj = 0;
+  for (int k = 0; k < nl; ++k) nj = std::max(nj, tl[k].jtileEnd + 1);
+  const bool query = !iatomStart;
+  if (!query) {
+    CHECK_ARG(*numLists >= nl && *numJtiles >= nj, "output arrays too small");
+    for (int k = 0; k < nl; ++k) {
+      iatomStart[k] = tl[k].iatomStart;
+      if (jtileStart) jtileStart[k] = tl[k].jtileStart;
+      if (jtileEnd) jtileEnd[k] = tl[k].jtileEnd;
+      if (shift) { shift[3 * k] = tl[k].offx; shift[3 * k + 1] = tl[k].offy; shift[3 * k + 2] = tl[k].offz; }
+    }
+    if (jatomStart) MI355NB_HIP(hipMemcpy(jatomStart, c->jAct.ptr, nj * sizeof(int), hipMemcpyDeviceToHost));
+    if (exclOut) MI355NB_HIP(hipMemcpy(exclOut, c->exAct.ptr, (size_t)nj * sizeof(TileExcl), hipMemcpyDeviceToHost));
+    if (slotAtomIndex) std::memcpy(slotAtomIndex, c->L.atomIndex.data(), c->L.atomIndex.size() * sizeof(int));
+  }
+  *numLists = nl;
+  *numJtiles = nj;
+  return MI355NB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,42 @@
+// Small HIP helpers shared by the .hip translation units: status handling and
+// geometrically grown device buffers (src/CudaUtils.h:214-400 reallocate_device
+// grows by a factor and never shrinks; the same policy is used here).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <string>
+
+namespace mi355nb {
+
+void setLastError(const std::string& msg);
+
+#define MI355NB_HIP(call)                                                     \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::mi355nb::setLastError(std::string(#call) + ": " + hipGetErrorString(e_)); \
+      return -(int)e_ - 1000;                                                 \
+    }                                                                         \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t cap = 0;   // elements
+  // Ensure capacity for n elements; returns hipSuccess or an error.
+  hipError_t reserve(size_t n, float factor = 1.2f) {
+    if (n <= cap && ptr) return hipSuccess;
+    if (ptr) { (void)hipFree(ptr); ptr = nullptr; cap = 0; }
+    size_t want = (size_t)((double)n * factor) + 1;
+    hipError_t e = hipMalloc((void**)&ptr, want * sizeof(T));
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
+};
+
+}  // namespace mi355nb

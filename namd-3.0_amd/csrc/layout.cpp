@@ -1,0 +1,329 @@
+// Host-side layout for the nonbonded path (see layout.h).
+#include "layout.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+namespace mi355nb {
+
+double quantize(double x, double q) { return std::nearbyint(x / q) * q; }
+
+double ewaldCoefficient(double cutoff, double tol) {
+  double b = 1.0;
+  while (std::erfc(b * cutoff) / cutoff >= tol) b *= 2.0;
+  double lo = 0.0, hi = b;
+  for (int i = 0; i < 100; ++i) {
+    b = 0.5 * (lo + hi);
+    if (std::erfc(b * cutoff) / cutoff >= tol) lo = b; else hi = b;
+  }
+  return b;
+}
+
+NbConstants getNonbondedCoef(double cutoffD, double switchD, double ewaldcof, double slowScaleD) {
+  // ComputeNonbondedUtil::select computes these in double; getNonbondedCoef
+  // then evaluates the products in float.
+  const double cutoff2D = cutoffD * cutoffD;
+  const double switch2D = switchD * switchD;
+  const double c0 = (switchD < cutoffD) ? 1.0 / (cutoff2D - switch2D) : 0.0;
+  const double c1 = c0 * c0 * c0;
+  const double pi_ewaldcof = 1.12837916709551 * ewaldcof;  // ComputeNonbondedUtil.C:570-573
+
+  const float cutoff = (float)cutoffD;
+  const float cutoff2 = (float)cutoff2D;
+  const float cutoffInv = 1.0f / cutoff;
+  const float cutoff2Inv = 1.0f / cutoff2;
+  const float scutoff2 = (float)switch2D;
+  const float denom = (float)c1;
+  const float slowScale = (float)slowScaleD;
+  NbConstants c;
+  c.lj_0 = denom * cutoff2 - 3.0f * scutoff2 * denom;
+  c.lj_1 = denom * 2.0f;
+  c.lj_2 = denom * -12.0f;
+  c.lj_3 = 12.0f * denom * scutoff2;
+  c.lj_4 = cutoff2;
+  c.lj_5 = scutoff2;
+  c.e_0 = cutoff2Inv * cutoffInv;
+  c.e_0_slow = cutoff2Inv * cutoffInv * (1.0f - slowScale);
+  c.e_1 = cutoff2Inv;
+  c.e_2 = cutoffInv;
+  c.ewald_0 = (float)ewaldcof;
+  c.ewald_1 = (float)pi_ewaldcof;
+  c.ewald_2 = (float)ewaldcof * (float)ewaldcof;
+  c.ewald_3_slow = (float)ewaldcof * (float)ewaldcof * (float)ewaldcof * slowScale;
+  c.slowScale = slowScale;
+  return c;
+}
+
+void ljMix(int ntypes, const double* sigma, const double* eps, int geometric, double* A, double* B) {
+  for (int i = 0; i < ntypes; ++i) {
+    for (int j = 0; j < ntypes; ++j) {
+      double s = geometric ? std::sqrt(sigma[i] * sigma[j]) : 0.5 * (sigma[i] + sigma[j]);
+      double e = std::sqrt(eps[i] * eps[j]);
+      double s6 = s * s * s;
+      s6 *= s6;
+      double b = 4.0 * s6 * e;
+      B[i * ntypes + j] = b;
+      A[i * ntypes + j] = b * s6;
+    }
+  }
+}
+
+bool buildExclusions(int natoms, const int64_t* exclPtr, const int* exclIdx,
+                     std::vector<I2>& startMaxdiff, std::vector<uint32_t>& bits,
+                     int& numLists, std::string& err) {
+  // unique lists keyed by their sorted offsets (offset 0 = self always present)
+  std::map<std::vector<int>, int> uniq;
+  std::vector<std::vector<int>> lists;
+  std::vector<int> listOfAtom(natoms);
+  std::vector<int> cur;
+  for (int i = 0; i < natoms; ++i) {
+    cur.clear();
+    cur.push_back(0);
+    for (int64_t k = exclPtr[i]; k < exclPtr[i + 1]; ++k) cur.push_back(exclIdx[k] - i);
+    std::sort(cur.begin(), cur.end());
+    cur.erase(std::unique(cur.begin(), cur.end()), cur.end());
+    auto it = uniq.find(cur);
+    if (it == uniq.end()) {
+      int id = (int)lists.size();
+      uniq.emplace(cur, id);
+      lists.push_back(cur);
+      listOfAtom[i] = id;
+    } else {
+      listOfAtom[i] = it->second;
+    }
+  }
+  // maxdiff per list; stable order by maxdiff (exlist_sortop)
+  const int nl = (int)lists.size();
+  std::vector<int> maxdiff(nl);
+  for (int l = 0; l < nl; ++l) {
+    int md = -lists[l].front();
+    if (lists[l].back() > md) md = lists[l].back();
+    maxdiff[l] = md;
+  }
+  std::vector<int> ord(nl);
+  for (int l = 0; l < nl; ++l) ord[l] = l;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return maxdiff[a] < maxdiff[b]; });
+  std::vector<long long> start(nl);
+  long long total = 0;
+  for (int k = 0; k < nl; ++k) {
+    int l = ord[k];
+    start[l] = total + maxdiff[l];
+    total += 2LL * maxdiff[l] + 1;
+  }
+  if (total & 31) total += 32 - (total & 31);
+  if (total / 32 > (1LL << 27)) {
+    err = "exclusion bit table needs more than 2^27 words";
+    return false;
+  }
+  bits.assign((size_t)(total / 32), 0u);
+  for (int l = 0; l < nl; ++l) {
+    for (int off : lists[l]) {
+      long long b = start[l] + off;
+      bits[(size_t)(b >> 5)] |= 1u << (b & 31);
+    }
+  }
+  startMaxdiff.resize(natoms);
+  for (int i = 0; i < natoms; ++i) {
+    int l = listOfAtom[i];
+    startMaxdiff[i] = I2{(int)start[l], maxdiff[l]};
+  }
+  numLists = nl;
+  return true;
+}
+
+void orbPartition(int* order, const double* pos, int begin, int end) {
+  if (!(begin / TILE < (end - 1) / TILE)) return;
+  int split = ((begin + end + TILE) / (TILE * 2)) * TILE;
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) lo[d] = hi[d] = pos[3 * order[begin] + d];
+  for (int i = begin + 1; i < end; ++i)
+    for (int d = 0; d < 3; ++d) {
+      double v = pos[3 * order[i] + d];
+      lo[d] = std::min(lo[d], v);
+      hi[d] = std::max(hi[d], v);
+    }
+  double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+  int ax = (ex >= ey && ex >= ez) ? 0 : ((ey >= ex && ey >= ez) ? 1 : 2);
+  std::nth_element(order + begin, order + split, order + end, [&](int a, int b) {
+    double va = pos[3 * a + ax], vb = pos[3 * b + ax];
+    return va < vb || (va == vb && a < b);
+  });
+  orbPartition(order, pos, begin, split);
+  orbPartition(order, pos, split, end);
+}
+
+static inline double wrap(double x, double L) {
+  double w = x - L * std::floor(x / L);
+  if (w >= L) w -= L;
+  return w;
+}
+
+bool PatchLayout::build(int n, const double* pos, const double boxIn[3], double patchDim,
+                        std::string& err) {
+  natoms = n;
+  for (int d = 0; d < 3; ++d) {
+    box[d] = quantize(boxIn[d], POS_QUANTUM);
+    if (!(box[d] > 0)) { err = "box edge must be positive"; return false; }
+    dims[d] = std::max(1, (int)std::floor(box[d] / patchDim));
+    patchSize[d] = box[d] / dims[d];
+    if (patchSize[d] > 40.0) { err = "patch edge above 40 A (coordinates would leave the exact FP32 range)"; return false; }
+  }
+  const int np = dims[0] * dims[1] * dims[2];
+  centre.assign(3 * (size_t)np, 0.0);
+  for (int gx = 0; gx < dims[0]; ++gx)
+    for (int gy = 0; gy < dims[1]; ++gy)
+      for (int gz = 0; gz < dims[2]; ++gz) {
+        int p = (gx * dims[1] + gy) * dims[2] + gz;
+        int g[3] = {gx, gy, gz};
+        for (int d = 0; d < 3; ++d) centre[3 * p + d] = quantize((g[d] + 0.5) * patchSize[d], CENTRE_QUANTUM);
+      }
+
+  // assign atoms
+  patchOfAtom.assign(n, 0);
+  std::vector<std::vector<int>> members(np);
+  std::vector<double> wpos(3 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    int g[3];
+    for (int d = 0; d < 3; ++d) {
+      double w = wrap(pos[3 * i + d], box[d]);
+      wpos[3 * i + d] = w;
+      g[d] = std::min(dims[d] - 1, std::max(0, (int)std::floor(w / patchSize[d])));
+    }
+    int p = (g[0] * dims[1] + g[1]) * dims[2] + g[2];
+    patchOfAtom[i] = p;
+    members[p].push_back(i);
+  }
+
+  // per-patch ORB order and padded storage
+  patches.resize(np);
+  slotOfAtom.assign(n, -1);
+  atomIndex.clear();
+  int slot = 0;
+  for (int p = 0; p < np; ++p) {
+    std::vector<int>& m = members[p];
+    if (!m.empty()) orbPartition(m.data(), wpos.data(), 0, (int)m.size());
+    patches[p].numAtoms = (int)m.size();
+    patches[p].numFreeAtoms = (int)m.size();
+    patches[p].atomStart = slot;
+    patches[p].patchID = p;
+    int padded = ((int)m.size() + TILE - 1) / TILE * TILE;
+    for (int k = 0; k < padded; ++k) {
+      int a = m.empty() ? -1 : m[std::min(k, (int)m.size() - 1)];
+      if (k < (int)m.size()) slotOfAtom[a] = slot + k;
+      atomIndex.push_back(a);
+    }
+    slot += padded;
+  }
+  atomStorageSize = slot;
+
+  // computes: self + 13 half-shell neighbours (the lexicographically positive
+  // direction of every +-d pair), with the image vector T of the neighbour.
+  computes.clear();
+  for (int gx = 0; gx < dims[0]; ++gx)
+    for (int gy = 0; gy < dims[1]; ++gy)
+      for (int gz = 0; gz < dims[2]; ++gz) {
+        int pa = (gx * dims[1] + gy) * dims[2] + gz;
+        for (int dx = -1; dx <= 1; ++dx)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+              int code = (dx * 3 + dy) * 3 + dz;
+              if (code < 0) continue;  // keep 0 (self) and the 13 positive directions
+              int g1[3] = {gx + dx, gy + dy, gz + dz};
+              int g2[3];
+              double T[3];
+              for (int d = 0; d < 3; ++d) {
+                int q = (int)std::floor((double)g1[d] / dims[d]);
+                g2[d] = g1[d] - q * dims[d];
+                T[d] = q * box[d];
+              }
+              int pb = (g2[0] * dims[1] + g2[1]) * dims[2] + g2[2];
+              ComputeRecord c;
+              c.patchInd[0] = pa;
+              c.patchInd[1] = pb;
+              // shift added to patch A atoms: c_A - c_B - T (exact in FP32)
+              c.offx = (float)quantize(centre[3 * pa + 0] - centre[3 * pb + 0] - T[0], CENTRE_QUANTUM);
+              c.offy = (float)quantize(centre[3 * pa + 1] - centre[3 * pb + 1] - T[1], CENTRE_QUANTUM);
+              c.offz = (float)quantize(centre[3 * pa + 2] - centre[3 * pb + 2] - T[2], CENTRE_QUANTUM);
+              c.self = (code == 0) ? 1 : 0;
+              computes.push_back(c);
+            }
+      }
+
+  // tile-list skeleton and worst-case j-tile capacity
+  tileListSkeleton.clear();
+  jtileCapacity = 0;
+  for (int ci = 0; ci < (int)computes.size(); ++ci) {
+    const ComputeRecord& c = computes[ci];
+    const PatchRecord& A = patches[c.patchInd[0]];
+    const PatchRecord& B = patches[c.patchInd[1]];
+    int na = (A.numAtoms + TILE - 1) / TILE;
+    int nb = (B.numAtoms + TILE - 1) / TILE;
+    for (int t = 0; t < na; ++t) {
+      TileList tl;
+      tl.iatomStart = A.atomStart + t * TILE;
+      tl.jtileStart = 0;
+      tl.jtileEnd = -1;
+      tl.offx = c.offx;
+      tl.offy = c.offy;
+      tl.offz = c.offz;
+      tl.patchInd[0] = c.patchInd[0];
+      tl.patchInd[1] = c.patchInd[1];
+      tl.icompute = ci;
+      tl.self = c.self;
+      tileListSkeleton.push_back(tl);
+      jtileCapacity += c.self ? (nb - t) : nb;
+    }
+  }
+
+  xyzq.assign(atomStorageSize, F4{0, 0, 0, 0});
+  vdwTypes.assign(atomStorageSize, 0);
+  exclStartMaxdiff.assign(atomStorageSize, I2{0, 0});
+  effPos.assign(3 * (size_t)n, 0.0);
+  return true;
+}
+
+void PatchLayout::packPositions(const double* pos) {
+  const int np = (int)patches.size();
+  for (int p = 0; p < np; ++p) {
+    const PatchRecord& pr = patches[p];
+    int padded = (pr.numAtoms + TILE - 1) / TILE * TILE;
+    for (int k = 0; k < padded; ++k) {
+      int s = pr.atomStart + k;
+      int a = atomIndex[s];
+      if (a < 0) continue;
+      double q3[3];
+      for (int d = 0; d < 3; ++d) {
+        // image of the atom closest to its patch centre
+        double x = pos[3 * a + d] - centre[3 * p + d];
+        x -= box[d] * std::nearbyint(x / box[d]);
+        q3[d] = quantize(x, POS_QUANTUM);
+      }
+      xyzq[s].x = (float)q3[0];
+      xyzq[s].y = (float)q3[1];
+      xyzq[s].z = (float)q3[2];
+      if (k < pr.numAtoms)
+        for (int d = 0; d < 3; ++d) effPos[3 * (size_t)a + d] = centre[3 * p + d] + q3[d];
+    }
+  }
+}
+
+void PatchLayout::pack(const double* pos, const double* chargeScaled, const int* vdwType,
+                       const I2* exclByAtom) {
+  for (int s = 0; s < atomStorageSize; ++s) {
+    int a = atomIndex[s];
+    if (a < 0) {
+      xyzq[s] = F4{0, 0, 0, 0};
+      vdwTypes[s] = 0;
+      exclStartMaxdiff[s] = I2{0, 0};
+      continue;
+    }
+    xyzq[s].w = (float)chargeScaled[a];
+    vdwTypes[s] = vdwType[a];
+    exclStartMaxdiff[s] = exclByAtom[a];
+  }
+  packPositions(pos);
+}
+
+}  // namespace mi355nb

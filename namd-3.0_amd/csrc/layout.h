@@ -1,0 +1,100 @@
+// Host-side data layout for the nonbonded path: constants, LJ table,
+// exclusion bit table, patch grid, per-patch ORB atom order, padded atom
+// storage and the tile-list skeleton.  Pure C++ (no HIP), so it is testable
+// on a machine without a GPU.
+#pragma once
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "nbtypes.h"
+
+namespace mi355nb {
+
+// Patch-centred coordinates are rounded to multiples of POS_QUANTUM and patch
+// centres / image shifts to multiples of CENTRE_QUANTUM.  With |x| < 128 A
+// every sum and difference of such values is exact in FP32, so the kernel's
+// displacement (x_j - (x_i + shift)) is the exact displacement and the pair
+// membership test depends only on the rounding of r^2 itself.
+constexpr double POS_QUANTUM = 1.0 / 131072.0;     // 2^-17 A
+constexpr double CENTRE_QUANTUM = 1.0 / 1024.0;    // 2^-10 A
+constexpr double COULOMB = 332.0636;               // src/common.h:53
+
+double quantize(double x, double q);
+
+// PMEEwaldCoefficient bisection (src/SimParameters.C:4243-4256).
+double ewaldCoefficient(double cutoff, double tol);
+
+// CudaComputeNonbonded::getNonbondedCoef (src/CudaComputeNonbonded.C:2318-2349)
+// with the switching constants of ComputeNonbondedUtil::select
+// (src/ComputeNonbondedUtil.C:513-567): c1 = 1/(rc^2-rs^2)^3.
+NbConstants getNonbondedCoef(double cutoff, double switchdist, double ewaldcof,
+                             double slowScale);
+
+// LJTable::compute_vdw_params mixing (src/LJTable.C:118-135):
+// sigma_ij = (s_i+s_j)/2 (or sqrt(s_i s_j) if geometric), eps_ij = sqrt(e_i e_j),
+// B = 4 eps sigma^6, A = B sigma^6.
+void ljMix(int ntypes, const double* sigma, const double* eps, int geometric,
+           double* A, double* B);
+
+// CudaComputeNonbonded::buildExclusions (src/CudaComputeNonbonded.C:2191-2316):
+// unique sorted offset lists (self included), sorted by maxdiff, packed into a
+// bit array; per atom (start, maxdiff) where start is the bit of offset 0.
+// Returns false and sets err if more than 2^27 words would be needed.
+bool buildExclusions(int natoms, const int64_t* exclPtr, const int* exclIdx,
+                     std::vector<I2>& startMaxdiff, std::vector<uint32_t>& bits,
+                     int& numLists, std::string& err);
+
+// Plain bit lookup used by the kernels on pairlist steps
+// (src/CudaComputeNonbondedKernel.cu:930-947).
+inline bool isExcluded(const I2& sm, const uint32_t* bits, int iatom, int jatom) {
+  int diff = jatom - iatom;
+  if (diff < -sm.y || diff > sm.y) return false;
+  unsigned idx = (unsigned)(sm.x + diff);
+  return (bits[idx >> 5] >> (idx & 31)) & 1u;
+}
+
+struct PatchLayout {
+  // grid
+  int dims[3] = {0, 0, 0};
+  double box[3] = {0, 0, 0};        // quantized edge lengths
+  double patchSize[3] = {0, 0, 0};
+  std::vector<double> centre;       // 3 per patch, multiples of CENTRE_QUANTUM
+
+  // records
+  std::vector<PatchRecord> patches;
+  std::vector<ComputeRecord> computes;
+  std::vector<TileList> tileListSkeleton;   // one per (compute, i-tile); jtile range empty
+  long long jtileCapacity = 0;              // worst-case j-tile entries
+
+  // storage
+  int natoms = 0;
+  int atomStorageSize = 0;          // padded, multiple of TILE
+  std::vector<int> patchOfAtom;     // global atom -> patch
+  std::vector<int> slotOfAtom;      // global atom -> storage slot
+  std::vector<int> atomIndex;       // slot -> global atom (padding: copy of last atom)
+  std::vector<F4> xyzq;             // slot: patch-centred quantized position + scaled charge
+  std::vector<int> vdwTypes;        // slot
+  std::vector<I2> exclStartMaxdiff; // slot
+  std::vector<double> effPos;       // global atom: centre + quantized patch-centred position
+
+  // Assign atoms to patches, ORB-sort each patch into TILE groups, build
+  // computes (self + 13 half-shell neighbours) and the tile-list skeleton.
+  bool build(int n, const double* pos, const double boxIn[3], double patchDim, std::string& err);
+
+  // Fill slot arrays for the current assignment (the xyzq staging of
+  // HomePatch::positionsReady, src/HomePatch.C:1965-1995).  chargeScaled is q*sqrt(COULOMB*scaling/eps).
+  void pack(const double* pos, const double* chargeScaled, const int* vdwType,
+            const I2* exclByAtom);
+
+  // Repack positions only (same patch assignment); atoms may drift outside
+  // their patch until the next build().
+  void packPositions(const double* pos);
+};
+
+// ORB partition of one patch into TILE-sized spatially compact groups
+// (src/SortAtoms.C:45-132): split at the multiple of TILE nearest the median
+// along the longest extent, recursively.
+void orbPartition(int* order, const double* pos, int begin, int end);
+
+}  // namespace mi355nb

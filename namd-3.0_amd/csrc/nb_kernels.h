@@ -1,0 +1,57 @@
+// Launchers for the nonbonded kernels (nb_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "nbtypes.h"
+
+namespace mi355nb {
+
+struct ForceArgsHost {
+  const int* numLists;
+  const TileList* tileLists;
+  const int* tileJatomStart;
+  TileExcl* tileExcls;
+  const F4* xyzq;
+  const int* vdwTypes;
+  int vdwWidth;
+  const float* vdwCoef;   // float2 pairs (A, B)
+  NbConstants c;
+  float cutoff2;
+  float plcutoff2;
+  const PatchRecord* patches;
+  const int* atomIndex;
+  const I2* exclStartMaxdiff;
+  const unsigned* exclBits;
+  int* jtileFlags;
+  int* listDepth;
+  TileListStat* stat;
+  float* fsoa;
+  int stride;
+  TileListVirialEnergy* virialEnergy;
+};
+
+hipError_t launchBoundingBoxes(int numTiles, const F4* xyzq, BoundingBox* bbox, hipStream_t s);
+hipError_t launchBuildJtiles(int pass, int numLists, TileList* lists, const PatchRecord* patches,
+                             const BoundingBox* bbox, float cull2, int* counts, const int* offsets,
+                             int* tileJatomStart, hipStream_t s);
+hipError_t scanExclusive(void* temp, size_t& tempBytes, const int* in, int* out, int n, hipStream_t s);
+hipError_t sortDescending(void* temp, size_t& tempBytes, const int* keysIn, int* keysOut,
+                          const int* valsIn, int* valsOut, int n, hipStream_t s);
+hipError_t launchSetupSort(int numLists, const int* depth, int* keys, int* vals, hipStream_t s);
+hipError_t launchRepack(int maxLists, const int* numActive, const int* sortedVals, const int* newStart,
+                        const TileList* inLists, const int* inJatom, const TileExcl* inExcl,
+                        const int* jtileFlags, TileList* outLists, int* outJatom, TileExcl* outExcl,
+                        hipStream_t s);
+hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& a, bool doEnergy, bool doVirial,
+                                bool doSlow, bool doPairlist, hipStream_t s);
+hipError_t launchTransposeForces(int n, int stride, float* fsoa, F4* f, F4* fs, bool doSlow, hipStream_t s);
+hipError_t launchFill(size_t n, float* p, float v, hipStream_t s);
+hipError_t launchClearStat(TileListStat* st, hipStream_t s);
+int reducePartialBlocks(int nSlots, int maxLists);
+hipError_t launchReduceVirialEnergy(int nSlots, int maxLists, const F4* xyzq, const F4* f, const F4* fs,
+                                    const int* numLists, const TileListVirialEnergy* tlve, bool doEnergy,
+                                    bool doVirial, bool doSlow, double* partials, VirialEnergy* out,
+                                    hipStream_t s);
+
+}  // namespace mi355nb

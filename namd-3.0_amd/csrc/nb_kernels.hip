@@ -1,0 +1,676 @@
+// Nonbonded hot-path kernels for gfx950 (wave64).
+//
+//  * buildBoundingBoxes      - per 32-atom tile AABB (src/CudaTileListKernel.cu:241-284)
+//  * countJtiles / fillJtiles- coarse bbox-vs-bbox culling at the pairlist cutoff
+//                              (buildTileListsBBKernel, src/CudaTileListKernel.cu:405-548);
+//                              two passes + scan instead of NAMD's atomic slot
+//                              allocation with a host retry loop, so the buffers
+//                              are sized once on the host and never overflow.
+//  * nonbondedForce<...>     - the pair kernel (src/CudaComputeNonbondedKernel.cu:629-1645)
+//  * repackTileLists         - drop empty j-tiles / lists after refinement,
+//                              lists ordered by descending depth
+//                              (sortTileLists/repackTileListsKernel, :550-715, :1208-1669)
+//  * transposeForces         - SOA -> float4 AOS and zero SOA (:2057-2073)
+//  * reduceVirialEnergy      - FP64 virial / energy reduction (:1649-1919)
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "nb_kernels.h"
+
+namespace mi355nb {
+
+// ---------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float waveSum(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int waveSumInt(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// bounding boxes
+// ---------------------------------------------------------------------------
+__global__ void buildBoundingBoxesKernel(int numTiles, const F4* __restrict__ xyzq,
+                                         BoundingBox* __restrict__ bbox) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= numTiles) return;
+  const F4* p = xyzq + (size_t)t * TILE;
+  float lx = p[0].x, ly = p[0].y, lz = p[0].z;
+  float hx = lx, hy = ly, hz = lz;
+  for (int k = 1; k < TILE; ++k) {
+    F4 a = p[k];
+    lx = fminf(lx, a.x); hx = fmaxf(hx, a.x);
+    ly = fminf(ly, a.y); hy = fmaxf(hy, a.y);
+    lz = fminf(lz, a.z); hz = fmaxf(hz, a.z);
+  }
+  BoundingBox b;
+  b.x = 0.5f * (lx + hx); b.y = 0.5f * (ly + hy); b.z = 0.5f * (lz + hz);
+  // round the half-widths up slightly so the culling test stays conservative
+  b.wx = 0.5f * (hx - lx) * 1.000001f + 1e-5f;
+  b.wy = 0.5f * (hy - ly) * 1.000001f + 1e-5f;
+  b.wz = 0.5f * (hz - lz) * 1.000001f + 1e-5f;
+  bbox[t] = b;
+}
+
+__device__ __forceinline__ float bboxDist2(const BoundingBox& a, float sx, float sy, float sz,
+                                           const BoundingBox& b) {
+  float dx = fmaxf(0.0f, fabsf(a.x + sx - b.x) - a.wx - b.wx);
+  float dy = fmaxf(0.0f, fabsf(a.y + sy - b.y) - a.wy - b.wy);
+  float dz = fmaxf(0.0f, fabsf(a.z + sz - b.z) - a.wz - b.wz);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// One thread per candidate tile list.  pass 0 counts, pass 1 writes.
+template <int pass>
+__global__ void buildJtilesKernel(int numLists, TileList* __restrict__ lists,
+                                  const PatchRecord* __restrict__ patches,
+                                  const BoundingBox* __restrict__ bbox, float cull2,
+                                  int* __restrict__ counts, const int* __restrict__ offsets,
+                                  int* __restrict__ tileJatomStart) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= numLists) return;
+  TileList tl = lists[l];
+  const PatchRecord pb = patches[tl.patchInd[1]];
+  const int itile = tl.iatomStart / TILE;
+  const int jt0 = tl.self ? itile : pb.atomStart / TILE;
+  const int jt1 = (pb.atomStart + pb.numAtoms + TILE - 1) / TILE;
+  const BoundingBox bi = bbox[itile];
+  int n = 0;
+  int base = (pass == 1) ? offsets[l] : 0;
+  for (int jt = jt0; jt < jt1; ++jt) {
+    bool keep = (tl.self && jt == itile) || bboxDist2(bi, tl.offx, tl.offy, tl.offz, bbox[jt]) < cull2;
+    if (keep) {
+      if (pass == 1) tileJatomStart[base + n] = jt * TILE;
+      ++n;
+    }
+  }
+  if (pass == 0) {
+    counts[l] = n;
+  } else {
+    lists[l].jtileStart = base;
+    lists[l].jtileEnd = base + n - 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pair math: energy-switched LJ + C1-split PME electrostatics, direct math
+// (cudaNBForceMagCalc_VdwEnergySwitch_PMEC1, src/CudaComputeNonbondedInteractions.h:37-214).
+// f and fSlow are (1/r) dE/dr, so F_i = f * (x_j - x_i).
+//
+// erfc uses Abramowitz & Stegun 7.1.26 (Hastings): erfc(x) = t*P(t)*exp(-x^2),
+// t = 1/(1+p x), |error| <= 1.5e-7, sharing exp(-x^2) with the gradient term.
+// ---------------------------------------------------------------------------
+#define AS_P 0.3275911f
+#define AS_A1 0.254829592f
+#define AS_A2 -0.284496736f
+#define AS_A3 1.421413741f
+#define AS_A4 -1.453152027f
+#define AS_A5 1.061405429f
+#define LOG2E_F 1.4426950408889634f
+
+template <bool doEnergy, bool doSlow>
+__device__ __forceinline__ void pairForce(const float r2, const float qq, const float A, const float B,
+                                          const NbConstants& c, float& f, float& fSlow,
+                                          float& eVdw, float& eElec, float& eSlow) {
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float rinv2 = rinv * rinv;
+  const float rinv3 = rinv * rinv2;
+  const float rinv6 = rinv3 * rinv3;
+  const float rinv8 = rinv6 * rinv2;
+
+  // LJ with energy switching (CudaComputeNonbondedInteractions.h:37-61)
+  const float ab_r6 = A * rinv6 - B;
+  const float w = ab_r6 * rinv6;
+  const float dw_r = (A * rinv6 + ab_r6) * -6.0f * rinv8;
+  const bool sw = r2 > c.lj_5;
+  const float delta_r = c.lj_4 - r2;
+  const float s = sw ? delta_r * delta_r * (c.lj_0 + c.lj_1 * r2) : 1.0f;
+  const float ds_r = sw ? delta_r * (c.lj_3 + c.lj_2 * r2) : 0.0f;
+  const float f_vdw = w * ds_r + dw_r * s;
+  if (doEnergy) eVdw += w * s;
+
+  // fast electrostatics (PMEFast_C1, :107-123)
+  const float elec_fast = -1.0f * rinv3 + c.e_0;
+  f = qq * elec_fast + f_vdw;
+  if (doEnergy || doSlow) {
+    const float slow_energy = 0.5f * c.e_2 * (3.0f - r2 * c.e_1);
+    if (doEnergy) eElec += qq * (rinv - slow_energy);
+    if (doSlow) {
+      // slow electrostatics (PMESlow_C1, :131-159)
+      const float r = r2 * rinv;
+      const float a = r * c.ewald_0;
+      const float ex = __builtin_amdgcn_exp2f(-(a * a) * LOG2E_F);
+      const float t = __builtin_amdgcn_rcpf(1.0f + AS_P * a);
+      const float poly = t * (AS_A1 + t * (AS_A2 + t * (AS_A3 + t * (AS_A4 + t * AS_A5))));
+      const float erfc_a = poly * ex;
+      const float corr_grad = erfc_a + c.ewald_1 * r * ex;
+      const float elec_slow = -1.0f * c.e_0 * r;
+      const float scor_grad = (elec_slow - (corr_grad - 1.0f) * rinv2) * rinv;
+      fSlow = qq * scor_grad;
+      if (doEnergy) eSlow += qq * (slow_energy + (erfc_a - 1.0f) * rinv);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the pair kernel
+//
+// One workgroup = one wave64 = one tile list.  Lane l holds i-atom (l & 31);
+// half-wave h = l >> 5.  For every j-tile of the list both halves sweep the
+// same 32x32 tile pair: at step t (0..15) lane l handles
+//     j = (i + t + 16 h) mod 32,
+// so within a half each j is touched by exactly one lane per step and the
+// j-force accumulates in LDS without atomics (one LDS copy per half).  After
+// the sweep the two copies are summed and added to the SOA force arrays with
+// coalesced float atomics; i-forces are summed across halves at the end of
+// the list.  On pairlist steps the kernel evaluates every slot against the
+// pairlist cutoff and the exclusion bit table and writes the 32-bit masks.
+// ---------------------------------------------------------------------------
+struct ForceArgs {
+  const int* numLists;               // device: number of lists to process
+  const TileList* tileLists;
+  const int* tileJatomStart;
+  TileExcl* tileExcls;
+  const F4* xyzq;
+  const int* vdwTypes;
+  int vdwWidth;
+  const float2* vdwCoef;
+  NbConstants c;
+  float cutoff2;
+  // pairlist
+  float plcutoff2;
+  const PatchRecord* patches;
+  const int* atomIndex;
+  const I2* exclStartMaxdiff;
+  const unsigned* exclBits;
+  int* jtileFlags;
+  int* listDepth;
+  TileListStat* stat;
+  // outputs
+  float* fsoa;                       // 6 * stride floats: fx fy fz fsx fsy fsz
+  int stride;
+  TileListVirialEnergy* virialEnergy;
+};
+
+template <bool doEnergy, bool doVirial, bool doSlow, bool doPairlist>
+__global__ void __launch_bounds__(64)
+nonbondedForceKernel(const ForceArgs args) {
+  const int list = blockIdx.x;
+  if (list >= *args.numLists) return;
+  const int lane = threadIdx.x;
+  const int i = lane & 31;
+  const int h = lane >> 5;
+
+  __shared__ F4 s_xyzq[TILE];
+  __shared__ int s_type[TILE];
+  __shared__ int s_index[TILE];
+  __shared__ F4 s_jf[2][TILE];
+  __shared__ F4 s_jfs[2][TILE];
+
+  const TileList tl = args.tileLists[list];
+  const int iatom = tl.iatomStart + i;
+  F4 xi = args.xyzq[iatom];
+  xi.x += tl.offx;
+  xi.y += tl.offy;
+  xi.z += tl.offz;
+  const int vdwRow = args.vdwTypes[iatom] * args.vdwWidth;
+
+  float ifx = 0.f, ify = 0.f, ifz = 0.f;
+  float ifsx = 0.f, ifsy = 0.f, ifsz = 0.f;
+  float eVdw = 0.f, eElec = 0.f, eSlow = 0.f;
+
+  // pairlist-step state
+  bool iValid = true;
+  int iIndex = 0;
+  I2 iExcl{0, 0};
+  int jEnd = 0;
+  int nPairsCut = 0, nPairsList = 0, nExcl = 0, depth = 0;
+  if (doPairlist) {
+    const PatchRecord pa = args.patches[tl.patchInd[0]];
+    const PatchRecord pb = args.patches[tl.patchInd[1]];
+    iValid = iatom < pa.atomStart + pa.numAtoms;
+    jEnd = pb.atomStart + pb.numAtoms;
+    iIndex = args.atomIndex[iatom];
+    iExcl = args.exclStartMaxdiff[iatom];
+  }
+
+  for (int jt = tl.jtileStart; jt <= tl.jtileEnd; ++jt) {
+    const int jatomStart = args.tileJatomStart[jt];
+    if (h == 0) {
+      s_xyzq[i] = args.xyzq[jatomStart + i];
+      s_type[i] = args.vdwTypes[jatomStart + i];
+      if (doPairlist) s_index[i] = args.atomIndex[jatomStart + i];
+    }
+    s_jf[h][i] = F4{0.f, 0.f, 0.f, 0.f};
+    if (doSlow) s_jfs[h][i] = F4{0.f, 0.f, 0.f, 0.f};
+    unsigned mask = doPairlist ? 0u : args.tileExcls[jt].excl[i];
+    const bool selfTile = doPairlist && tl.self && (jatomStart == tl.iatomStart);
+    __syncthreads();
+
+#pragma unroll 2
+    for (int t = 0; t < 16; ++t) {
+      const int j = (i + t + 16 * h) & 31;
+      const F4 xj = s_xyzq[j];
+      const float dx = xj.x - xi.x;
+      const float dy = xj.y - xi.y;
+      const float dz = xj.z - xi.z;
+      const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+      bool active;
+      if (doPairlist) {
+        bool inList = iValid && (jatomStart + j < jEnd) && (!selfTile || j > i) && (r2 < args.plcutoff2);
+        bool excluded = false;
+        if (inList) {
+          const int diff = s_index[j] - iIndex;
+          if (diff >= -iExcl.y && diff <= iExcl.y) {
+            const unsigned b = (unsigned)(iExcl.x + diff);
+            excluded = (args.exclBits[b >> 5] >> (b & 31)) & 1u;
+          }
+        }
+        if (inList && excluded) ++nExcl;
+        inList = inList && !excluded;
+        if (inList) { mask |= 1u << j; ++nPairsList; }
+        active = inList && (r2 < args.cutoff2);
+        if (active) ++nPairsCut;
+      } else {
+        active = ((mask >> j) & 1u) && (r2 < args.cutoff2);
+      }
+      if (active) {
+        const float2 ab = args.vdwCoef[vdwRow + s_type[j]];
+        float f, fs = 0.f;
+        pairForce<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, f, fs, eVdw, eElec, eSlow);
+        const float fx = dx * f, fy = dy * f, fz = dz * f;
+        ifx += fx; ify += fy; ifz += fz;
+        F4 jf = s_jf[h][j];
+        jf.x -= fx; jf.y -= fy; jf.z -= fz;
+        s_jf[h][j] = jf;
+        if (doSlow) {
+          const float fsx = dx * fs, fsy = dy * fs, fsz = dz * fs;
+          ifsx += fsx; ifsy += fsy; ifsz += fsz;
+          F4 jfs = s_jfs[h][j];
+          jfs.x -= fsx; jfs.y -= fsy; jfs.z -= fsz;
+          s_jfs[h][j] = jfs;
+        }
+      }
+    }
+    __syncthreads();
+
+    // j-forces: sum the two half-wave copies, coalesced atomics to SOA
+    if (h == 0) {
+      const int ja = jatomStart + i;
+      const F4 a = s_jf[0][i], b = s_jf[1][i];
+      atomicAdd(&args.fsoa[ja], a.x + b.x);
+      atomicAdd(&args.fsoa[args.stride + ja], a.y + b.y);
+      atomicAdd(&args.fsoa[2 * args.stride + ja], a.z + b.z);
+      if (doSlow) {
+        const F4 c = s_jfs[0][i], d = s_jfs[1][i];
+        atomicAdd(&args.fsoa[3 * args.stride + ja], c.x + d.x);
+        atomicAdd(&args.fsoa[4 * args.stride + ja], c.y + d.y);
+        atomicAdd(&args.fsoa[5 * args.stride + ja], c.z + d.z);
+      }
+    }
+    if (doPairlist) {
+      const unsigned full = mask | __shfl_down(mask, 32, 64);
+      const bool any = __any(full != 0u);
+      if (h == 0) args.tileExcls[jt].excl[i] = full;
+      if (lane == 0) args.jtileFlags[jt] = any ? 1 : 0;
+      depth += any ? 1 : 0;
+    }
+    __syncthreads();
+  }
+
+  // i-forces: combine halves, atomics to SOA
+  ifx += __shfl_down(ifx, 32, 64);
+  ify += __shfl_down(ify, 32, 64);
+  ifz += __shfl_down(ifz, 32, 64);
+  if (doSlow) {
+    ifsx += __shfl_down(ifsx, 32, 64);
+    ifsy += __shfl_down(ifsy, 32, 64);
+    ifsz += __shfl_down(ifsz, 32, 64);
+  }
+  if (h == 0) {
+    atomicAdd(&args.fsoa[iatom], ifx);
+    atomicAdd(&args.fsoa[args.stride + iatom], ify);
+    atomicAdd(&args.fsoa[2 * args.stride + iatom], ifz);
+    if (doSlow) {
+      atomicAdd(&args.fsoa[3 * args.stride + iatom], ifsx);
+      atomicAdd(&args.fsoa[4 * args.stride + iatom], ifsy);
+      atomicAdd(&args.fsoa[5 * args.stride + iatom], ifsz);
+    }
+  }
+
+  if (doPairlist) {
+    const int a = waveSumInt(nPairsCut);
+    const int b = waveSumInt(nPairsList);
+    const int e = waveSumInt(nExcl);
+    if (lane == 0) {
+      args.listDepth[list] = depth;
+      if (depth > 0) atomicAdd(&args.stat->numTileLists, 1);
+      atomicAdd(&args.stat->numJtiles, depth);
+      atomicMax(&args.stat->maxDepth, depth);
+      atomicAdd(&args.stat->numExcluded, e);
+      atomicAdd(&args.stat->numPairsCutoff, (unsigned long long)a);
+      atomicAdd(&args.stat->numPairsList, (unsigned long long)b);
+    }
+  }
+  if (doVirial) {
+    // per-list i-force sums (lanes 0..31 hold the combined values; zero the rest)
+    const float m = (h == 0) ? 1.0f : 0.0f;
+    const float sx = waveSum(ifx * m), sy = waveSum(ify * m), sz = waveSum(ifz * m);
+    float ssx = 0.f, ssy = 0.f, ssz = 0.f;
+    if (doSlow) { ssx = waveSum(ifsx * m); ssy = waveSum(ifsy * m); ssz = waveSum(ifsz * m); }
+    if (lane == 0) {
+      TileListVirialEnergy& ve = args.virialEnergy[list];
+      ve.shx = tl.offx; ve.shy = tl.offy; ve.shz = tl.offz;
+      ve.forcex = sx; ve.forcey = sy; ve.forcez = sz;
+      ve.forceSlowx = ssx; ve.forceSlowy = ssy; ve.forceSlowz = ssz;
+    }
+  }
+  if (doEnergy) {
+    const float a = waveSum(eVdw), b = waveSum(eElec), e = doSlow ? waveSum(eSlow) : 0.f;
+    if (lane == 0) {
+      TileListVirialEnergy& ve = args.virialEnergy[list];
+      ve.energyVdw = a; ve.energyElec = b; ve.energySlow = e;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// repack: compacted, depth-ordered lists for the non-pairlist steps
+// ---------------------------------------------------------------------------
+__global__ void setupSortKernel(int numLists, const int* __restrict__ depth, int* __restrict__ keys,
+                                int* __restrict__ vals) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= numLists) return;
+  keys[l] = depth[l];
+  vals[l] = l;
+}
+
+// one wave per output list
+__global__ void __launch_bounds__(64)
+repackTileListsKernel(const int* __restrict__ numActive, const int* __restrict__ sortedVals,
+                      const int* __restrict__ newStart, const TileList* __restrict__ inLists,
+                      const int* __restrict__ inJatom, const TileExcl* __restrict__ inExcl,
+                      const int* __restrict__ jtileFlags, TileList* __restrict__ outLists,
+                      int* __restrict__ outJatom, TileExcl* __restrict__ outExcl) {
+  const int k = blockIdx.x;
+  if (k >= *numActive) return;
+  const int lane = threadIdx.x;
+  const int old = sortedVals[k];
+  TileList tl = inLists[old];
+  int pos = newStart[k];
+  const int first = pos;
+  for (int jt = tl.jtileStart; jt <= tl.jtileEnd; ++jt) {
+    if (jtileFlags[jt]) {
+      if (lane < TILE) outExcl[pos].excl[lane] = inExcl[jt].excl[lane];
+      if (lane == 0) outJatom[pos] = inJatom[jt];
+      ++pos;
+    }
+  }
+  if (lane == 0) {
+    tl.jtileStart = first;
+    tl.jtileEnd = pos - 1;
+    outLists[k] = tl;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SOA -> AOS and zero SOA (transposeForcesKernel)
+// ---------------------------------------------------------------------------
+__global__ void transposeForcesKernel(int n, int stride, float* __restrict__ fsoa, F4* __restrict__ f,
+                                      F4* __restrict__ fs, int doSlow) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  f[k] = F4{fsoa[k], fsoa[stride + k], fsoa[2 * stride + k], 0.f};
+  fsoa[k] = 0.f; fsoa[stride + k] = 0.f; fsoa[2 * stride + k] = 0.f;
+  if (doSlow) {
+    fs[k] = F4{fsoa[3 * stride + k], fsoa[4 * stride + k], fsoa[5 * stride + k], 0.f};
+    fsoa[3 * stride + k] = 0.f; fsoa[4 * stride + k] = 0.f; fsoa[5 * stride + k] = 0.f;
+  }
+}
+
+__global__ void fillKernel(size_t n, float* __restrict__ p, float v) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+__global__ void clearStatKernel(TileListStat* s) {
+  if (threadIdx.x == 0) {
+    s->numTileLists = 0; s->numJtiles = 0; s->numExcluded = 0; s->maxDepth = 0;
+    s->numPairsCutoff = 0ull; s->numPairsList = 0ull;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 virial / energy reduction, deterministic two-pass
+//   virial      = sum_slots  F_slot (x) x_slot          (reduceNonbondedVirialKernel)
+//               + sum_lists  F_list (x) shift_list       (reduceVirialEnergyKernel)
+//   component order a*3+b = F_a * x_b as in NAMD.
+// Partial record per block: 9 fast + 9 slow + 3 energies = 21 doubles.
+// ---------------------------------------------------------------------------
+constexpr int RED_THREADS = 256;
+constexpr int RED_N = 21;
+
+__device__ __forceinline__ double blockSumD(double v, double* sh) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < RED_THREADS / 64; ++k) s += sh[k];
+  return s;
+}
+
+__global__ void __launch_bounds__(RED_THREADS)
+reducePartialsKernel(int nAtomBlocks, int nSlots, const F4* __restrict__ xyzq, const F4* __restrict__ f,
+                     const F4* __restrict__ fs, const int* __restrict__ numLists,
+                     const TileListVirialEnergy* __restrict__ tlve, int doEnergy, int doVirial, int doSlow,
+                     double* __restrict__ partials) {
+  __shared__ double sh[RED_THREADS / 64];
+  double v[RED_N];
+#pragma unroll
+  for (int k = 0; k < RED_N; ++k) v[k] = 0.0;
+  if ((int)blockIdx.x < nAtomBlocks) {
+    const int s = blockIdx.x * RED_THREADS + threadIdx.x;
+    if (doVirial && s < nSlots) {
+      const F4 p = xyzq[s];
+      const F4 a = f[s];
+      const double x[3] = {p.x, p.y, p.z};
+      const double fa[3] = {a.x, a.y, a.z};
+      for (int u = 0; u < 3; ++u)
+        for (int w = 0; w < 3; ++w) v[u * 3 + w] = fa[u] * x[w];
+      if (doSlow) {
+        const F4 b = fs[s];
+        const double fb[3] = {b.x, b.y, b.z};
+        for (int u = 0; u < 3; ++u)
+          for (int w = 0; w < 3; ++w) v[9 + u * 3 + w] = fb[u] * x[w];
+      }
+    }
+  } else {
+    const int l = (blockIdx.x - nAtomBlocks) * RED_THREADS + threadIdx.x;
+    if (l < *numLists) {
+      const TileListVirialEnergy e = tlve[l];
+      if (doVirial) {
+        const double sh3[3] = {e.shx, e.shy, e.shz};
+        const double fa[3] = {e.forcex, e.forcey, e.forcez};
+        const double fb[3] = {e.forceSlowx, e.forceSlowy, e.forceSlowz};
+        for (int u = 0; u < 3; ++u)
+          for (int w = 0; w < 3; ++w) {
+            v[u * 3 + w] = fa[u] * sh3[w];
+            if (doSlow) v[9 + u * 3 + w] = fb[u] * sh3[w];
+          }
+      }
+      if (doEnergy) {
+        v[18] = e.energyVdw;
+        v[19] = e.energyElec;
+        v[20] = doSlow ? e.energySlow : 0.0;
+      }
+    }
+  }
+  for (int k = 0; k < RED_N; ++k) {
+    double s = blockSumD(v[k], sh);
+    if (threadIdx.x == 0) partials[(size_t)blockIdx.x * RED_N + k] = s;
+  }
+}
+
+__global__ void __launch_bounds__(RED_THREADS)
+reduceFinalKernel(int nBlocks, const double* __restrict__ partials, VirialEnergy* __restrict__ out) {
+  __shared__ double sh[RED_THREADS / 64];
+  for (int k = 0; k < RED_N; ++k) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nBlocks; b += RED_THREADS) v += partials[(size_t)b * RED_N + k];
+    double s = blockSumD(v, sh);
+    if (threadIdx.x == 0) {
+      if (k < 9) out->virial[k] = s;
+      else if (k < 18) out->virialSlow[k - 9] = s;
+      else if (k == 18) out->energyVdw = s;
+      else if (k == 19) out->energyElec = s;
+      else out->energySlow = s;
+    }
+  }
+  if (threadIdx.x == 0) {
+    out->energyVdw_s = out->energyElec_s = out->energySlow_s = 0.0;
+    out->energyVdw_ti_1 = out->energyVdw_ti_2 = 0.0;
+    out->energyElec_ti_1 = out->energyElec_ti_2 = 0.0;
+    out->energySlow_ti_1 = out->energySlow_ti_2 = 0.0;
+    out->energyGBIS = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launchBoundingBoxes(int numTiles, const F4* xyzq, BoundingBox* bbox, hipStream_t s) {
+  if (numTiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(buildBoundingBoxesKernel, dim3((numTiles + 127) / 128), dim3(128), 0, s,
+                     numTiles, xyzq, bbox);
+  return hipGetLastError();
+}
+
+hipError_t launchBuildJtiles(int pass, int numLists, TileList* lists, const PatchRecord* patches,
+                             const BoundingBox* bbox, float cull2, int* counts, const int* offsets,
+                             int* tileJatomStart, hipStream_t s) {
+  if (numLists <= 0) return hipSuccess;
+  dim3 g((numLists + 127) / 128), b(128);
+  if (pass == 0)
+    hipLaunchKernelGGL(buildJtilesKernel<0>, g, b, 0, s, numLists, lists, patches, bbox, cull2, counts,
+                       offsets, tileJatomStart);
+  else
+    hipLaunchKernelGGL(buildJtilesKernel<1>, g, b, 0, s, numLists, lists, patches, bbox, cull2, counts,
+                       offsets, tileJatomStart);
+  return hipGetLastError();
+}
+
+hipError_t scanExclusive(void* temp, size_t& tempBytes, const int* in, int* out, int n, hipStream_t s) {
+  return hipcub::DeviceScan::ExclusiveSum(temp, tempBytes, in, out, n, s);
+}
+
+hipError_t sortDescending(void* temp, size_t& tempBytes, const int* keysIn, int* keysOut,
+                          const int* valsIn, int* valsOut, int n, hipStream_t s) {
+  return hipcub::DeviceRadixSort::SortPairsDescending(temp, tempBytes, keysIn, keysOut, valsIn, valsOut, n,
+                                                       0, 8 * (int)sizeof(int), s);
+}
+
+hipError_t launchSetupSort(int numLists, const int* depth, int* keys, int* vals, hipStream_t s) {
+  if (numLists <= 0) return hipSuccess;
+  hipLaunchKernelGGL(setupSortKernel, dim3((numLists + 255) / 256), dim3(256), 0, s, numLists, depth, keys,
+                     vals);
+  return hipGetLastError();
+}
+
+hipError_t launchRepack(int maxLists, const int* numActive, const int* sortedVals, const int* newStart,
+                        const TileList* inLists, const int* inJatom, const TileExcl* inExcl,
+                        const int* jtileFlags, TileList* outLists, int* outJatom, TileExcl* outExcl,
+                        hipStream_t s) {
+  if (maxLists <= 0) return hipSuccess;
+  hipLaunchKernelGGL(repackTileListsKernel, dim3(maxLists), dim3(64), 0, s, numActive, sortedVals, newStart,
+                     inLists, inJatom, inExcl, jtileFlags, outLists, outJatom, outExcl);
+  return hipGetLastError();
+}
+
+hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEnergy, bool doVirial,
+                                bool doSlow, bool doPairlist, hipStream_t s) {
+  if (maxLists <= 0) return hipSuccess;
+  ForceArgs a;
+  a.numLists = h.numLists;
+  a.tileLists = h.tileLists;
+  a.tileJatomStart = h.tileJatomStart;
+  a.tileExcls = h.tileExcls;
+  a.xyzq = h.xyzq;
+  a.vdwTypes = h.vdwTypes;
+  a.vdwWidth = h.vdwWidth;
+  a.vdwCoef = reinterpret_cast<const float2*>(h.vdwCoef);
+  a.c = h.c;
+  a.cutoff2 = h.cutoff2;
+  a.plcutoff2 = h.plcutoff2;
+  a.patches = h.patches;
+  a.atomIndex = h.atomIndex;
+  a.exclStartMaxdiff = h.exclStartMaxdiff;
+  a.exclBits = h.exclBits;
+  a.jtileFlags = h.jtileFlags;
+  a.listDepth = h.listDepth;
+  a.stat = h.stat;
+  a.fsoa = h.fsoa;
+  a.stride = h.stride;
+  a.virialEnergy = h.virialEnergy;
+  dim3 g(maxLists), b(64);
+  const int key = (doEnergy ? 8 : 0) | (doVirial ? 4 : 0) | (doSlow ? 2 : 0) | (doPairlist ? 1 : 0);
+  switch (key) {
+#define NB_CASE(E, V, S, P)                                                                  \
+  case (E * 8 + V * 4 + S * 2 + P):                                                          \
+    hipLaunchKernelGGL((nonbondedForceKernel<E, V, S, P>), g, b, 0, s, a);                   \
+    break;
+    NB_CASE(0, 0, 0, 0) NB_CASE(0, 0, 0, 1) NB_CASE(0, 0, 1, 0) NB_CASE(0, 0, 1, 1)
+    NB_CASE(0, 1, 0, 0) NB_CASE(0, 1, 0, 1) NB_CASE(0, 1, 1, 0) NB_CASE(0, 1, 1, 1)
+    NB_CASE(1, 0, 0, 0) NB_CASE(1, 0, 0, 1) NB_CASE(1, 0, 1, 0) NB_CASE(1, 0, 1, 1)
+    NB_CASE(1, 1, 0, 0) NB_CASE(1, 1, 0, 1) NB_CASE(1, 1, 1, 0) NB_CASE(1, 1, 1, 1)
+#undef NB_CASE
+  }
+  return hipGetLastError();
+}
+
+hipError_t launchTransposeForces(int n, int stride, float* fsoa, F4* f, F4* fs, bool doSlow, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(transposeForcesKernel, dim3((n + 255) / 256), dim3(256), 0, s, n, stride, fsoa, f, fs,
+                     doSlow ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launchFill(size_t n, float* p, float v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fillKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, p, v);
+  return hipGetLastError();
+}
+
+hipError_t launchClearStat(TileListStat* st, hipStream_t s) {
+  hipLaunchKernelGGL(clearStatKernel, dim3(1), dim3(64), 0, s, st);
+  return hipGetLastError();
+}
+
+int reducePartialBlocks(int nSlots, int maxLists) {
+  return (nSlots + RED_THREADS - 1) / RED_THREADS + (maxLists + RED_THREADS - 1) / RED_THREADS;
+}
+
+hipError_t launchReduceVirialEnergy(int nSlots, int maxLists, const F4* xyzq, const F4* f, const F4* fs,
+                                    const int* numLists, const TileListVirialEnergy* tlve, bool doEnergy,
+                                    bool doVirial, bool doSlow, double* partials, VirialEnergy* out,
+                                    hipStream_t s) {
+  const int nA = (nSlots + RED_THREADS - 1) / RED_THREADS;
+  const int nL = (maxLists + RED_THREADS - 1) / RED_THREADS;
+  hipLaunchKernelGGL(reducePartialsKernel, dim3(nA + nL), dim3(RED_THREADS), 0, s, nA, nSlots, xyzq, f, fs,
+                     numLists, tlve, doEnergy ? 1 : 0, doVirial ? 1 : 0, doSlow ? 1 : 0, partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(reduceFinalKernel, dim3(1), dim3(RED_THREADS), 0, s, nA + nL, partials, out);
+  return hipGetLastError();
+}
+
+}  // namespace mi355nb

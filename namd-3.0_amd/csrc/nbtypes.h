@@ -1,0 +1,120 @@
+// Plain-old-data records shared by the host layout code, the HIP kernels and
+// the C ABI.  Field meanings follow NAMD's GPU nonbonded path; the reference
+// definitions are cited on each record.
+#pragma once
+#include <stdint.h>
+
+namespace mi355nb {
+
+// Tile width.  Patches are padded to a multiple of TILE atoms and sorted so
+// that each TILE-atom group is spatially compact.  A wave64 sweeps one
+// TILE x TILE tile pair with both half-waves (lanes 0-31 and 32-63 each cover
+// half of the j rotation).
+constexpr int TILE = 32;
+constexpr int WAVE = 64;
+
+struct F4 { float x, y, z, w; };
+struct I2 { int x, y; };
+
+// CudaNBConstants (src/CudaUtils.h:574-590), same field order.
+struct NbConstants {
+  float lj_0;        // c1*rc^2 - 3*c1*rs^2
+  float lj_1;        // 2*c1
+  float lj_2;        // -12*c1
+  float lj_3;        // 12*c1*rs^2
+  float lj_4;        // rc^2
+  float lj_5;        // rs^2
+  float e_0;         // 1/rc^3
+  float e_0_slow;    // (1/rc^3)*(1-slowScale)
+  float e_1;         // 1/rc^2
+  float e_2;         // 1/rc
+  float ewald_0;     // beta
+  float ewald_1;     // 2*beta/sqrt(pi)
+  float ewald_2;     // beta^2
+  float ewald_3_slow;// beta^3*slowScale
+  float slowScale;   // fullElectFrequency/nonbondedFrequency
+};
+
+// CudaPatchRecord (src/CudaTileListKernel.h:50-57).
+struct PatchRecord {
+  int numAtoms;
+  int numFreeAtoms;
+  int atomStart;     // first storage slot, multiple of TILE
+  int patchID;
+};
+
+// CudaComputeRecord (src/CudaTileListKernel.h:41-45).  offsetXYZ here is the
+// Cartesian shift in Angstrom that is added to patchInd[0] atoms to bring them
+// into the frame of patchInd[1]; NAMD stores lattice units and multiplies by
+// the lattice vectors inside the kernel.  The shift is exact in FP32 (patch
+// centres are multiples of 2^-10 A).
+struct ComputeRecord {
+  int patchInd[2];
+  float offx, offy, offz;
+  int self;          // 1 for a self compute (same patch, zero shift)
+};
+
+// TileList (src/CudaTileListKernel.h:10-21).
+struct TileList {
+  int iatomStart;
+  int jtileStart;
+  int jtileEnd;      // inclusive
+  float offx, offy, offz;
+  int patchInd[2];
+  int icompute;
+  int self;          // first j-tile is the self tile (upper triangle only)
+};
+
+// TileExcl (src/CudaTileListKernel.h:6-8).  Bit = 1 means the pair is in the
+// list (within the pairlist cutoff and not excluded).  Bit j of excl[i] is the
+// pair (iatomStart+i, jatomStart+j); NAMD stores the same bits rotated
+// (bit t <-> j = (i+t) mod 32).  mi355nb_tile_excl_to_namd() converts.
+struct TileExcl {
+  uint32_t excl[TILE];
+};
+
+// BoundingBox (src/CudaTileListKernel.h:33-36).
+struct BoundingBox {
+  float x, y, z;
+  float wx, wy, wz;
+};
+
+// TileListStat (src/CudaTileListKernel.h:62-70) plus the pair checksum.
+struct TileListStat {
+  int numTileLists;          // lists with at least one interacting j-tile
+  int numJtiles;             // j-tiles kept after refinement
+  int numExcluded;           // excluded pairs within the pairlist cutoff
+  int maxDepth;              // largest number of j-tiles in one list
+  unsigned long long numPairsCutoff;   // non-excluded pairs with r2 < cutoff2
+  unsigned long long numPairsList;     // non-excluded pairs with r2 < plcutoff2
+};
+
+// TileListVirialEnergy (src/CudaTileListKernel.h:72-94): the per-list partials
+// that the FP64 reduction consumes.  Energies are FP32 per list.
+struct TileListVirialEnergy {
+  float shx, shy, shz;
+  float forcex, forcey, forcez;
+  float forceSlowx, forceSlowy, forceSlowz;
+  float energyVdw, energyElec, energySlow;
+};
+
+// VirialEnergy (src/CudaTileListKernel.h:96-117), same field order.
+struct VirialEnergy {
+  double virial[9];
+  double virialSlow[9];
+  double energyVdw;
+  double energyElec;
+  double energySlow;
+  double energyVdw_s;
+  double energyElec_s;
+  double energySlow_s;
+  double energyVdw_ti_1;
+  double energyVdw_ti_2;
+  double energyElec_ti_1;
+  double energyElec_ti_2;
+  double energySlow_ti_1;
+  double energySlow_ti_2;
+  double energyGBIS;
+};
+
+}  // namespace mi355nb

@@ -1,0 +1,300 @@
+// PME real-space kernels for gfx950: charge spreading, force gathering, self
+// energy and fractional coordinates.  Grid convention follows NAMD's GPU PME
+// (src/CudaPmeSolverUtilKernel.cu:426-915): with frx = K*x (x the fractional
+// coordinate in [0,1)), frxi = (int)frx and w = frx - frxi, grid point
+// (frxi + k) mod K receives the order-n cardinal B-spline weight theta[k].
+//
+// Mapping to the wave: a workgroup is one wave64 and owns 64 atoms.  Pass 1
+// gives each lane one atom and computes its 3*order weights into LDS; pass 2
+// walks the 64 atoms and gives each lane one point of the order^3 stencil
+// (order 4: exactly one point per lane), so each atom's x-runs of the stencil
+// are contiguous in memory.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "../../include/mi355nb.h"
+#include "devutil.h"
+#include "nbtypes.h"
+
+namespace mi355nb {
+
+// Order-n cardinal B-spline weights and derivatives for fractional offset w,
+// by the standard recursion M_n(u) = (u M_{n-1}(u) + (n-u) M_{n-1}(u-1))/(n-1)
+// (Essmann et al. 1995); theta[k] is the weight of grid point frxi + k.
+template <int ORDER>
+__device__ __forceinline__ void bsplineWeights(float w, float* th, float* dth) {
+  th[0] = 1.0f - w;
+  th[1] = w;
+#pragma unroll
+  for (int k = 2; k < ORDER; ++k) th[k] = 0.0f;
+#pragma unroll
+  for (int n = 3; n <= ORDER; ++n) {
+    if (n == ORDER && dth) {
+      // derivative of the order-n spline from the order-(n-1) weights
+      dth[0] = -th[0];
+#pragma unroll
+      for (int k = 1; k < ORDER; ++k) dth[k] = th[k - 1] - th[k];
+    }
+    const float div = 1.0f / (float)(n - 1);
+    th[n - 1] = div * w * th[n - 2];
+#pragma unroll
+    for (int j = 1; j <= n - 2; ++j)
+      th[n - j - 1] = div * ((w + (float)j) * th[n - j - 2] + ((float)(n - j) - w) * th[n - j - 1]);
+    th[0] = div * (1.0f - w) * th[0];
+  }
+}
+
+template <int ORDER>
+__global__ void __launch_bounds__(64)
+spreadChargeKernel(const F4* __restrict__ atoms, int natoms, int K1, int K2, int K3, int xdim,
+                   float* __restrict__ grid) {
+  __shared__ int s_i[3][64];
+  __shared__ float s_t[3][64 * ORDER];
+  const int base = blockIdx.x * 64;
+  const int lane = threadIdx.x;
+  const int a = base + lane;
+  if (a < natoms) {
+    const F4 at = atoms[a];
+    const float fr[3] = {(float)K1 * at.x, (float)K2 * at.y, (float)K3 * at.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      // clamp keeps the stencil inside the grid even for inputs outside [0,1)
+      const int fi = min(max((int)fr[d], 0), (d == 0 ? K1 : (d == 1 ? K2 : K3)) - 1);
+      float th[ORDER];
+      bsplineWeights<ORDER>(fr[d] - (float)fi, th, nullptr);
+      s_i[d][lane] = fi;
+#pragma unroll
+      for (int k = 0; k < ORDER; ++k) s_t[d][lane * ORDER + k] = (d == 0 ? at.w : 1.0f) * th[k];
+    }
+  }
+  __syncthreads();
+  const int na = min(64, natoms - base);
+  constexpr int P = ORDER * ORDER * ORDER;
+  for (int k = 0; k < na; ++k) {
+    for (int p = lane; p < P; p += 64) {
+      const int x0 = p % ORDER, y0 = (p / ORDER) % ORDER, z0 = p / (ORDER * ORDER);
+      int x = s_i[0][k] + x0; if (x >= K1) x -= K1;
+      int y = s_i[1][k] + y0; if (y >= K2) y -= K2;
+      int z = s_i[2][k] + z0; if (z >= K3) z -= K3;
+      const float v = s_t[0][k * ORDER + x0] * s_t[1][k * ORDER + y0] * s_t[2][k * ORDER + z0];
+      atomicAdd(&grid[x + (size_t)xdim * (y + (size_t)K2 * z)], v);
+    }
+  }
+}
+
+__device__ __forceinline__ float waveSum64(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+struct Recip { float m[9]; int cartesian; };
+struct RecipD { double m[9]; };
+
+template <int ORDER>
+__global__ void __launch_bounds__(64)
+gatherForceKernel(const F4* __restrict__ atoms, int natoms, int K1, int K2, int K3, int xdim,
+                  const float* __restrict__ grid, Recip rc, F4* __restrict__ force) {
+  __shared__ int s_i[3][64];
+  __shared__ float s_t[3][64 * ORDER];
+  __shared__ float s_d[3][64 * ORDER];
+  __shared__ float s_f[3][64];
+  const int base = blockIdx.x * 64;
+  const int lane = threadIdx.x;
+  const int a = base + lane;
+  float q = 0.0f;
+  if (a < natoms) {
+    const F4 at = atoms[a];
+    q = at.w;
+    const float fr[3] = {(float)K1 * at.x, (float)K2 * at.y, (float)K3 * at.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      // clamp keeps the stencil inside the grid even for inputs outside [0,1)
+      const int fi = min(max((int)fr[d], 0), (d == 0 ? K1 : (d == 1 ? K2 : K3)) - 1);
+      float th[ORDER], dth[ORDER];
+      bsplineWeights<ORDER>(fr[d] - (float)fi, th, dth);
+      s_i[d][lane] = fi;
+#pragma unroll
+      for (int k = 0; k < ORDER; ++k) {
+        s_t[d][lane * ORDER + k] = th[k];
+        s_d[d][lane * ORDER + k] = dth[k];
+      }
+    }
+  }
+  __syncthreads();
+  const int na = min(64, natoms - base);
+  constexpr int P = ORDER * ORDER * ORDER;
+  for (int k = 0; k < na; ++k) {
+    float f1 = 0.f, f2 = 0.f, f3 = 0.f;
+    for (int p = lane; p < P; p += 64) {
+      const int x0 = p % ORDER, y0 = (p / ORDER) % ORDER, z0 = p / (ORDER * ORDER);
+      int x = s_i[0][k] + x0; if (x >= K1) x -= K1;
+      int y = s_i[1][k] + y0; if (y >= K2) y -= K2;
+      int z = s_i[2][k] + z0; if (z >= K3) z -= K3;
+      const float Q = grid[x + (size_t)xdim * (y + (size_t)K2 * z)];
+      const float tx = s_t[0][k * ORDER + x0], ty = s_t[1][k * ORDER + y0], tz = s_t[2][k * ORDER + z0];
+      f1 += s_d[0][k * ORDER + x0] * ty * tz * Q;
+      f2 += tx * s_d[1][k * ORDER + y0] * tz * Q;
+      f3 += tx * ty * s_d[2][k * ORDER + z0] * Q;
+    }
+    f1 = waveSum64(f1);
+    f2 = waveSum64(f2);
+    f3 = waveSum64(f3);
+    if (lane == 0) { s_f[0][k] = f1; s_f[1][k] = f2; s_f[2][k] = f3; }
+  }
+  __syncthreads();
+  if (a < natoms) {
+    const float g1 = -q * s_f[0][lane] * (float)K1;
+    const float g2 = -q * s_f[1][lane] * (float)K2;
+    const float g3 = -q * s_f[2][lane] * (float)K3;
+    F4 out;
+    if (rc.cartesian) {
+      out.x = g1 * rc.m[0] + g2 * rc.m[3] + g3 * rc.m[6];
+      out.y = g1 * rc.m[1] + g2 * rc.m[4] + g3 * rc.m[7];
+      out.z = g1 * rc.m[2] + g2 * rc.m[5] + g3 * rc.m[8];
+    } else {
+      out.x = g1; out.y = g2; out.z = g3;
+    }
+    out.w = 0.0f;
+    force[a] = out;
+  }
+}
+
+constexpr int SE_THREADS = 256;
+
+__global__ void __launch_bounds__(SE_THREADS)
+selfEnergyKernel(const F4* __restrict__ atoms, int natoms, double* __restrict__ partials) {
+  __shared__ double sh[SE_THREADS / 64];
+  double v = 0.0;
+  for (int a = blockIdx.x * SE_THREADS + threadIdx.x; a < natoms; a += gridDim.x * SE_THREADS) {
+    const double q = atoms[a].w;
+    v += q * q;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < SE_THREADS / 64; ++k) s += sh[k];
+    partials[blockIdx.x] = s;
+  }
+}
+
+__global__ void selfEnergyFinalKernel(int nb, const double* __restrict__ partials, double c,
+                                      double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += partials[b];
+    *out = c * s;
+  }
+}
+
+__global__ void fracCoordsKernel(const double* __restrict__ pos, const double* __restrict__ charge, int natoms,
+                                 RecipD rc, double ox, double oy, double oz, double qscale, F4* __restrict__ out) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= natoms) return;
+  const double px = pos[3 * a] - ox, py = pos[3 * a + 1] - oy, pz = pos[3 * a + 2] - oz;
+  double w[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double r = px * rc.m[3 * d] + py * rc.m[3 * d + 1] + pz * rc.m[3 * d + 2];
+    w[d] = r - (floor(r + 0.5) - 0.5);
+  }
+  F4 o;
+  o.x = (float)w[0]; if (o.x >= 1.0f) o.x -= 1.0f;
+  o.y = (float)w[1]; if (o.y >= 1.0f) o.y -= 1.0f;
+  o.z = (float)w[2]; if (o.z >= 1.0f) o.z -= 1.0f;
+  o.w = (float)(charge[a] * qscale);
+  out[a] = o;
+}
+
+}  // namespace mi355nb
+
+using namespace mi355nb;
+
+#define PME_CHECK(cond, msg)            \
+  do {                                  \
+    if (!(cond)) {                      \
+      setLastError(msg);                \
+      return MI355NB_ERR_ARG;           \
+    }                                   \
+  } while (0)
+
+static int checkGrid(int natoms, int K1, int K2, int K3, int xdim, int order) {
+  PME_CHECK(natoms >= 0, "natoms < 0");
+  PME_CHECK(order == 4 || order == 6 || order == 8, "order must be 4, 6 or 8");
+  PME_CHECK(K1 >= order && K2 >= order && K3 >= order, "grid smaller than the interpolation order");
+  PME_CHECK(xdim >= K1, "xdim < nfftx");
+  PME_CHECK((double)xdim * K2 * K3 < 2147483647.0 * 4.0, "grid too large");
+  return MI355NB_OK;
+}
+
+extern "C" {
+
+int mi355nb_spread_charge(const float* atoms, int natoms, int nfftx, int nffty, int nfftz, int xdim, float* grid,
+                          int order, void* stream) {
+  int rc = checkGrid(natoms, nfftx, nffty, nfftz, xdim, order);
+  if (rc) return rc;
+  if (natoms == 0) return MI355NB_OK;
+  PME_CHECK(atoms && grid, "null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  dim3 g((natoms + 63) / 64), b(64);
+  const F4* at = reinterpret_cast<const F4*>(atoms);
+  if (order == 4) hipLaunchKernelGGL(spreadChargeKernel<4>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid);
+  else if (order == 6) hipLaunchKernelGGL(spreadChargeKernel<6>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid);
+  else hipLaunchKernelGGL(spreadChargeKernel<8>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+int mi355nb_gather_force(const float* atoms, int natoms, int nfftx, int nffty, int nfftz, int xdim,
+                         const float* grid, int order, const double* recip, float* force, void* stream) {
+  int rc = checkGrid(natoms, nfftx, nffty, nfftz, xdim, order);
+  if (rc) return rc;
+  if (natoms == 0) return MI355NB_OK;
+  PME_CHECK(atoms && grid && force, "null pointer");
+  Recip r;
+  r.cartesian = recip ? 1 : 0;
+  for (int k = 0; k < 9; ++k) r.m[k] = recip ? (float)recip[k] : 0.0f;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 g((natoms + 63) / 64), b(64);
+  const F4* at = reinterpret_cast<const F4*>(atoms);
+  F4* f = reinterpret_cast<F4*>(force);
+  if (order == 4) hipLaunchKernelGGL(gatherForceKernel<4>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid, r, f);
+  else if (order == 6) hipLaunchKernelGGL(gatherForceKernel<6>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid, r, f);
+  else hipLaunchKernelGGL(gatherForceKernel<8>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid, r, f);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+int mi355nb_self_energy(const float* atoms, int natoms, double ewaldcof, double* energy, void* stream) {
+  PME_CHECK(natoms >= 0 && energy && (natoms == 0 || atoms), "bad self-energy arguments");
+  hipStream_t s = (hipStream_t)stream;
+  static thread_local DevBuf<double> partials;
+  const int nb = 64;
+  MI355NB_HIP(partials.reserve(nb, 1.0f));
+  hipLaunchKernelGGL(selfEnergyKernel, dim3(nb), dim3(SE_THREADS), 0, s, reinterpret_cast<const F4*>(atoms), natoms,
+                     partials.ptr);
+  MI355NB_HIP(hipGetLastError());
+  const double c = -ewaldcof / std::sqrt(M_PI);
+  hipLaunchKernelGGL(selfEnergyFinalKernel, dim3(1), dim3(64), 0, s, nb, partials.ptr, c, energy);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+int mi355nb_pme_frac_coords(const double* pos, const double* charge, int natoms, const double recip[9],
+                            const double origin[3], float* atoms, void* stream) {
+  PME_CHECK(natoms >= 0 && recip && origin && (natoms == 0 || (pos && charge && atoms)), "bad arguments");
+  if (natoms == 0) return MI355NB_OK;
+  RecipD r;
+  for (int k = 0; k < 9; ++k) r.m[k] = recip[k];
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(fracCoordsKernel, dim3((natoms + 255) / 256), dim3(256), 0, s, pos, charge, natoms, r, origin[0],
+                     origin[1], origin[2], std::sqrt(332.0636), reinterpret_cast<F4*>(atoms));
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,228 @@
+"""Python driver of the C ABI, mirroring NAMD's CudaComputeNonbonded call
+surface (src/CudaComputeNonbonded.h:279-309) and the PME real-space free
+functions (src/CudaPmeSolverUtilKernel.h:6-51).
+
+The host object lives in C++ (libmi355nb.so); this module only marshals
+arrays.  Device buffers for the PME calls are torch tensors (plumbing only).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import lib as L
+from .systems import System, ewald_coefficient
+
+COULOMB = 332.0636
+
+
+class ComputeNonbonded:
+    """One compute object per (device, stream); see include/mi355nb.h."""
+
+    def __init__(self, cutoff=12.0, switchdist=10.0, pairlistdist=13.5, ewaldcof=None, ewald_tol=1e-6,
+                 slow_scale=1.0, margin=0.0, device=0, stream=None):
+        if ewaldcof is None:
+            ewaldcof = L.ewald_coefficient(cutoff, ewald_tol)
+        self.params = L.NbParams(cutoff, switchdist, pairlistdist, ewaldcof, slow_scale, margin)
+        self._h = C.c_void_p()
+        L.check(L.lib().mi355nb_compute_create(C.byref(self.params), int(device), stream, C.byref(self._h)),
+                "compute_create")
+        self.natoms = 0
+
+    @classmethod
+    def from_system(cls, s: System, device=0, stream=None, pairlistdist=None):
+        p = s.params
+        beta = ewald_coefficient(p.cutoff, p.ewald_tol)
+        c = cls(p.cutoff, p.switchdist, pairlistdist or p.pairlistdist, beta, slow_scale=1.0, margin=p.margin,
+                device=device, stream=stream)
+        A, B = L.lj_mix(s.lj_sigma, s.lj_eps)
+        c.set_molecule(s.vdw_type, s.charge, A, B, s.excl_ptr, s.excl_idx)
+        c.assign_patches(s.box, s.pos)
+        return c
+
+    def close(self):
+        if self._h:
+            L.lib().mi355nb_compute_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- registration / staging -----------------------------------------
+    def set_molecule(self, vdw_type, charge, A, B, excl_ptr, excl_idx):
+        vdw_type = np.ascontiguousarray(vdw_type, np.int32)
+        charge = np.ascontiguousarray(charge, np.float64)
+        A = np.ascontiguousarray(A, np.float64).reshape(-1)
+        B = np.ascontiguousarray(B, np.float64).reshape(-1)
+        ntypes = int(round(math.sqrt(A.size)))
+        excl_ptr = np.ascontiguousarray(excl_ptr, np.int64)
+        excl_idx = np.ascontiguousarray(excl_idx, np.int32)
+        if excl_idx.size == 0:
+            excl_idx = np.zeros(1, np.int32)
+        self.natoms = vdw_type.size
+        L.check(L.lib().mi355nb_compute_set_molecule(self._h, self.natoms, vdw_type, charge, ntypes, A, B,
+                                                     excl_ptr, excl_idx), "set_molecule")
+
+    def assign_patches(self, box, pos):
+        L.check(L.lib().mi355nb_compute_assign_patches(self._h, np.ascontiguousarray(box, np.float64),
+                                                       np.ascontiguousarray(pos, np.float64).reshape(-1)),
+                "assign_patches")
+
+    def update_positions(self, pos):
+        L.check(L.lib().mi355nb_compute_update_positions(self._h, np.ascontiguousarray(pos, np.float64).reshape(-1)),
+                "update_positions")
+
+    # --- launchWork / finish ---------------------------------------------
+    def launch_work(self, do_energy=True, do_virial=True, do_slow=True, do_pairlist=False):
+        L.check(L.lib().mi355nb_compute_launch(self._h, int(do_energy), int(do_virial), int(do_slow),
+                                               int(do_pairlist)), "launch")
+
+    def finish(self, forces=True):
+        n = self.natoms
+        f = np.zeros(3 * n) if forces else None
+        fs = np.zeros(3 * n) if forces else None
+        ve = L.VirialEnergy()
+        st = L.TileListStat()
+        L.check(L.lib().mi355nb_compute_finish(self._h, f.ctypes.data if forces else None,
+                                               fs.ctypes.data if forces else None, C.byref(ve), C.byref(st)),
+                "finish")
+        out = dict(
+            e_vdw=ve.energyVdw, e_elec=ve.energyElec, e_slow=ve.energySlow,
+            virial=np.array(ve.virial[:]).reshape(3, 3), virial_slow=np.array(ve.virialSlow[:]).reshape(3, 3),
+            stat=dict(numTileLists=st.numTileLists, numJtiles=st.numJtiles, numExcluded=st.numExcluded,
+                      maxDepth=st.maxDepth, numPairsCutoff=st.numPairsCutoff, numPairsList=st.numPairsList))
+        if forces:
+            out["f"] = f.reshape(n, 3)
+            out["fs"] = fs.reshape(n, 3)
+        return out
+
+    def compute(self, do_energy=True, do_virial=True, do_slow=True, do_pairlist=True):
+        self.launch_work(do_energy, do_virial, do_slow, do_pairlist)
+        return self.finish()
+
+    # --- kernel timing (HIP events on the compute's stream) ----------------
+    def set_timing(self, enable=True):
+        L.check(L.lib().mi355nb_compute_set_timing(self._h, int(enable)), "set_timing")
+
+    def kernel_time(self):
+        """(ms_pairlist_total, n_pairlist, ms_force_total, n_force) since the last call; resets."""
+        a, b = C.c_double(0), C.c_double(0)
+        na, nb = C.c_int32(0), C.c_int32(0)
+        L.check(L.lib().mi355nb_compute_get_kernel_time(self._h, C.byref(a), C.byref(na), C.byref(b), C.byref(nb)),
+                "get_kernel_time")
+        return a.value, na.value, b.value, nb.value
+
+    # --- introspection ----------------------------------------------------
+    def info(self):
+        i = L.LayoutInfo()
+        L.check(L.lib().mi355nb_compute_get_info(self._h, C.byref(i)), "get_info")
+        return dict(natoms=i.natoms, atomStorageSize=i.atomStorageSize, numPatches=i.numPatches,
+                    numComputes=i.numComputes, numCandidateLists=i.numCandidateLists,
+                    patchDims=tuple(i.patchDims), jtileCapacity=i.jtileCapacity, box=tuple(i.box))
+
+    def device_buffers(self):
+        d = L.DeviceBuffers()
+        L.check(L.lib().mi355nb_compute_get_device_buffers(self._h, C.byref(d)), "get_device_buffers")
+        return dict(xyzq=d.xyzq, force=d.force, forceSlow=d.forceSlow, atomIndex=d.atomIndex)
+
+    def nonbonded_coef(self):
+        c = L.NbConstants()
+        L.check(L.lib().mi355nb_compute_get_nonbonded_coef(self._h, C.byref(c)), "get_nonbonded_coef")
+        return c
+
+    def effective_positions(self):
+        p = np.zeros(3 * self.natoms)
+        L.check(L.lib().mi355nb_compute_get_effective_positions(self._h, p), "get_effective_positions")
+        return p.reshape(self.natoms, 3)
+
+    def tile_lists(self):
+        nl = C.c_int32(0)
+        nj = C.c_int32(0)
+        L.check(L.lib().mi355nb_compute_get_tile_lists(self._h, C.byref(nl), C.byref(nj), None, None, None, None,
+                                                       None, None, None), "get_tile_lists")
+        info = self.info()
+        ia = np.zeros(nl.value, np.int32)
+        js = np.zeros(nl.value, np.int32)
+        je = np.zeros(nl.value, np.int32)
+        sh = np.zeros(3 * nl.value, np.float32)
+        ja = np.zeros(max(1, nj.value), np.int32)
+        ex = np.zeros((max(1, nj.value), 32), np.uint32)
+        slots = np.zeros(info["atomStorageSize"], np.int32)
+        L.check(L.lib().mi355nb_compute_get_tile_lists(self._h, C.byref(nl), C.byref(nj), ia.ctypes.data,
+                                                       js.ctypes.data, je.ctypes.data, sh.ctypes.data,
+                                                       ja.ctypes.data, ex.ctypes.data, slots.ctypes.data),
+                "get_tile_lists")
+        return dict(iatomStart=ia, jtileStart=js, jtileEnd=je, shift=sh.reshape(-1, 3), jatomStart=ja[: nj.value],
+                    excl=ex[: nj.value], atomIndex=slots)
+
+
+def decode_pairs(tl, natoms_storage_valid=None):
+    """Global (i, j) atom pairs (i < j) encoded by the tile masks of ComputeNonbonded.tile_lists()."""
+    ia = tl["iatomStart"]
+    js, je = tl["jtileStart"], tl["jtileEnd"]
+    lens = je - js + 1
+    list_of_jtile = np.repeat(np.arange(len(ia)), lens)
+    jt = np.concatenate([np.arange(a, b + 1) for a, b in zip(js, je)]) if len(ia) else np.zeros(0, np.int64)
+    ex = tl["excl"][jt]                                   # (nj, 32) words, word i = i-lane
+    bits = ((ex[:, :, None] >> np.arange(32, dtype=np.uint32)[None, None, :]) & 1).astype(bool)  # (nj, i, j)
+    t_idx, i_idx, j_idx = np.nonzero(bits)
+    islot = ia[list_of_jtile[t_idx]] + i_idx
+    jslot = tl["jatomStart"][jt[t_idx]] + j_idx
+    ai = tl["atomIndex"][islot].astype(np.int64)
+    aj = tl["atomIndex"][jslot].astype(np.int64)
+    return np.minimum(ai, aj), np.maximum(ai, aj)
+
+
+# ---------------------------------------------------------------------------
+# PME real space on device tensors
+# ---------------------------------------------------------------------------
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return stream
+
+
+def spread_charge(atoms, grid, K, order=4, xdim=None, stream=None):
+    """atoms: cuda float32 (n,4); grid: cuda float32 zeroed (K3*K2*xdim). Accumulates."""
+    K1, K2, K3 = K
+    L.check(L.lib().mi355nb_spread_charge(atoms.data_ptr(), atoms.shape[0], K1, K2, K3, xdim or K1,
+                                          grid.data_ptr(), order, _stream_ptr(stream)), "spread_charge")
+
+
+def gather_force(atoms, grid, K, order=4, recip=None, out=None, xdim=None, stream=None):
+    import torch
+    K1, K2, K3 = K
+    if out is None:
+        out = torch.empty((atoms.shape[0], 4), dtype=torch.float32, device=atoms.device)
+    r = None if recip is None else np.ascontiguousarray(recip, np.float64).reshape(9)
+    L.check(L.lib().mi355nb_gather_force(atoms.data_ptr(), atoms.shape[0], K1, K2, K3, xdim or K1, grid.data_ptr(),
+                                         order, r.ctypes.data if r is not None else None, out.data_ptr(),
+                                         _stream_ptr(stream)), "gather_force")
+    return out
+
+
+def self_energy(atoms, ewaldcof, out=None, stream=None):
+    import torch
+    if out is None:
+        out = torch.zeros(1, dtype=torch.float64, device=atoms.device)
+    L.check(L.lib().mi355nb_self_energy(atoms.data_ptr(), atoms.shape[0], float(ewaldcof), out.data_ptr(),
+                                        _stream_ptr(stream)), "self_energy")
+    return out
+
+
+def frac_coords(pos, charge, recip, origin=(0.0, 0.0, 0.0), out=None, stream=None):
+    """pos: cuda float64 (n,3); charge: cuda float64 (n,) in e. Returns cuda float32 (n,4)."""
+    import torch
+    if out is None:
+        out = torch.empty((pos.shape[0], 4), dtype=torch.float32, device=pos.device)
+    L.check(L.lib().mi355nb_pme_frac_coords(pos.data_ptr(), charge.data_ptr(), pos.shape[0],
+                                            np.ascontiguousarray(recip, np.float64).reshape(9),
+                                            np.ascontiguousarray(origin, np.float64), out.data_ptr(),
+                                            _stream_ptr(stream)), "pme_frac_coords")
+    return out

@@ -1,0 +1,160 @@
+"""ctypes binding of libmi355nb.so (the C ABI declared in include/mi355nb.h).
+
+There is no CPU fallback: if the library is missing or fails to load, every
+call raises.  Errors returned by the library are raised as Mi355nbError with
+the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libmi355nb.so")
+
+_LIB = None
+
+
+class Mi355nbError(RuntimeError):
+    pass
+
+
+class NbConstants(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("lj_0", "lj_1", "lj_2", "lj_3", "lj_4", "lj_5", "e_0", "e_0_slow", "e_1",
+                                          "e_2", "ewald_0", "ewald_1", "ewald_2", "ewald_3_slow", "slowScale")]
+
+
+class VirialEnergy(C.Structure):
+    _fields_ = [("virial", C.c_double * 9), ("virialSlow", C.c_double * 9)] + [
+        (n, C.c_double) for n in ("energyVdw", "energyElec", "energySlow", "energyVdw_s", "energyElec_s",
+                                  "energySlow_s", "energyVdw_ti_1", "energyVdw_ti_2", "energyElec_ti_1",
+                                  "energyElec_ti_2", "energySlow_ti_1", "energySlow_ti_2", "energyGBIS")]
+
+
+class TileListStat(C.Structure):
+    _fields_ = [("numTileLists", C.c_int32), ("numJtiles", C.c_int32), ("numExcluded", C.c_int32),
+                ("maxDepth", C.c_int32), ("numPairsCutoff", C.c_uint64), ("numPairsList", C.c_uint64)]
+
+
+class NbParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("cutoff", "switchdist", "pairlistdist", "ewaldcof", "slowScale", "margin")]
+
+
+class LayoutInfo(C.Structure):
+    _fields_ = [("natoms", C.c_int32), ("atomStorageSize", C.c_int32), ("numPatches", C.c_int32),
+                ("numComputes", C.c_int32), ("numCandidateLists", C.c_int32), ("patchDims", C.c_int32 * 3),
+                ("jtileCapacity", C.c_int64), ("box", C.c_double * 3)]
+
+
+class DeviceBuffers(C.Structure):
+    _fields_ = [("xyzq", C.c_void_p), ("force", C.c_void_p), ("forceSlow", C.c_void_p), ("atomIndex", C.c_void_p)]
+
+
+EXPORTED = [
+    "mi355nb_last_error", "mi355nb_abi_version", "mi355nb_ewald_coefficient", "mi355nb_get_nonbonded_coef",
+    "mi355nb_get_do_table", "mi355nb_lj_mix", "mi355nb_build_exclusions", "mi355nb_tile_excl_to_namd",
+    "mi355nb_compute_create", "mi355nb_compute_destroy", "mi355nb_compute_set_molecule",
+    "mi355nb_compute_assign_patches", "mi355nb_compute_update_positions", "mi355nb_compute_launch",
+    "mi355nb_compute_finish", "mi355nb_compute_set_timing", "mi355nb_compute_get_kernel_time",
+    "mi355nb_compute_get_info", "mi355nb_compute_get_device_buffers",
+    "mi355nb_compute_get_nonbonded_coef", "mi355nb_compute_get_effective_positions",
+    "mi355nb_compute_get_tile_lists", "mi355nb_spread_charge", "mi355nb_gather_force", "mi355nb_self_energy",
+    "mi355nb_pme_frac_coords",
+]
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+vp = C.c_void_p
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise Mi355nbError(f"{LIB_PATH} is not built (run __graft_entry__.build() or namd-3.0_amd/build.py)")
+    L = C.CDLL(LIB_PATH)
+    L.mi355nb_last_error.restype = C.c_char_p
+    L.mi355nb_ewald_coefficient.restype = C.c_double
+    L.mi355nb_ewald_coefficient.argtypes = [C.c_double, C.c_double]
+    L.mi355nb_get_nonbonded_coef.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(NbConstants)]
+    L.mi355nb_get_do_table.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.mi355nb_lj_mix.argtypes = [C.c_int, dp, dp, C.c_int, dp, dp]
+    L.mi355nb_build_exclusions.argtypes = [C.c_int, i64p, i32p, vp, vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    L.mi355nb_tile_excl_to_namd.argtypes = [u32p, u32p, C.c_int64]
+    L.mi355nb_compute_create.argtypes = [C.POINTER(NbParams), C.c_int, vp, C.POINTER(vp)]
+    L.mi355nb_compute_destroy.argtypes = [vp]
+    L.mi355nb_compute_set_molecule.argtypes = [vp, C.c_int, i32p, dp, C.c_int, dp, dp, i64p, i32p]
+    L.mi355nb_compute_assign_patches.argtypes = [vp, dp, dp]
+    L.mi355nb_compute_update_positions.argtypes = [vp, dp]
+    L.mi355nb_compute_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.mi355nb_compute_finish.argtypes = [vp, vp, vp, C.POINTER(VirialEnergy), C.POINTER(TileListStat)]
+    L.mi355nb_compute_set_timing.argtypes = [vp, C.c_int]
+    L.mi355nb_compute_get_kernel_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32),
+                                                  C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    L.mi355nb_compute_get_info.argtypes = [vp, C.POINTER(LayoutInfo)]
+    L.mi355nb_compute_get_device_buffers.argtypes = [vp, C.POINTER(DeviceBuffers)]
+    L.mi355nb_compute_get_nonbonded_coef.argtypes = [vp, C.POINTER(NbConstants)]
+    L.mi355nb_compute_get_effective_positions.argtypes = [vp, dp]
+    L.mi355nb_compute_get_tile_lists.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), vp, vp, vp, vp, vp,
+                                                 vp, vp]
+    L.mi355nb_spread_charge.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp]
+    L.mi355nb_gather_force.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp]
+    L.mi355nb_self_energy.argtypes = [vp, C.c_int, C.c_double, vp, vp]
+    L.mi355nb_pme_frac_coords.argtypes = [vp, vp, C.c_int, dp, dp, vp, vp]
+    _LIB = L
+    return L
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().mi355nb_last_error()
+        raise Mi355nbError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+    return rc
+
+
+def ewald_coefficient(cutoff: float, tol: float) -> float:
+    return lib().mi355nb_ewald_coefficient(cutoff, tol)
+
+
+def nonbonded_coef(cutoff, switchdist, ewaldcof, slow_scale=1.0) -> NbConstants:
+    c = NbConstants()
+    check(lib().mi355nb_get_nonbonded_coef(cutoff, switchdist, ewaldcof, slow_scale, C.byref(c)), "getNonbondedCoef")
+    return c
+
+
+def lj_mix(sigma, eps, geometric=False):
+    sigma = np.ascontiguousarray(sigma, np.float64)
+    eps = np.ascontiguousarray(eps, np.float64)
+    n = sigma.size
+    A = np.zeros(n * n)
+    B = np.zeros(n * n)
+    check(lib().mi355nb_lj_mix(n, sigma, eps, int(geometric), A, B), "lj_mix")
+    return A.reshape(n, n), B.reshape(n, n)
+
+
+def build_exclusions(natoms, excl_ptr, excl_idx):
+    excl_ptr = np.ascontiguousarray(excl_ptr, np.int64)
+    excl_idx = np.ascontiguousarray(excl_idx, np.int32)
+    if excl_idx.size == 0:
+        excl_idx = np.zeros(1, np.int32)
+    nwords = C.c_int64(0)
+    nlists = C.c_int32(0)
+    check(lib().mi355nb_build_exclusions(natoms, excl_ptr, excl_idx, None, None, C.byref(nwords), C.byref(nlists)),
+          "buildExclusions")
+    sm = np.zeros(2 * natoms, np.int32)
+    bits = np.zeros(max(1, nwords.value), np.uint32)
+    check(lib().mi355nb_build_exclusions(natoms, excl_ptr, excl_idx, sm.ctypes.data, bits.ctypes.data,
+                                         C.byref(nwords), C.byref(nlists)), "buildExclusions")
+    return sm.reshape(natoms, 2), bits[: nwords.value], nlists.value
+
+
+def tile_excl_to_namd(masks):
+    m = np.ascontiguousarray(masks, np.uint32).reshape(-1, 32)
+    out = np.zeros_like(m)
+    check(lib().mi355nb_tile_excl_to_namd(m.reshape(-1), out.reshape(-1), m.shape[0]), "tile_excl_to_namd")
+    return out

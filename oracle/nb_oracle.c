@@ -1,0 +1,505 @@
+/*
+ * ORACLE - test infrastructure only.  Nothing in the product library links,
+ * loads or calls this file; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it, as the checker.
+ *
+ * CPU restatements of the reference's nonbonded and PME real-space algorithms:
+ *
+ *  oracle_nb_fp64          FP64 analytic pair forces, energies and virial for
+ *                          every non-excluded minimum-image pair with r < rc:
+ *                          energy-switched LJ (ComputeNonbondedUtil.C:960-971,
+ *                          CudaComputeNonbondedInteractions.h:37-61) and the
+ *                          C1-split Ewald real space (CudaComputeNonbondedInteractions.h:107-159;
+ *                          fast + slow = erfc(beta r)/r).
+ *  oracle_pairs_fp32       the canonical pair-membership rule the kernels use:
+ *                          r2 = fma(dz,dz,fma(dy,dy,dx*dx)) in FP32 on exact
+ *                          displacements, r2 < (float)rc^2; counts and
+ *                          order-independent checksums of the pair sets, or
+ *                          the sets themselves.
+ *  oracle_pme_spread_gpu / oracle_pme_gather_gpu
+ *                          B-spline spread / gather in FP64 with NAMD's GPU
+ *                          grid convention (CudaPmeSolverUtilKernel.cu:331-915).
+ *  oracle_pme_scale_coordinates / oracle_pme_spread_cpu / oracle_pme_gather_cpu
+ *                          NAMD's CPU convention (PmeBase.inl:17-148,
+ *                          PmeRealSpace.C:237-425), order 4 with the explicit
+ *                          cubics, other orders with compute_b_spline.
+ *
+ * Parity pins are listed in the test files that use each function.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define TWO_OVER_SQRT_PI 1.1283791670955126
+
+static int is_excluded(const int64_t* ptr, const int32_t* idx, int i, int j) {
+  for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
+    if (idx[k] == j) return 1;
+  return 0;
+}
+
+static inline double min_image(double d, double L) { return d - L * nearbyint(d / L); }
+
+/* ---- FP64 pair physics --------------------------------------------------- */
+typedef struct {
+  double rc2, rs2, c1, rc, rcinv, rc3inv, beta;
+} PairConst;
+
+static void pair_fp64(const PairConst* c, double r2, double qq, double A, double B, double* f, double* fs,
+                      double* ev, double* ee, double* es) {
+  const double r = sqrt(r2);
+  const double rinv = 1.0 / r;
+  const double rinv2 = rinv * rinv;
+  const double rinv6 = rinv2 * rinv2 * rinv2;
+  /* LJ: E = A r^-12 - B r^-6, times S(r^2) above the switching distance */
+  const double E = (A * rinv6 - B) * rinv6;
+  const double dE_r = (-12.0 * A * rinv6 + 6.0 * B) * rinv6 * rinv2; /* (1/r) dE/dr */
+  double S = 1.0, dS_r = 0.0;
+  if (r2 > c->rs2) {
+    const double d = c->rc2 - r2;
+    S = d * d * (c->rc2 + 2.0 * r2 - 3.0 * c->rs2) * c->c1;
+    /* dS/dr / r = 2 dS/d(r^2) */
+    dS_r = 2.0 * c->c1 * (-2.0 * d * (c->rc2 + 2.0 * r2 - 3.0 * c->rs2) + 2.0 * d * d);
+  }
+  const double fv = dE_r * S + E * dS_r;
+  *ev += E * S;
+  /* fast: E = qq (1/r - 3/(2rc) + r^2/(2 rc^3)) */
+  *f = fv + qq * (-rinv * rinv2 + c->rc3inv);
+  *ee += qq * (rinv - 1.5 * c->rcinv + 0.5 * r2 * c->rc3inv);
+  /* slow: E = qq (erfc(beta r)/r - 1/r + 3/(2rc) - r^2/(2 rc^3)) */
+  const double a = c->beta * r;
+  const double ec = erfc(a);
+  const double g = ec + TWO_OVER_SQRT_PI * a * exp(-a * a); /* -r^2 d/dr(erfc(beta r)/r) */
+  *fs = qq * ((1.0 - g) * rinv * rinv2 - c->rc3inv);
+  *es += qq * ((ec - 1.0) * rinv + 1.5 * c->rcinv - 0.5 * r2 * c->rc3inv);
+}
+
+/* cell list over an orthorhombic box; calls back for every i<j pair with r2 < cut2 */
+typedef struct {
+  int n, nc[3];
+  double L[3], cs[3];
+  int* head;
+  int* next;
+} Cells;
+
+static void cells_build(Cells* C, int n, const double* pos, const double L[3], double cut) {
+  C->n = n;
+  for (int d = 0; d < 3; ++d) {
+    C->L[d] = L[d];
+    C->nc[d] = (int)floor(L[d] / cut);
+    if (C->nc[d] < 1) C->nc[d] = 1;
+    C->cs[d] = L[d] / C->nc[d];
+  }
+  const int ncell = C->nc[0] * C->nc[1] * C->nc[2];
+  C->head = (int*)malloc(sizeof(int) * ncell);
+  C->next = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
+  for (int k = 0; k < ncell; ++k) C->head[k] = -1;
+  for (int i = n - 1; i >= 0; --i) {
+    int g[3];
+    for (int d = 0; d < 3; ++d) {
+      double w = pos[3 * i + d] - L[d] * floor(pos[3 * i + d] / L[d]);
+      g[d] = (int)floor(w / C->cs[d]);
+      if (g[d] >= C->nc[d]) g[d] = C->nc[d] - 1;
+      if (g[d] < 0) g[d] = 0;
+    }
+    const int cidx = (g[0] * C->nc[1] + g[1]) * C->nc[2] + g[2];
+    C->next[i] = C->head[cidx];
+    C->head[cidx] = i;
+  }
+}
+
+static void cells_free(Cells* C) {
+  free(C->head);
+  free(C->next);
+}
+
+/* Neighbour cells of cell (gx,gy,gz) as a de-duplicated list (handles nc < 3). */
+static int cell_neighbours(const Cells* C, int gx, int gy, int gz, int* out) {
+  int m = 0;
+  int rng[3][3], cnt[3];
+  const int g[3] = {gx, gy, gz};
+  for (int d = 0; d < 3; ++d) {
+    cnt[d] = 0;
+    for (int o = -1; o <= 1; ++o) {
+      int v = ((g[d] + o) % C->nc[d] + C->nc[d]) % C->nc[d];
+      int dup = 0;
+      for (int k = 0; k < cnt[d]; ++k)
+        if (rng[d][k] == v) dup = 1;
+      if (!dup) rng[d][cnt[d]++] = v;
+    }
+  }
+  for (int a = 0; a < cnt[0]; ++a)
+    for (int b = 0; b < cnt[1]; ++b)
+      for (int c = 0; c < cnt[2]; ++c) out[m++] = (rng[0][a] * C->nc[1] + rng[1][b]) * C->nc[2] + rng[2][c];
+  return m;
+}
+
+/*
+ * FP64 oracle.  pos: 3n, q: scaled charges (q*sqrt(COULOMB)), type: n,
+ * A,B: ntypes^2 row-major.  Outputs (any may be NULL): f, fs (3n), energies[3]
+ * = vdw, elec, slow; virial[9], virialSlow[9] (virial[a*3+b] = sum over
+ * pairs of F_ij,a * (x_i - x_j)_b, i.e. NAMD's sum_atoms F_a x_b);
+ * npairs = number of pairs with r < rc.  nthreads <= 0: all available.
+ */
+int oracle_nb_fp64(int n, const double* pos, const double* q, const int32_t* type, int ntypes, const double* A,
+                   const double* B, const int64_t* eptr, const int32_t* eidx, const double L[3], double cutoff,
+                   double switchdist, double beta, double* f, double* fs, double* energies, double* virial,
+                   double* virialSlow, int64_t* npairs, int nthreads) {
+  PairConst c;
+  c.rc = cutoff;
+  c.rc2 = cutoff * cutoff;
+  c.rs2 = switchdist * switchdist;
+  c.c1 = (switchdist < cutoff) ? 1.0 / ((c.rc2 - c.rs2) * (c.rc2 - c.rs2) * (c.rc2 - c.rs2)) : 0.0;
+  c.rcinv = 1.0 / cutoff;
+  c.rc3inv = c.rcinv * c.rcinv * c.rcinv;
+  c.beta = beta;
+  Cells C;
+  cells_build(&C, n, pos, L, cutoff);
+  const int ncell = C.nc[0] * C.nc[1] * C.nc[2];
+  int nt = 1;
+#ifdef _OPENMP
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+  double* fb = (double*)calloc((size_t)nt * 6 * n, sizeof(double));
+  double* acc = (double*)calloc((size_t)nt * 24, sizeof(double));
+  int64_t* np = (int64_t*)calloc((size_t)nt, sizeof(int64_t));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+#endif
+  for (int cell = 0; cell < ncell; ++cell) {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double* F = fb + (size_t)tid * 6 * n;
+    double* S = acc + (size_t)tid * 24;
+    int nb[27];
+    const int gz = cell % C.nc[2], gy = (cell / C.nc[2]) % C.nc[1], gx = cell / (C.nc[1] * C.nc[2]);
+    const int m = cell_neighbours(&C, gx, gy, gz, nb);
+    for (int i = C.head[cell]; i >= 0; i = C.next[i]) {
+      for (int k = 0; k < m; ++k) {
+        for (int j = C.head[nb[k]]; j >= 0; j = C.next[j]) {
+          if (j <= i) continue;
+          double d[3];
+          for (int u = 0; u < 3; ++u) d[u] = min_image(pos[3 * j + u] - pos[3 * i + u], L[u]);
+          const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          if (!(r2 < c.rc2)) continue;
+          if (is_excluded(eptr, eidx, i, j)) continue;
+          double ff, ffs;
+          const int ab = type[i] * ntypes + type[j];
+          pair_fp64(&c, r2, q[i] * q[j], A[ab], B[ab], &ff, &ffs, &S[18], &S[19], &S[20]);
+          for (int u = 0; u < 3; ++u) {
+            F[3 * i + u] += ff * d[u];
+            F[3 * j + u] -= ff * d[u];
+            F[3 * n + 3 * i + u] += ffs * d[u];
+            F[3 * n + 3 * j + u] -= ffs * d[u];
+          }
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+              S[a * 3 + b] -= ff * d[a] * d[b];
+              S[9 + a * 3 + b] -= ffs * d[a] * d[b];
+            }
+          np[tid] += 1;
+        }
+      }
+    }
+  }
+  if (f) memset(f, 0, sizeof(double) * 3 * n);
+  if (fs) memset(fs, 0, sizeof(double) * 3 * n);
+  double tot[24] = {0};
+  int64_t ntot = 0;
+  for (int t = 0; t < nt; ++t) {
+    const double* F = fb + (size_t)t * 6 * n;
+    if (f)
+      for (int k = 0; k < 3 * n; ++k) f[k] += F[k];
+    if (fs)
+      for (int k = 0; k < 3 * n; ++k) fs[k] += F[3 * n + k];
+    for (int k = 0; k < 24; ++k) tot[k] += acc[(size_t)t * 24 + k];
+    ntot += np[t];
+  }
+  if (virial) memcpy(virial, tot, 9 * sizeof(double));
+  if (virialSlow) memcpy(virialSlow, tot + 9, 9 * sizeof(double));
+  if (energies) memcpy(energies, tot + 18, 3 * sizeof(double));
+  if (npairs) *npairs = ntot;
+  free(fb);
+  free(acc);
+  free(np);
+  cells_free(&C);
+  return 0;
+}
+
+/* ---- canonical FP32 pair sets ------------------------------------------- */
+static inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+uint64_t oracle_pair_hash(int64_t i, int64_t j, int64_t n) {
+  const int64_t a = i < j ? i : j, b = i < j ? j : i;
+  return splitmix64((uint64_t)(a * n + b));
+}
+
+/*
+ * pos must be the effective positions (multiples of 2^-17 A); the minimum-
+ * image displacement is then exact in FP64 and in FP32.  For each unordered
+ * pair: r2f = fmaf(dz,dz,fmaf(dy,dy,dx*dx)).  Counts:
+ *   out[0] non-excluded pairs with r2f < cut2f
+ *   out[1] non-excluded pairs with r2f < plcut2f
+ *   out[2] excluded pairs with r2f < plcut2f
+ * and checksums (sum and xor of oracle_pair_hash) of the sets in out[0], out[1]:
+ *   sums[0], xors[0] for the cutoff set, sums[1], xors[1] for the list set.
+ * If pairs != NULL (capacity maxpairs pairs), the list set is written as
+ * (i, j) with i < j; out[3] = pairs written (stops at capacity).
+ */
+int oracle_pairs_fp32(int n, const double* pos, const int64_t* eptr, const int32_t* eidx, const double L[3],
+                      float cut2f, float plcut2f, int64_t* out, uint64_t* sums, uint64_t* xors, int32_t* pairs,
+                      int64_t maxpairs) {
+  const double rl = sqrt((double)plcut2f) * 1.0001 + 1e-6;
+  Cells C;
+  cells_build(&C, n, pos, L, rl);
+  const int ncell = C.nc[0] * C.nc[1] * C.nc[2];
+  int64_t cnt[3] = {0, 0, 0}, written = 0;
+  uint64_t s0 = 0, s1 = 0, x0 = 0, x1 = 0;
+  for (int cell = 0; cell < ncell; ++cell) {
+    int nb[27];
+    const int gz = cell % C.nc[2], gy = (cell / C.nc[2]) % C.nc[1], gx = cell / (C.nc[1] * C.nc[2]);
+    const int m = cell_neighbours(&C, gx, gy, gz, nb);
+    for (int i = C.head[cell]; i >= 0; i = C.next[i])
+      for (int k = 0; k < m; ++k)
+        for (int j = C.head[nb[k]]; j >= 0; j = C.next[j]) {
+          if (j <= i) continue;
+          float df[3];
+          for (int u = 0; u < 3; ++u) df[u] = (float)min_image(pos[3 * j + u] - pos[3 * i + u], L[u]);
+          const float r2 = fmaf(df[2], df[2], fmaf(df[1], df[1], df[0] * df[0]));
+          if (!(r2 < plcut2f)) continue;
+          if (is_excluded(eptr, eidx, i, j)) {
+            cnt[2]++;
+            continue;
+          }
+          const uint64_t h = oracle_pair_hash(i, j, n);
+          cnt[1]++;
+          s1 += h;
+          x1 ^= h;
+          if (pairs && written < maxpairs) {
+            pairs[2 * written] = i;
+            pairs[2 * written + 1] = j;
+            written++;
+          }
+          if (r2 < cut2f) {
+            cnt[0]++;
+            s0 += h;
+            x0 ^= h;
+          }
+        }
+  }
+  out[0] = cnt[0];
+  out[1] = cnt[1];
+  out[2] = cnt[2];
+  out[3] = written;
+  if (sums) { sums[0] = s0; sums[1] = s1; }
+  if (xors) { xors[0] = x0; xors[1] = x1; }
+  cells_free(&C);
+  return 0;
+}
+
+/* ---- PME real space ------------------------------------------------------ */
+
+/* order-n weights by the B-spline recursion, GPU indexing (theta[k] at frxi + k) */
+static void bspline_gpu(double w, int order, double* th, double* dth) {
+  for (int k = 0; k < order; ++k) th[k] = 0.0;
+  th[0] = 1.0 - w;
+  th[1] = w;
+  for (int n = 3; n <= order; ++n) {
+    if (n == order && dth) {
+      dth[0] = -th[0];
+      for (int k = 1; k < order; ++k) dth[k] = th[k - 1] - th[k];
+    }
+    const double div = 1.0 / (n - 1);
+    th[n - 1] = div * w * th[n - 2];
+    for (int j = 1; j <= n - 2; ++j) th[n - j - 1] = div * ((w + j) * th[n - j - 2] + (n - j - w) * th[n - j - 1]);
+    th[0] = div * (1.0 - w) * th[0];
+  }
+}
+
+/* atoms: float4 (x,y,z fractional in [0,1), q); grid: double[K1*K2*K3], x fastest; accumulates */
+int oracle_pme_spread_gpu(int n, const float* atoms, int K1, int K2, int K3, int order, double* grid) {
+  double tx[8], ty[8], tz[8];
+  for (int a = 0; a < n; ++a) {
+    const double q = atoms[4 * a + 3];
+    const float frx = (float)K1 * atoms[4 * a], fry = (float)K2 * atoms[4 * a + 1], frz = (float)K3 * atoms[4 * a + 2];
+    const int ix = (int)frx, iy = (int)fry, iz = (int)frz;
+    bspline_gpu((double)frx - ix, order, tx, NULL);
+    bspline_gpu((double)fry - iy, order, ty, NULL);
+    bspline_gpu((double)frz - iz, order, tz, NULL);
+    for (int c = 0; c < order; ++c)
+      for (int b = 0; b < order; ++b)
+        for (int d = 0; d < order; ++d) {
+          const int x = (ix + d) % K1, y = (iy + b) % K2, z = (iz + c) % K3;
+          grid[x + (size_t)K1 * (y + (size_t)K2 * z)] += q * tx[d] * ty[b] * tz[c];
+        }
+  }
+  return 0;
+}
+
+/* fractional-gradient forces (-q K1 sum dtx ty tz Q, ...) or Cartesian with recip (row-major a*,b*,c*) */
+int oracle_pme_gather_gpu(int n, const float* atoms, int K1, int K2, int K3, int order, const double* grid,
+                          const double* recip, double* force) {
+  double tx[8], ty[8], tz[8], dx[8], dy[8], dz[8];
+  for (int a = 0; a < n; ++a) {
+    const double q = atoms[4 * a + 3];
+    const float frx = (float)K1 * atoms[4 * a], fry = (float)K2 * atoms[4 * a + 1], frz = (float)K3 * atoms[4 * a + 2];
+    const int ix = (int)frx, iy = (int)fry, iz = (int)frz;
+    bspline_gpu((double)frx - ix, order, tx, dx);
+    bspline_gpu((double)fry - iy, order, ty, dy);
+    bspline_gpu((double)frz - iz, order, tz, dz);
+    double f1 = 0, f2 = 0, f3 = 0;
+    for (int c = 0; c < order; ++c)
+      for (int b = 0; b < order; ++b)
+        for (int d = 0; d < order; ++d) {
+          const int x = (ix + d) % K1, y = (iy + b) % K2, z = (iz + c) % K3;
+          const double Q = grid[x + (size_t)K1 * (y + (size_t)K2 * z)];
+          f1 += dx[d] * ty[b] * tz[c] * Q;
+          f2 += tx[d] * dy[b] * tz[c] * Q;
+          f3 += tx[d] * ty[b] * dz[c] * Q;
+        }
+    const double g1 = -q * K1 * f1, g2 = -q * K2 * f2, g3 = -q * K3 * f3;
+    if (recip) {
+      force[3 * a] = g1 * recip[0] + g2 * recip[3] + g3 * recip[6];
+      force[3 * a + 1] = g1 * recip[1] + g2 * recip[4] + g3 * recip[7];
+      force[3 * a + 2] = g1 * recip[2] + g2 * recip[5] + g3 * recip[8];
+    } else {
+      force[3 * a] = g1;
+      force[3 * a + 1] = g2;
+      force[3 * a + 2] = g3;
+    }
+  }
+  return 0;
+}
+
+/* scale_coordinates (PmeBase.inl:17-57): p = K (s - floor s), s = shift + r.a*, shift = ((K+order-1)/2)/K */
+int oracle_pme_scale_coordinates(int n, const double* pos, const double origin[3], const double* recip, int K1,
+                                 int K2, int K3, int order, double* p) {
+  const int K[3] = {K1, K2, K3};
+  for (int d = 0; d < 3; ++d) {
+    const double shift = (double)((K[d] + order - 1) / 2) / (double)K[d];
+    for (int i = 0; i < n; ++i) {
+      const double px = pos[3 * i] - origin[0], py = pos[3 * i + 1] - origin[1], pz = pos[3 * i + 2] - origin[2];
+      const double s = shift + px * recip[3 * d] + py * recip[3 * d + 1] + pz * recip[3 * d + 2];
+      double v = K[d] * (s - floor(s));
+      if (v == K[d]) v = 0;
+      p[3 * i + d] = v;
+    }
+  }
+  return 0;
+}
+
+/* compute_b_spline (PmeBase.inl:85-148) for one atom; M, dM: 3*order */
+static void compute_b_spline_cpu(const double fr[3], double* M, double* dM, int order) {
+  for (int d = 0; d < 3; ++d) {
+    double* Mx = M + d * order - 1;  /* 1-based like the reference */
+    double* dMx = dM + d * order - 1;
+    const double x = fr[d];
+    const double x1 = 1.0 - x;
+    Mx[1] = 0.5 * x1 * x1;
+    Mx[2] = x1 * x + 0.5;
+    Mx[3] = 0.5 * x * x;
+    Mx[order] = 0.0;
+    for (int n = 4; n <= order - 1; ++n) {
+      const double div = 1.0 / (n - 1);
+      Mx[n] = x * div * Mx[n - 1];
+      for (int j = 1; j <= n - 2; ++j) Mx[n - j] = ((x + j) * Mx[n - j - 1] + (n - x - j) * Mx[n - j]) * div;
+      Mx[1] *= (1.0 - x) * div;
+    }
+    dMx[1] = -Mx[1];
+    for (int j = 2; j <= order; ++j) dMx[j] = Mx[j - 1] - Mx[j];
+    const double div = 1.0 / (order - 1);
+    Mx[order] = x * div * Mx[order - 1];
+    for (int j = 1; j <= order - 2; ++j)
+      Mx[order - j] = ((x + j) * Mx[order - j - 1] + (order - x - j) * Mx[order - j]) * div;
+    Mx[1] *= (1.0 - x) * div;
+  }
+}
+
+/*
+ * fill_charges (PmeRealSpace.C:237-345) on a plain wrapped grid: grid point
+ * ((int)p - order + 1 + k) mod K gets M[k].  order 4 uses the explicit cubics
+ * in float as fill_charges_order4 does (fr = (float)(p - (double)u)).
+ */
+int oracle_pme_spread_cpu(int n, const double* p, const double* q, int K1, int K2, int K3, int order, double* grid) {
+  double M[24], dM[24];
+  const int K[3] = {K1, K2, K3};
+  for (int a = 0; a < n; ++a) {
+    int u[3];
+    double fr[3];
+    for (int d = 0; d < 3; ++d) {
+      u[d] = (int)p[3 * a + d];
+      fr[d] = (double)(float)(p[3 * a + d] - (double)u[d]);
+    }
+    if (order == 4) {
+      for (int d = 0; d < 3; ++d) {
+        const float f = (float)fr[d];
+        M[4 * d + 0] = (float)((((-1. / 6.) * f + 0.5) * f - 0.5) * f + (1. / 6.));
+        M[4 * d + 1] = (float)(((0.5 * f - 1.0) * f) * f + (2. / 3.));
+        M[4 * d + 2] = (float)(((-0.5 * f + 0.5) * f + 0.5) * f + (1. / 6.));
+        M[4 * d + 3] = (float)((1. / 6.) * f * f * f);
+      }
+    } else {
+      compute_b_spline_cpu(fr, M, dM, order);
+    }
+    for (int i = 0; i < order; ++i)
+      for (int j = 0; j < order; ++j)
+        for (int k = 0; k < order; ++k) {
+          int x = u[0] - order + 1 + i, y = u[1] - order + 1 + j, z = u[2] - order + 1 + k;
+          x = ((x % K[0]) + K[0]) % K[0];
+          y = ((y % K[1]) + K[1]) % K[1];
+          z = ((z % K[2]) + K[2]) % K[2];
+          grid[x + (size_t)K1 * (y + (size_t)K2 * z)] += q[a] * M[i] * M[order + j] * M[2 * order + k];
+        }
+  }
+  return 0;
+}
+
+/* compute_forces (PmeRealSpace.C:347-425) on a plain wrapped grid; fractional-gradient forces */
+int oracle_pme_gather_cpu(int n, const double* p, const double* q, int K1, int K2, int K3, int order,
+                          const double* grid, double* force) {
+  double M[24], dM[24];
+  const int K[3] = {K1, K2, K3};
+  for (int a = 0; a < n; ++a) {
+    int u[3];
+    double fr[3];
+    for (int d = 0; d < 3; ++d) {
+      u[d] = (int)p[3 * a + d];
+      fr[d] = p[3 * a + d] - (double)u[d];
+    }
+    compute_b_spline_cpu(fr, M, dM, order);
+    double f1 = 0, f2 = 0, f3 = 0;
+    for (int i = 0; i < order; ++i)
+      for (int j = 0; j < order; ++j)
+        for (int k = 0; k < order; ++k) {
+          int x = u[0] - order + 1 + i, y = u[1] - order + 1 + j, z = u[2] - order + 1 + k;
+          x = ((x % K[0]) + K[0]) % K[0];
+          y = ((y % K[1]) + K[1]) % K[1];
+          z = ((z % K[2]) + K[2]) % K[2];
+          const double t = grid[x + (size_t)K1 * (y + (size_t)K2 * z)];
+          f1 -= K1 * dM[i] * M[order + j] * M[2 * order + k] * t;
+          f2 -= K2 * M[i] * dM[order + j] * M[2 * order + k] * t;
+          f3 -= K3 * M[i] * M[order + j] * dM[2 * order + k] * t;
+        }
+    force[3 * a] = q[a] * f1;
+    force[3 * a + 1] = q[a] * f2;
+    force[3 * a + 2] = q[a] * f3;
+  }
+  return 0;
+}
+
+/* B-spline partition of unity check helper: sum of weights and derivative weights (GPU recursion) */
+int oracle_bspline_gpu(double w, int order, double* th, double* dth) {
+  bspline_gpu(w, order, th, dth);
+  return 0;
+}

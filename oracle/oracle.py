@@ -1,0 +1,152 @@
+"""ORACLE - test infrastructure only (the checker).
+
+ctypes wrappers over liboracle.so (nb_oracle.c).  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module;
+the product library never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built: run python oracle/build.py")
+        _LIB = C.CDLL(path)
+        L = _LIB
+        L.oracle_nb_fp64.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, i64p, i32p, dp, C.c_double, C.c_double,
+                                     C.c_double, dp, dp, dp, dp, dp, i64p, C.c_int]
+        L.oracle_pairs_fp32.argtypes = [C.c_int, dp, i64p, i32p, dp, C.c_float, C.c_float, i64p, u64p, u64p,
+                                        C.c_void_p, C.c_int64]
+        L.oracle_pair_hash.restype = C.c_uint64
+        L.oracle_pair_hash.argtypes = [C.c_int64, C.c_int64, C.c_int64]
+        L.oracle_pme_spread_gpu.argtypes = [C.c_int, fp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
+        L.oracle_pme_gather_gpu.argtypes = [C.c_int, fp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_void_p, dp]
+        L.oracle_pme_scale_coordinates.argtypes = [C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
+        L.oracle_pme_spread_cpu.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
+        L.oracle_pme_gather_cpu.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.oracle_bspline_gpu.argtypes = [C.c_double, C.c_int, dp, dp]
+    return _LIB
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def nb_fp64(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff, switchdist, beta, nthreads=0):
+    """FP64 analytic oracle. Returns dict(f, fs, e_vdw, e_elec, e_slow, virial, virial_slow, npairs)."""
+    n = len(q_scaled)
+    ntypes = int(round(np.sqrt(np.asarray(A).size)))
+    f = np.zeros(3 * n)
+    fs = np.zeros(3 * n)
+    en = np.zeros(3)
+    vir = np.zeros(9)
+    virs = np.zeros(9)
+    npairs = np.zeros(1, np.int64)
+    lib().oracle_nb_fp64(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32),
+                         ntypes, _c(A, np.float64).reshape(-1), _c(B, np.float64).reshape(-1),
+                         _c(excl_ptr, np.int64), _c(excl_idx, np.int32), _c(box, np.float64), cutoff, switchdist,
+                         beta, f, fs, en, vir, virs, npairs, int(nthreads))
+    return dict(f=f.reshape(n, 3), fs=fs.reshape(n, 3), e_vdw=en[0], e_elec=en[1], e_slow=en[2],
+                virial=vir.reshape(3, 3), virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]))
+
+
+def pairs_fp32(pos, excl_ptr, excl_idx, box, cutoff2f, plcutoff2f, want_pairs=False, maxpairs=None):
+    """Canonical FP32 pair sets (see nb_oracle.c). Returns dict of counts, checksums, optional pairs."""
+    n = pos.shape[0]
+    out = np.zeros(4, np.int64)
+    sums = np.zeros(2, np.uint64)
+    xors = np.zeros(2, np.uint64)
+    buf = None
+    cap = 0
+    if want_pairs:
+        cap = int(maxpairs) if maxpairs else 200 * n
+        buf = np.zeros((cap, 2), np.int32)
+    lib().oracle_pairs_fp32(n, _c(pos, np.float64).reshape(-1), _c(excl_ptr, np.int64), _c(excl_idx, np.int32),
+                            _c(box, np.float64), np.float32(cutoff2f), np.float32(plcutoff2f), out, sums, xors,
+                            buf.ctypes.data if buf is not None else None, cap)
+    res = dict(n_cut=int(out[0]), n_list=int(out[1]), n_excl_list=int(out[2]),
+               sum_cut=int(sums[0]), xor_cut=int(xors[0]), sum_list=int(sums[1]), xor_list=int(xors[1]))
+    if want_pairs:
+        if out[3] < out[1]:
+            raise ValueError("pair buffer too small")
+        res["pairs"] = buf[: out[3]].copy()
+    return res
+
+
+def pair_hash(i, j, n):
+    """Vectorised oracle_pair_hash (splitmix64 of min*n+max) in numpy uint64 arithmetic."""
+    i = np.asarray(i, np.int64)
+    j = np.asarray(j, np.int64)
+    a = np.minimum(i, j).astype(np.uint64)
+    b = np.maximum(i, j).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        x = a * np.uint64(n) + b
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return x
+
+
+def pme_spread_gpu(atoms_f4, K, order):
+    K1, K2, K3 = K
+    g = np.zeros(K1 * K2 * K3)
+    lib().oracle_pme_spread_gpu(atoms_f4.shape[0], _c(atoms_f4, np.float32).reshape(-1), K1, K2, K3, order, g)
+    return g.reshape(K3, K2, K1)
+
+
+def pme_gather_gpu(atoms_f4, K, order, grid, recip=None):
+    K1, K2, K3 = K
+    n = atoms_f4.shape[0]
+    f = np.zeros(3 * n)
+    r = None if recip is None else _c(recip, np.float64).reshape(-1)
+    lib().oracle_pme_gather_gpu(n, _c(atoms_f4, np.float32).reshape(-1), K1, K2, K3, order,
+                                _c(grid, np.float64).reshape(-1), r.ctypes.data if r is not None else None, f)
+    return f.reshape(n, 3)
+
+
+def pme_scale_coordinates(pos, origin, recip, K, order):
+    n = pos.shape[0]
+    p = np.zeros(3 * n)
+    lib().oracle_pme_scale_coordinates(n, _c(pos, np.float64).reshape(-1), _c(origin, np.float64),
+                                       _c(recip, np.float64).reshape(-1), K[0], K[1], K[2], order, p)
+    return p.reshape(n, 3)
+
+
+def pme_spread_cpu(p, q, K, order):
+    K1, K2, K3 = K
+    g = np.zeros(K1 * K2 * K3)
+    lib().oracle_pme_spread_cpu(p.shape[0], _c(p, np.float64).reshape(-1), _c(q, np.float64), K1, K2, K3, order, g)
+    return g.reshape(K3, K2, K1)
+
+
+def pme_gather_cpu(p, q, K, order, grid):
+    K1, K2, K3 = K
+    n = p.shape[0]
+    f = np.zeros(3 * n)
+    lib().oracle_pme_gather_cpu(n, _c(p, np.float64).reshape(-1), _c(q, np.float64), K1, K2, K3, order,
+                                _c(grid, np.float64).reshape(-1), f)
+    return f.reshape(n, 3)
+
+
+def bspline_gpu(w, order):
+    th = np.zeros(order)
+    dth = np.zeros(order)
+    lib().oracle_bspline_gpu(float(w), order, th, dth)
+    return th, dth

@@ -1,0 +1,137 @@
+"""GPU parity of the nonbonded path (through the C ABI) against the oracle.
+
+Tolerances: forces (fast and slow separately) and energies within 1e-5
+relative RMS / relative error of the FP64 oracle evaluated at the same
+effective positions; the virial within 1e-5 of its Frobenius norm.  Pair
+sets, pair counts and exclusion counts are bit-exact against the oracle's
+canonical FP32 rule.
+"""
+import numpy as np
+import pytest
+
+from mi355nb import systems
+from mi355nb.compute import decode_pairs
+from nbcheck import gpu_compute, oracle_for, pair_set_checksums, rel_rms
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _small():
+    return systems.small_box(n_waters=300, n_chains=3, chain_len=20, box=22.0, seed=7)
+
+
+def _check_against_oracle(s, c, r, tol=TOL):
+    pos = c.effective_positions()
+    box = c.info()["box"]
+    o = oracle_for(s, pos, box)
+    ef = rel_rms(r["f"], o["f"])
+    es = rel_rms(r["fs"], o["fs"])
+    print(f"[{s.name}] force rel-RMS fast {ef:.2e} slow {es:.2e}")
+    assert ef < tol and es < tol
+    for k in ("e_vdw", "e_elec", "e_slow"):
+        err = abs(r[k] - o[k]) / max(abs(o[k]), 1e-12)
+        print(f"[{s.name}] {k} gpu {r[k]:.10e} oracle {o[k]:.10e} rel {err:.2e}")
+        assert err < tol, k
+    for k, ko in (("virial", "virial"), ("virial_slow", "virial_slow")):
+        err = np.linalg.norm(r[k] - o[ko]) / np.linalg.norm(o[ko])
+        print(f"[{s.name}] {k} rel {err:.2e}")
+        assert err < tol, k
+    return o
+
+
+def test_small_box_forces_energies_virial(torch_cuda):
+    s = _small()
+    c, r = gpu_compute(s)
+    o = _check_against_oracle(s, c, r)
+    assert r["stat"]["numPairsCutoff"] == o["npairs"] or abs(r["stat"]["numPairsCutoff"] - o["npairs"]) <= 2
+    c.close()
+
+
+def test_small_box_pair_sets_bit_exact(torch_cuda):
+    s = _small()
+    c, r = gpu_compute(s)
+    pos = c.effective_positions()
+    box = np.asarray(c.info()["box"])
+    p = s.params
+    cut2 = np.float32(p.cutoff * p.cutoff)
+    pl2 = np.float32(p.pairlistdist * p.pairlistdist)
+    o = orc.pairs_fp32(pos, s.excl_ptr, s.excl_idx, box, cut2, pl2, want_pairs=True)
+    st = r["stat"]
+    assert st["numPairsCutoff"] == o["n_cut"]
+    assert st["numPairsList"] == o["n_list"]
+    assert st["numExcluded"] == o["n_excl_list"]
+    i, j = decode_pairs(c.tile_lists())
+    assert i.size == o["n_list"]
+    got = set(zip(i.tolist(), j.tolist()))
+    want = set(map(tuple, o["pairs"].tolist()))
+    assert got == want
+    c.close()
+
+
+def test_pairlist_replay_matches_pairlist_step(torch_cuda):
+    s = _small()
+    c, r0 = gpu_compute(s)
+    c.launch_work(True, True, True, do_pairlist=False)
+    r1 = c.finish()
+    assert rel_rms(r1["f"], r0["f"]) < 1e-6
+    assert rel_rms(r1["fs"], r0["fs"]) < 1e-6
+    assert abs(r1["e_elec"] - r0["e_elec"]) <= 1e-6 * abs(r0["e_elec"])
+    c.close()
+
+
+def test_flag_combinations(torch_cuda):
+    s = _small()
+    c, ref = gpu_compute(s)
+    for e in (0, 1):
+        for v in (0, 1):
+            for sl in (0, 1):
+                c.launch_work(e, v, sl, do_pairlist=False)
+                r = c.finish()
+                assert rel_rms(r["f"], ref["f"]) < 1e-6, (e, v, sl)
+                if sl:
+                    assert rel_rms(r["fs"], ref["fs"]) < 1e-6
+                if e:
+                    assert abs(r["e_vdw"] - ref["e_vdw"]) <= 1e-6 * abs(ref["e_vdw"])
+                if v:
+                    assert np.linalg.norm(r["virial"] - ref["virial"]) <= 1e-6 * np.linalg.norm(ref["virial"])
+    c.close()
+
+
+def test_momentum_conservation(torch_cuda):
+    s = _small()
+    c, r = gpu_compute(s)
+    scale = np.abs(r["f"]).sum()
+    assert np.abs(r["f"].sum(0)).max() < 1e-5 * scale
+    assert np.abs(r["fs"].sum(0)).max() < 1e-5 * np.abs(r["fs"]).sum()
+    c.close()
+
+
+def test_water_box_9k(torch_cuda):
+    s = systems.water_box_9k()
+    c, r = gpu_compute(s)
+    _check_against_oracle(s, c, r)
+    pos = c.effective_positions()
+    box = np.asarray(c.info()["box"])
+    p = s.params
+    o = orc.pairs_fp32(pos, s.excl_ptr, s.excl_idx, box, np.float32(p.cutoff ** 2), np.float32(p.pairlistdist ** 2))
+    st = r["stat"]
+    assert (st["numPairsCutoff"], st["numPairsList"], st["numExcluded"]) == (o["n_cut"], o["n_list"], o["n_excl_list"])
+    i, j = decode_pairs(c.tile_lists())
+    assert pair_set_checksums(i, j, s.natoms) == (o["sum_list"], o["xor_list"])
+    c.close()
+
+
+def test_update_positions_then_rebuild(torch_cuda):
+    s = _small()
+    c, r0 = gpu_compute(s)
+    rng = np.random.default_rng(3)
+    pos2 = s.pos + rng.uniform(-0.2, 0.2, s.pos.shape)
+    c.update_positions(pos2)
+    c.launch_work(True, True, True, do_pairlist=True)
+    r1 = c.finish()
+    s2 = systems.System(**{**s.__dict__, "pos": pos2})
+    _check_against_oracle(s2, c, r1)
+    c.close()
