@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--pairlist-every", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
@@ -81,6 +83,14 @@ def main():
     atoms = torch.empty((n, 4), dtype=torch.float32, device=dev)
     grid = torch.zeros(K[0] * K[1] * K[2], dtype=torch.float32, device=dev)
     fpme = torch.empty((n, 4), dtype=torch.float32, device=dev)
+
+    if args.calibrate_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ev = {}
 

@@ -70,6 +70,7 @@ struct mi355nb_compute {
   DevBuf<int> jCand, jAct;
   DevBuf<TileExcl> exCand, exAct;
   DevBuf<int> counts, offsets, jtileFlags, listDepth;
+  DevBuf<int4> listStat;
   DevBuf<int> sortKeys, sortKeysOut, sortVals, sortValsOut, newStart;
   DevBuf<char> temp;
   size_t tempBytes = 0;
@@ -113,7 +114,7 @@ struct mi355nb_compute {
     counts.release(); offsets.release(); jtileFlags.release(); listDepth.release();
     sortKeys.release(); sortKeysOut.release(); sortVals.release(); sortValsOut.release(); newStart.release();
     temp.release(); fsoa.release(); tlve.release(); partials.release(); ve.release(); stat.release();
-    numCand.release();
+    numCand.release(); listStat.release();
     recycleEvents();
     for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
     evPool.clear();
@@ -295,6 +296,7 @@ int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], cons
   MI355NB_HIP(c->counts.reserve(nl + 1));
   MI355NB_HIP(c->offsets.reserve(nl + 1));
   MI355NB_HIP(c->listDepth.reserve(nl + 1));
+  MI355NB_HIP(c->listStat.reserve(nl + 1));
   MI355NB_HIP(c->sortKeys.reserve(nl + 1));
   MI355NB_HIP(c->sortKeysOut.reserve(nl + 1));
   MI355NB_HIP(c->sortVals.reserve(nl + 1));
@@ -368,7 +370,7 @@ int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int d
   a.exclBits = c->bits.ptr;
   a.jtileFlags = c->jtileFlags.ptr;
   a.listDepth = c->listDepth.ptr;
-  a.stat = c->stat.ptr;
+  a.listStat = c->listStat.ptr;
   a.fsoa = c->fsoa.ptr;
   a.stride = n;
   a.virialEnergy = c->tlve.ptr;
@@ -383,7 +385,6 @@ int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int d
     MI355NB_HIP(scanExclusive(c->temp.ptr, tb, c->counts.ptr, c->offsets.ptr, nl, s));
     MI355NB_HIP(launchBuildJtiles(1, nl, c->listsCand.ptr, c->patches.ptr, c->bbox.ptr, cull * cull,
                                   c->counts.ptr, c->offsets.ptr, c->jCand.ptr, s));
-    MI355NB_HIP(launchClearStat(c->stat.ptr, s));
     a.numLists = c->numCand.ptr;
     a.tileLists = c->listsCand.ptr;
     a.tileJatomStart = c->jCand.ptr;
@@ -399,6 +400,7 @@ int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int d
       c->evPairlist.emplace_back(e0, e1);
     }
     // refine: depth-ordered, compacted lists
+    MI355NB_HIP(launchReduceTileListStat(nl, c->listStat.ptr, c->stat.ptr, s));
     MI355NB_HIP(launchSetupSort(nl, c->listDepth.ptr, c->sortKeys.ptr, c->sortVals.ptr, s));
     tb = c->tempBytes;
     MI355NB_HIP(sortDescending(c->temp.ptr, tb, c->sortKeys.ptr, c->sortKeysOut.ptr, c->sortVals.ptr,

@@ -25,7 +25,7 @@ struct ForceArgsHost {
   const unsigned* exclBits;
   int* jtileFlags;
   int* listDepth;
-  TileListStat* stat;
+  int4* listStat;
   float* fsoa;
   int stride;
   TileListVirialEnergy* virialEnergy;
@@ -47,7 +47,7 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& a, bool doEne
                                 bool doSlow, bool doPairlist, hipStream_t s);
 hipError_t launchTransposeForces(int n, int stride, float* fsoa, F4* f, F4* fs, bool doSlow, hipStream_t s);
 hipError_t launchFill(size_t n, float* p, float v, hipStream_t s);
-hipError_t launchClearStat(TileListStat* st, hipStream_t s);
+hipError_t launchReduceTileListStat(int numLists, const int4* listStat, TileListStat* st, hipStream_t s);
 int reducePartialBlocks(int nSlots, int maxLists);
 hipError_t launchReduceVirialEnergy(int nSlots, int maxLists, const F4* xyzq, const F4* f, const F4* fs,
                                     const int* numLists, const TileListVirialEnergy* tlve, bool doEnergy,

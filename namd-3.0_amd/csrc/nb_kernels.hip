@@ -27,6 +27,11 @@ __device__ __forceinline__ float waveSum(float v) {
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+__device__ __forceinline__ double waveSumD(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 __device__ __forceinline__ int waveSumInt(int v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -103,8 +108,20 @@ __global__ void buildJtilesKernel(int numLists, TileList* __restrict__ lists,
 // (cudaNBForceMagCalc_VdwEnergySwitch_PMEC1, src/CudaComputeNonbondedInteractions.h:37-214).
 // f and fSlow are (1/r) dE/dr, so F_i = f * (x_j - x_i).
 //
-// erfc uses Abramowitz & Stegun 7.1.26 (Hastings): erfc(x) = t*P(t)*exp(-x^2),
-// t = 1/(1+p x), |error| <= 1.5e-7, sharing exp(-x^2) with the gradient term.
+// The slow force is written through h(a) = 1 - erfc(a) - (2/sqrt(pi)) a e^{-a^2}
+// (a = beta r), which is NAMD's (1 - corr_grad):  fSlow = qq (h/r^3 - 1/rc^3).
+//  * a >= 0.8: erfc from Abramowitz & Stegun 7.1.26 (Hastings),
+//    erfc(x) = t P(t) e^{-x^2}, t = 1/(1+p x), |error| <= 1.5e-7, sharing
+//    e^{-x^2} with the gradient term; h >= 0.266 there, so the absolute
+//    error stays below 6e-7 relative.
+//  * a < 0.8: h(a) = a^3 H(a^2) with the Taylor series
+//    H(u) = (4/sqrt(pi)) sum_n (-u)^n / (n! (2n+3)), n = 0..9 (truncation
+//    below 4e-9), which avoids the cancellation of 1 - erfc - ... at short
+//    range.
+// Energies (only on energy steps) use the cancellation-free form
+//    E_fast = qq (1 - x)^2 (1 + x/2) / r, x = r/rc   (= qq (1/r - 3/(2rc) + r^2/(2rc^3)))
+//    E_slow = qq erfc(a)/r - E_fast
+// with the device library's erfcf, and accumulate per lane in FP64.
 // ---------------------------------------------------------------------------
 #define AS_P 0.3275911f
 #define AS_A1 0.254829592f
@@ -113,11 +130,29 @@ __global__ void buildJtilesKernel(int numLists, TileList* __restrict__ lists,
 #define AS_A4 -1.453152027f
 #define AS_A5 1.061405429f
 #define LOG2E_F 1.4426950408889634f
+#define SERIES_A0 0.8f
 
-template <bool doEnergy, bool doSlow>
+constexpr double FOUR_OVER_SQRT_PI = 2.2567583341910251478;
+constexpr float H0 = (float)(FOUR_OVER_SQRT_PI / 3.0);
+constexpr float H1 = (float)(-FOUR_OVER_SQRT_PI / 5.0);
+constexpr float H2 = (float)(FOUR_OVER_SQRT_PI / (2.0 * 7.0));
+constexpr float H3 = (float)(-FOUR_OVER_SQRT_PI / (6.0 * 9.0));
+constexpr float H4 = (float)(FOUR_OVER_SQRT_PI / (24.0 * 11.0));
+constexpr float H5 = (float)(-FOUR_OVER_SQRT_PI / (120.0 * 13.0));
+constexpr float H6 = (float)(FOUR_OVER_SQRT_PI / (720.0 * 15.0));
+constexpr float H7 = (float)(-FOUR_OVER_SQRT_PI / (5040.0 * 17.0));
+constexpr float H8 = (float)(FOUR_OVER_SQRT_PI / (40320.0 * 19.0));
+constexpr float H9 = (float)(-FOUR_OVER_SQRT_PI / (362880.0 * 21.0));
+
+template <bool doEnergy>
+struct EnergyAcc { float v = 0.f; };
+template <>
+struct EnergyAcc<true> { double v = 0.0; };
+
+template <bool doEnergy, bool doSlow, typename Acc>
 __device__ __forceinline__ void pairForce(const float r2, const float qq, const float A, const float B,
                                           const NbConstants& c, float& f, float& fSlow,
-                                          float& eVdw, float& eElec, float& eSlow) {
+                                          Acc& eVdw, Acc& eElec, Acc& eSlow) {
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float rinv2 = rinv * rinv;
   const float rinv3 = rinv * rinv2;
@@ -133,28 +168,34 @@ __device__ __forceinline__ void pairForce(const float r2, const float qq, const 
   const float s = sw ? delta_r * delta_r * (c.lj_0 + c.lj_1 * r2) : 1.0f;
   const float ds_r = sw ? delta_r * (c.lj_3 + c.lj_2 * r2) : 0.0f;
   const float f_vdw = w * ds_r + dw_r * s;
-  if (doEnergy) eVdw += w * s;
 
   // fast electrostatics (PMEFast_C1, :107-123)
   const float elec_fast = -1.0f * rinv3 + c.e_0;
   f = qq * elec_fast + f_vdw;
-  if (doEnergy || doSlow) {
-    const float slow_energy = 0.5f * c.e_2 * (3.0f - r2 * c.e_1);
-    if (doEnergy) eElec += qq * (rinv - slow_energy);
-    if (doSlow) {
-      // slow electrostatics (PMESlow_C1, :131-159)
-      const float r = r2 * rinv;
-      const float a = r * c.ewald_0;
+  const float r = r2 * rinv;
+  const float a = r * c.ewald_0;
+  if (doSlow) {
+    // slow electrostatics (PMESlow_C1, :131-159) through h(a)
+    float h;
+    if (a < SERIES_A0) {
+      const float u = a * a;
+      const float H = H0 + u * (H1 + u * (H2 + u * (H3 + u * (H4 + u * (H5 + u * (H6 + u * (H7 + u * (H8 + u * H9))))))));
+      h = a * u * H;
+    } else {
       const float ex = __builtin_amdgcn_exp2f(-(a * a) * LOG2E_F);
       const float t = __builtin_amdgcn_rcpf(1.0f + AS_P * a);
       const float poly = t * (AS_A1 + t * (AS_A2 + t * (AS_A3 + t * (AS_A4 + t * AS_A5))));
-      const float erfc_a = poly * ex;
-      const float corr_grad = erfc_a + c.ewald_1 * r * ex;
-      const float elec_slow = -1.0f * c.e_0 * r;
-      const float scor_grad = (elec_slow - (corr_grad - 1.0f) * rinv2) * rinv;
-      fSlow = qq * scor_grad;
-      if (doEnergy) eSlow += qq * (slow_energy + (erfc_a - 1.0f) * rinv);
+      h = (1.0f - poly * ex) - c.ewald_1 * r * ex;
     }
+    fSlow = qq * (h * rinv3 - c.e_0);
+  }
+  if (doEnergy) {
+    const float x = r * c.e_2;
+    const float omx = 1.0f - x;
+    const float fastE = omx * omx * (1.0f + 0.5f * x) * rinv;
+    eVdw.v += w * s;
+    eElec.v += qq * fastE;
+    if (doSlow) eSlow.v += qq * (erfcf(a) * rinv - fastE);
   }
 }
 
@@ -191,7 +232,7 @@ struct ForceArgs {
   const unsigned* exclBits;
   int* jtileFlags;
   int* listDepth;
-  TileListStat* stat;
+  int4* listStat;                    // per list: depth, pairs < cutoff, pairs < plcutoff, excluded
   // outputs
   float* fsoa;                       // 6 * stride floats: fx fy fz fsx fsy fsz
   int stride;
@@ -223,7 +264,7 @@ nonbondedForceKernel(const ForceArgs args) {
 
   float ifx = 0.f, ify = 0.f, ifz = 0.f;
   float ifsx = 0.f, ifsy = 0.f, ifsz = 0.f;
-  float eVdw = 0.f, eElec = 0.f, eSlow = 0.f;
+  EnergyAcc<doEnergy> eVdw, eElec, eSlow;
 
   // pairlist-step state
   bool iValid = true;
@@ -350,12 +391,7 @@ nonbondedForceKernel(const ForceArgs args) {
     const int e = waveSumInt(nExcl);
     if (lane == 0) {
       args.listDepth[list] = depth;
-      if (depth > 0) atomicAdd(&args.stat->numTileLists, 1);
-      atomicAdd(&args.stat->numJtiles, depth);
-      atomicMax(&args.stat->maxDepth, depth);
-      atomicAdd(&args.stat->numExcluded, e);
-      atomicAdd(&args.stat->numPairsCutoff, (unsigned long long)a);
-      atomicAdd(&args.stat->numPairsList, (unsigned long long)b);
+      args.listStat[list] = make_int4(depth, a, b, e);
     }
   }
   if (doVirial) {
@@ -372,7 +408,7 @@ nonbondedForceKernel(const ForceArgs args) {
     }
   }
   if (doEnergy) {
-    const float a = waveSum(eVdw), b = waveSum(eElec), e = doSlow ? waveSum(eSlow) : 0.f;
+    const double a = waveSumD(eVdw.v), b = waveSumD(eElec.v), e = doSlow ? waveSumD(eSlow.v) : 0.0;
     if (lane == 0) {
       TileListVirialEnergy& ve = args.virialEnergy[list];
       ve.energyVdw = a; ve.energyElec = b; ve.energySlow = e;
@@ -439,10 +475,49 @@ __global__ void fillKernel(size_t n, float* __restrict__ p, float v) {
   if (k < n) p[k] = v;
 }
 
-__global__ void clearStatKernel(TileListStat* s) {
+// TileListStat from the per-list records (one workgroup; replaces NAMD's
+// per-warp atomics on the stat record, which serialise on one address).
+constexpr int STAT_THREADS = 1024;
+
+__global__ void __launch_bounds__(STAT_THREADS)
+reduceTileListStatKernel(int numLists, const int4* __restrict__ listStat, TileListStat* __restrict__ s) {
+  __shared__ long long sh[5][STAT_THREADS / 64];
+  long long nl = 0, nj = 0, nc = 0, np = 0, ne = 0;
+  int mx = 0;
+  for (int l = threadIdx.x; l < numLists; l += STAT_THREADS) {
+    const int4 v = listStat[l];
+    nl += v.x > 0 ? 1 : 0;
+    nj += v.x;
+    mx = max(mx, v.x);
+    nc += v.y;
+    np += v.z;
+    ne += v.w;
+  }
+  long long vals[5] = {nl, nj, nc, np, ne};
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    long long v = vals[k];
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) sh[k][w] = v;
+  }
+  for (int o = 32; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+  __shared__ int shm[STAT_THREADS / 64];
+  if (lane == 0) shm[w] = mx;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    s->numTileLists = 0; s->numJtiles = 0; s->numExcluded = 0; s->maxDepth = 0;
-    s->numPairsCutoff = 0ull; s->numPairsList = 0ull;
+    long long t[5] = {0, 0, 0, 0, 0};
+    int m = 0;
+    for (int k = 0; k < STAT_THREADS / 64; ++k) {
+      for (int q = 0; q < 5; ++q) t[q] += sh[q][k];
+      m = max(m, shm[k]);
+    }
+    s->numTileLists = (int)t[0];
+    s->numJtiles = (int)t[1];
+    s->maxDepth = m;
+    s->numPairsCutoff = (unsigned long long)t[2];
+    s->numPairsList = (unsigned long long)t[3];
+    s->numExcluded = (int)t[4];
   }
 }
 
@@ -617,7 +692,7 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
   a.exclBits = h.exclBits;
   a.jtileFlags = h.jtileFlags;
   a.listDepth = h.listDepth;
-  a.stat = h.stat;
+  a.listStat = h.listStat;
   a.fsoa = h.fsoa;
   a.stride = h.stride;
   a.virialEnergy = h.virialEnergy;
@@ -650,8 +725,8 @@ hipError_t launchFill(size_t n, float* p, float v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launchClearStat(TileListStat* st, hipStream_t s) {
-  hipLaunchKernelGGL(clearStatKernel, dim3(1), dim3(64), 0, s, st);
+hipError_t launchReduceTileListStat(int numLists, const int4* listStat, TileListStat* st, hipStream_t s) {
+  hipLaunchKernelGGL(reduceTileListStatKernel, dim3(1), dim3(STAT_THREADS), 0, s, numLists, listStat, st);
   return hipGetLastError();
 }
 

@@ -90,12 +90,13 @@ struct TileListStat {
 };
 
 // TileListVirialEnergy (src/CudaTileListKernel.h:72-94): the per-list partials
-// that the FP64 reduction consumes.  Energies are FP32 per list.
+// that the FP64 reduction consumes.  Per-list i-force sums are FP32 (as in
+// NAMD); per-list energies are FP64 (accumulated per lane in FP64).
 struct TileListVirialEnergy {
   float shx, shy, shz;
   float forcex, forcey, forcez;
   float forceSlowx, forceSlowy, forceSlowz;
-  float energyVdw, energyElec, energySlow;
+  double energyVdw, energyElec, energySlow;
 };
 
 // VirialEnergy (src/CudaTileListKernel.h:96-117), same field order.

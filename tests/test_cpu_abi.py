@@ -1,0 +1,106 @@
+"""The C-ABI library loads without a GPU, exports every symbol the header
+declares, and its host-only entry points agree with what the reference
+fixes (no GPU)."""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from mi355nb import lib as L
+from mi355nb import systems
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "mi355nb.h")).read()
+    return sorted(set(re.findall(r"\b(mi355nb_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L.lib()  # loads (no GPU needed)
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (mi355nb_[a-z0-9_]+)", out))
+    missing = [s for s in _declared() if s not in exported]
+    assert not missing, missing
+    assert set(L.EXPORTED) <= exported
+    assert L.lib().mi355nb_abi_version() == 1
+
+
+def test_nonbonded_coef_matches_formulas():
+    rc, rs, beta = 12.0, 10.0, 0.257952
+    c = L.nonbonded_coef(rc, rs, beta, 1.0)
+    c1 = 1.0 / (rc * rc - rs * rs) ** 3
+    assert c.lj_4 == np.float32(rc * rc) and c.lj_5 == np.float32(rs * rs)
+    assert abs(c.lj_0 - (c1 * rc * rc - 3 * rs * rs * c1)) < 1e-6 * abs(c1 * rc * rc)
+    assert abs(c.lj_1 - 2 * c1) < 1e-6 * 2 * c1
+    assert abs(c.lj_2 + 12 * c1) < 1e-6 * 12 * c1
+    assert abs(c.lj_3 - 12 * c1 * rs * rs) < 1e-6 * 12 * c1 * rs * rs
+    assert abs(c.e_0 - rc ** -3) < 1e-6 * rc ** -3
+    assert abs(c.e_2 - 1 / rc) < 1e-7
+    assert abs(c.ewald_1 - 2 * beta / math.sqrt(math.pi)) < 1e-6
+    assert c.slowScale == 1.0
+    assert L.lib().mi355nb_get_do_table(1, 1, 1) == 0
+
+
+def test_lj_mix_closed_form():
+    sig = np.array([3.0, 3.5, 2.5])
+    eps = np.array([0.1, 0.2, 0.05])
+    A, B = L.lj_mix(sig, eps)
+    for i in range(3):
+        for j in range(3):
+            s = 0.5 * (sig[i] + sig[j])
+            e = math.sqrt(eps[i] * eps[j])
+            assert abs(B[i, j] - 4 * e * s ** 6) < 1e-12 * B[i, j]
+            assert abs(A[i, j] - 4 * e * s ** 12) < 1e-12 * A[i, j]
+            # LJ minimum at r = 2^(1/6) sigma with depth -eps
+            r = 2 ** (1 / 6) * s
+            assert abs(A[i, j] / r ** 12 - B[i, j] / r ** 6 + e) < 1e-12
+    Ag, Bg = L.lj_mix(sig, eps, geometric=True)
+    assert abs(Bg[0, 1] - 4 * math.sqrt(0.02) * (3.0 * 3.5) ** 3) < 1e-9 * Bg[0, 1]
+
+
+def test_exclusion_bit_table_matches_lists():
+    s = systems.small_box(n_waters=40, n_chains=2, chain_len=9, box=20.0, seed=3)
+    sm, bits, nlists = L.build_exclusions(s.natoms, s.excl_ptr, s.excl_idx)
+    # waters (3 lists) + chain interior/ends: a handful of unique lists
+    assert 3 <= nlists <= 12
+
+    def excluded(i, j):
+        start, md = sm[i]
+        d = j - i
+        if abs(d) > md:
+            return False
+        b = start + d
+        return bool((bits[b >> 5] >> (b & 31)) & 1)
+
+    ex = set(map(tuple, s.exclusion_pairs().tolist()))
+    n = s.natoms
+    for i in range(n):
+        assert excluded(i, i)  # self always in the list
+        for j in range(max(0, i - 6), min(n, i + 7)):
+            if i != j:
+                assert excluded(i, j) == ((min(i, j), max(i, j)) in ex), (i, j)
+
+
+def test_tile_excl_to_namd_is_the_rotation():
+    rng = np.random.default_rng(0)
+    m = rng.integers(0, 2 ** 32, size=(5, 32), dtype=np.uint64).astype(np.uint32)
+    r = L.tile_excl_to_namd(m)
+    for t in range(5):
+        for i in range(32):
+            for k in range(32):
+                j = (i + k) & 31
+                assert ((int(r[t, i]) >> k) & 1) == ((int(m[t, i]) >> j) & 1)
+
+
+def test_compute_create_without_gpu_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(L.Mi355nbError):
+        from mi355nb.compute import ComputeNonbonded
+        ComputeNonbonded()

@@ -31,8 +31,14 @@ def _check_against_oracle(s, c, r, tol=TOL):
     es = rel_rms(r["fs"], o["fs"])
     print(f"[{s.name}] force rel-RMS fast {ef:.2e} slow {es:.2e}")
     assert ef < tol and es < tol
+    # The C1 split makes e_elec and e_slow a near-cancelling pair in dense neutral
+    # boxes (9k water: e_slow = +83 from terms of magnitude ~1e2 each over 1e7
+    # pairs), so each electrostatic part is held to 1e-5 of the real-space
+    # Coulomb energy e_elec + e_slow (the physical quantity); e_vdw to itself.
+    coul = abs(o["e_elec"] + o["e_slow"])
     for k in ("e_vdw", "e_elec", "e_slow"):
-        err = abs(r[k] - o[k]) / max(abs(o[k]), 1e-12)
+        scale = abs(o[k]) if k == "e_vdw" else max(abs(o[k]), coul)
+        err = abs(r[k] - o[k]) / max(scale, 1e-12)
         print(f"[{s.name}] {k} gpu {r[k]:.10e} oracle {o[k]:.10e} rel {err:.2e}")
         assert err < tol, k
     for k, ko in (("virial", "virial"), ("virial_slow", "virial_slow")):
@@ -134,4 +140,22 @@ def test_update_positions_then_rebuild(torch_cuda):
     r1 = c.finish()
     s2 = systems.System(**{**s.__dict__, "pos": pos2})
     _check_against_oracle(s2, c, r1)
+    c.close()
+
+
+def test_solvated_96k_full_size(torch_cuda):
+    """The benchmark workload itself: bit-exact pair/exclusion counts and set
+    checksums against the oracle's canonical rule; forces, energies, virial
+    against the FP64 oracle."""
+    s = systems.solvated_96k()
+    c, r = gpu_compute(s)
+    _check_against_oracle(s, c, r)
+    pos = c.effective_positions()
+    box = np.asarray(c.info()["box"])
+    p = s.params
+    o = orc.pairs_fp32(pos, s.excl_ptr, s.excl_idx, box, np.float32(p.cutoff ** 2), np.float32(p.pairlistdist ** 2))
+    st = r["stat"]
+    assert (st["numPairsCutoff"], st["numPairsList"], st["numExcluded"]) == (o["n_cut"], o["n_list"], o["n_excl_list"])
+    i, j = decode_pairs(c.tile_lists())
+    assert pair_set_checksums(i, j, s.natoms) == (o["sum_list"], o["xor_list"])
     c.close()
