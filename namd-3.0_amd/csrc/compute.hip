@@ -121,6 +121,17 @@ struct mi355nb_compute {
   }
 };
 
+// The FP32 r^2 (one product, two fused multiply-adds) is within 3 * 2^-24
+// relative of the exact value; the band [lo, hi] is 4e-7 relative wide on
+// each side, rounded outward.
+static CutTest makeCutTest(double c) {
+  CutTest t;
+  t.c2 = c * c;
+  t.lo = std::nextafter((float)(t.c2 * (1.0 - 4e-7)), 0.0f);
+  t.hi = std::nextafter((float)(t.c2 * (1.0 + 4e-7)), 1e30f);
+  return t;
+}
+
 extern "C" {
 
 const char* mi355nb_last_error(void) { return g_lastError.c_str(); }
@@ -352,8 +363,8 @@ int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int d
   MI355NB_HIP(hipSetDevice(c->device));
   const int n = c->L.atomStorageSize;
   const int nl = c->numCandLists;
-  const float cutoff2 = (float)(c->p.cutoff * c->p.cutoff);
-  const float plcutoff2 = (float)(c->p.pairlistdist * c->p.pairlistdist);
+  const CutTest cut = makeCutTest(c->p.cutoff);
+  const CutTest plcut = makeCutTest(c->p.pairlistdist);
   hipStream_t s = c->stream;
 
   ForceArgsHost a;
@@ -362,8 +373,8 @@ int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int d
   a.vdwWidth = c->ntypes;
   a.vdwCoef = c->vdwCoef.ptr;
   a.c = c->c;
-  a.cutoff2 = cutoff2;
-  a.plcutoff2 = plcutoff2;
+  a.cut = cut;
+  a.plcut = plcut;
   a.patches = c->patches.ptr;
   a.atomIndex = c->atomIndex.ptr;
   a.exclStartMaxdiff = c->excl.ptr;

@@ -17,8 +17,8 @@ struct ForceArgsHost {
   int vdwWidth;
   const float* vdwCoef;   // float2 pairs (A, B)
   NbConstants c;
-  float cutoff2;
-  float plcutoff2;
+  CutTest cut;
+  CutTest plcut;
   const PatchRecord* patches;
   const int* atomIndex;
   const I2* exclStartMaxdiff;

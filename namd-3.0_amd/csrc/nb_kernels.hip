@@ -199,6 +199,15 @@ __device__ __forceinline__ void pairForce(const float r2, const float qq, const 
   }
 }
 
+// r^2 <= c2 exactly; dx, dy, dz are exact (multiples of 2^-17 A below 128 A),
+// so their FP64 squares and sum are exact.
+__device__ __forceinline__ bool within(float dx, float dy, float dz, float r2, const CutTest& t) {
+  if (r2 < t.lo) return true;
+  if (r2 > t.hi) return false;
+  const double x = dx, y = dy, z = dz;
+  return x * x + y * y + z * z <= t.c2;
+}
+
 // ---------------------------------------------------------------------------
 // the pair kernel
 //
@@ -223,9 +232,9 @@ struct ForceArgs {
   int vdwWidth;
   const float2* vdwCoef;
   NbConstants c;
-  float cutoff2;
+  CutTest cut;
   // pairlist
-  float plcutoff2;
+  CutTest plcut;
   const PatchRecord* patches;
   const int* atomIndex;
   const I2* exclStartMaxdiff;
@@ -304,7 +313,7 @@ nonbondedForceKernel(const ForceArgs args) {
       const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
       bool active;
       if (doPairlist) {
-        bool inList = iValid && (jatomStart + j < jEnd) && (!selfTile || j > i) && (r2 < args.plcutoff2);
+        bool inList = iValid && (jatomStart + j < jEnd) && (!selfTile || j > i) && within(dx, dy, dz, r2, args.plcut);
         bool excluded = false;
         if (inList) {
           const int diff = s_index[j] - iIndex;
@@ -316,10 +325,10 @@ nonbondedForceKernel(const ForceArgs args) {
         if (inList && excluded) ++nExcl;
         inList = inList && !excluded;
         if (inList) { mask |= 1u << j; ++nPairsList; }
-        active = inList && (r2 < args.cutoff2);
+        active = inList && within(dx, dy, dz, r2, args.cut);
         if (active) ++nPairsCut;
       } else {
-        active = ((mask >> j) & 1u) && (r2 < args.cutoff2);
+        active = ((mask >> j) & 1u) && within(dx, dy, dz, r2, args.cut);
       }
       if (active) {
         const float2 ab = args.vdwCoef[vdwRow + s_type[j]];
@@ -684,8 +693,8 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
   a.vdwWidth = h.vdwWidth;
   a.vdwCoef = reinterpret_cast<const float2*>(h.vdwCoef);
   a.c = h.c;
-  a.cutoff2 = h.cutoff2;
-  a.plcutoff2 = h.plcutoff2;
+  a.cut = h.cut;
+  a.plcut = h.plcut;
   a.patches = h.patches;
   a.atomIndex = h.atomIndex;
   a.exclStartMaxdiff = h.exclStartMaxdiff;

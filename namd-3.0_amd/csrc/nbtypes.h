@@ -54,6 +54,15 @@ struct ComputeRecord {
   int self;          // 1 for a self compute (same patch, zero shift)
 };
 
+// Inclusive cutoff test r^2 <= c2 on exact displacements (NAMD's CPU rule,
+// src/ComputeNonbondedInl.h:80-117): FP32 r^2 decides outside [lo, hi], a
+// band wider than the FP32 rounding of r^2; inside it the exact FP64 r^2 of
+// the (exact) FP32 displacement decides.
+struct CutTest {
+  float lo, hi;
+  double c2;
+};
+
 // TileList (src/CudaTileListKernel.h:10-21).
 struct TileList {
   int iatomStart;

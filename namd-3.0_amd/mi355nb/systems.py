@@ -127,19 +127,52 @@ def _random_unit(rng, n):
     return v / np.linalg.norm(v, axis=1, keepdims=True)
 
 
-def _chain_walk(rng, start, nbeads, box, bond=1.5, min_cos=-0.5, avoid=2.2, tries=64):
-    """One persistent random walk (bond angle >= 120 deg) with soft self-avoidance."""
+class _BeadGrid:
+    """Spatial hash of placed beads (cell = avoidance radius) for self- and
+    cross-chain avoidance."""
+
+    def __init__(self, box, cell):
+        self.box = box
+        self.n = np.maximum(1, np.floor(box / cell).astype(int))
+        self.cell = box / self.n
+        self.cells = {}
+
+    def _key(self, p):
+        g = np.floor(np.mod(p, self.box) / self.cell).astype(int) % self.n
+        return int(g[0]), int(g[1]), int(g[2])
+
+    def add(self, p, tag):
+        self.cells.setdefault(self._key(p), []).append((p.copy(), tag))
+
+    def near(self, p, skip):
+        gx, gy, gz = self._key(p)
+        out = []
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    k = ((gx + dx) % self.n[0], (gy + dy) % self.n[1], (gz + dz) % self.n[2])
+                    for q, tag in self.cells.get(k, ()):
+                        if tag not in skip:
+                            out.append(q)
+        return np.array(out) if out else np.zeros((0, 3))
+
+
+def _chain_walk(rng, start, nbeads, box, grid, chain_id, bond=1.5, min_cos=-0.5, avoid=2.2, tries=64):
+    """One persistent random walk (bond angle >= 120 deg) avoiding earlier beads
+    of this and previous chains (soft: the farthest candidate wins if none is clear)."""
     pts = np.empty((nbeads, 3))
     pts[0] = start
     prev_dir = _random_unit(rng, 1)[0]
+    grid.add(pts[0], (chain_id, 0))
     for k in range(1, nbeads):
         cand = _random_unit(rng, tries)
         # bond angle >= 120 deg  <=>  dot(new, prev) >= cos(60 deg) = 0.5
         ok = cand @ prev_dir >= -min_cos if k > 1 else np.ones(tries, bool)
         cand = cand[ok] if ok.any() else cand
         newp = pts[k - 1] + bond * cand
-        if k > 3:
-            d = newp[:, None, :] - pts[None, : k - 2, :]
+        near = grid.near(pts[k - 1], skip={(chain_id, k - 1), (chain_id, k - 2)})
+        if near.shape[0]:
+            d = newp[:, None, :] - near[None, :, :]
             d -= box * np.round(d / box)
             dmin = np.sqrt((d * d).sum(-1)).min(axis=1)
             good = np.nonzero(dmin >= avoid)[0]
@@ -148,12 +181,39 @@ def _chain_walk(rng, start, nbeads, box, bond=1.5, min_cos=-0.5, avoid=2.2, trie
             pick = 0
         pts[k] = newp[pick]
         prev_dir = cand[pick]
+        grid.add(pts[k], (chain_id, k))
     return pts
+
+
+def _relax_contacts(rng, wat, base, bead_pos, boxv, jitter, dmin, iters=60):
+    """Re-draw the orientation and jitter of waters that have an intermolecular
+    contact closer than dmin (to another water or a bead), a few rounds; the
+    lattice + jitter scheme is kept, only the random draws are repeated."""
+    if dmin <= 0 or wat.shape[0] == 0:
+        return wat
+    from scipy.spatial import cKDTree
+    nw = wat.shape[0]
+    nb = bead_pos.shape[0]
+    for _ in range(iters):
+        pts = np.mod(np.concatenate([bead_pos, wat.reshape(-1, 3)], 0), boxv)
+        tree = cKDTree(pts, boxsize=boxv)
+        pairs = tree.query_pairs(dmin, output_type="ndarray")
+        if pairs.size == 0:
+            break
+        mol = np.concatenate([-1 - np.arange(nb), np.repeat(np.arange(nw), 3)])
+        a, b = mol[pairs[:, 0]], mol[pairs[:, 1]]
+        inter = a != b
+        bad = np.unique(np.concatenate([a[inter & (a >= 0)], b[inter & (b >= 0)]]))
+        if bad.size == 0:
+            break
+        oxy = base[bad] + rng.uniform(-jitter, jitter, (bad.size, 3))
+        wat[bad] = _place_waters(rng, oxy)
+    return wat
 
 
 def make_system(name: str, n_waters: int, n_chains: int, chain_len: int, box: float,
                 n_bead_types: int, seed: int, params: NbParams | None = None,
-                lattice_jitter: float = 0.3, lattice_shape=None) -> System:
+                lattice_jitter: float = 0.3, lattice_shape=None, min_contact: float = 1.2) -> System:
     """Water + chain box.  Atom order: chains first (bead 0..L-1 per chain), then waters O,H1,H2."""
     rng = np.random.default_rng(seed)
     boxv = np.array([box, box, box], dtype=np.float64)
@@ -167,9 +227,10 @@ def make_system(name: str, n_waters: int, n_chains: int, chain_len: int, box: fl
 
     # ---- chains
     beads = []
+    grid = _BeadGrid(boxv, 3.7)
     for c in range(n_chains):
         start = rng.random(3) * boxv
-        beads.append(_chain_walk(rng, start, chain_len, boxv))
+        beads.append(_chain_walk(rng, start, chain_len, boxv, grid, c))
     bead_pos = np.concatenate(beads, 0) if beads else np.zeros((0, 3))
     n_beads = bead_pos.shape[0]
     bead_type = rng.integers(0, n_bead_types, n_beads).astype(np.int32) if n_beads else np.zeros(0, np.int32)
@@ -202,8 +263,11 @@ def make_system(name: str, n_waters: int, n_chains: int, chain_len: int, box: fl
     if sites.shape[0] < n_waters:
         raise ValueError(f"{name}: only {sites.shape[0]} lattice sites for {n_waters} waters")
     choose = np.sort(rng.choice(sites.shape[0], n_waters, replace=False))
-    oxy = sites[choose] + rng.uniform(-lattice_jitter, lattice_jitter, (n_waters, 3))
-    wat = _place_waters(rng, oxy).reshape(-1, 3)
+    base = sites[choose]
+    oxy = base + rng.uniform(-lattice_jitter, lattice_jitter, (n_waters, 3))
+    wat = _place_waters(rng, oxy)
+    wat = _relax_contacts(rng, wat, base, bead_pos, boxv, lattice_jitter, min_contact)
+    wat = wat.reshape(-1, 3)
 
     pos = np.concatenate([bead_pos, wat], 0)
     pos = np.mod(pos, boxv)
@@ -248,7 +312,7 @@ def water_box_9k(seed: int = 1) -> System:
     The density is about three times liquid water, so close contacts are hard.
     """
     p = NbParams(cutoff=12.0, switchdist=10.0, pairlistdist=13.5, pme_grid=(32, 32, 32))
-    return make_system("water9k", 3000, 0, 0, 31.0, 0, seed, p, lattice_shape=(15, 15, 14))
+    return make_system("water9k", 3000, 0, 0, 31.0, 0, seed, p, lattice_shape=(15, 15, 14), min_contact=1.0)
 
 
 def solvated_96k(seed: int = 2) -> System:

@@ -23,6 +23,12 @@
  *                          NAMD's CPU convention (PmeBase.inl:17-148,
  *                          PmeRealSpace.C:237-425), order 4 with the explicit
  *                          cubics, other orders with compute_b_spline.
+ *  oracle_pairs_exact      NAMD's CPU membership rule (r^2 <= rc^2 in FP64,
+ *                          exact on 2^-17 A effective positions).
+ *  oracle_namd_tables / oracle_namd_cpu
+ *                          NAMD's CPU nonbonded path: FP64 r^2 cubic-spline
+ *                          tables of ComputeNonbondedUtil::select and the pair
+ *                          loop of ComputeNonbondedBase2.h.
  *
  * Parity pins are listed in the test files that use each function.
  */
@@ -187,7 +193,7 @@ int oracle_nb_fp64(int n, const double* pos, const double* q, const int32_t* typ
           double d[3];
           for (int u = 0; u < 3; ++u) d[u] = min_image(pos[3 * j + u] - pos[3 * i + u], L[u]);
           const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-          if (!(r2 < c.rc2)) continue;
+          if (!(r2 <= c.rc2)) continue;
           if (is_excluded(eptr, eidx, i, j)) continue;
           double ff, ffs;
           const int ab = type[i] * ntypes + type[j];
@@ -501,5 +507,321 @@ int oracle_pme_gather_cpu(int n, const double* p, const double* q, int K1, int K
 /* B-spline partition of unity check helper: sum of weights and derivative weights (GPU recursion) */
 int oracle_bspline_gpu(double w, int order, double* th, double* dth) {
   bspline_gpu(w, order, th, dth);
+  return 0;
+}
+
+/* ---- exact pair sets (NAMD CPU rule) -------------------------------------
+ * With effective positions on the 2^-17 A grid the FP64 displacement and r^2
+ * are exact, so this is NAMD's CPU membership (pairlist build r2 <= plcutoff2,
+ * ComputeNonbondedBase.h:1282; per step r2 + r2_delta <= cutoff2 + r2_delta,
+ * ComputeNonbondedInl.h:80-117) without rounding.  Same outputs as
+ * oracle_pairs_fp32. */
+int oracle_pairs_exact(int n, const double* pos, const int64_t* eptr, const int32_t* eidx, const double L[3],
+                       double cut2, double plcut2, int64_t* out, uint64_t* sums, uint64_t* xors, int32_t* pairs,
+                       int64_t maxpairs) {
+  Cells C;
+  cells_build(&C, n, pos, L, sqrt(plcut2) * 1.0001 + 1e-6);
+  const int ncell = C.nc[0] * C.nc[1] * C.nc[2];
+  int64_t cnt[3] = {0, 0, 0}, written = 0;
+  uint64_t s0 = 0, s1 = 0, x0 = 0, x1 = 0;
+  for (int cell = 0; cell < ncell; ++cell) {
+    int nb[27];
+    const int gz = cell % C.nc[2], gy = (cell / C.nc[2]) % C.nc[1], gx = cell / (C.nc[1] * C.nc[2]);
+    const int m = cell_neighbours(&C, gx, gy, gz, nb);
+    for (int i = C.head[cell]; i >= 0; i = C.next[i])
+      for (int k = 0; k < m; ++k)
+        for (int j = C.head[nb[k]]; j >= 0; j = C.next[j]) {
+          if (j <= i) continue;
+          double d[3];
+          for (int u = 0; u < 3; ++u) d[u] = min_image(pos[3 * j + u] - pos[3 * i + u], L[u]);
+          const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          if (!(r2 <= plcut2)) continue;
+          if (is_excluded(eptr, eidx, i, j)) {
+            cnt[2]++;
+            continue;
+          }
+          const uint64_t h = oracle_pair_hash(i, j, n);
+          cnt[1]++;
+          s1 += h;
+          x1 ^= h;
+          if (pairs && written < maxpairs) {
+            pairs[2 * written] = i;
+            pairs[2 * written + 1] = j;
+            written++;
+          }
+          if (r2 <= cut2) {
+            cnt[0]++;
+            s0 += h;
+            x0 ^= h;
+          }
+        }
+  }
+  out[0] = cnt[0];
+  out[1] = cnt[1];
+  out[2] = cnt[2];
+  out[3] = written;
+  if (sums) { sums[0] = s0; sums[1] = s1; }
+  if (xors) { xors[0] = x0; xors[1] = x1; }
+  cells_free(&C);
+  return 0;
+}
+
+/* ---- NAMD CPU nonbonded path: FP64 r^2 cubic-spline tables ----------------
+ * Restates ComputeNonbondedUtil::select (ComputeNonbondedUtil.C:604-1124) for
+ * PME + C1 splitting + LJ energy switching: bins of 64 per octave of
+ * r^2 + r2_delta starting at r2_delta = 1/16 (:607-626), node values of
+ * fast/scor/slow/vdwa/vdwb energy and gradient (:763-1037; gradients are
+ * d/d(r^2)), linear continuation below r2_limit for fast/vdwa/vdwb (:1080-1100),
+ * Hermite cubic per bin with two refinement steps (:1101-1124).  The pair loop
+ * restates ComputeNonbondedBase2.h (table index from the double's exponent and
+ * top 6 mantissa bits, :86-92; the -6d,-4c,-2b,-a table_four layout of
+ * ComputeNonbondedUtil.C:1263-1297 and its Horner order, :207-330). */
+typedef struct {
+  int n;
+  double r2_delta;
+  int r2_delta_exp;
+  double* r2_table;  /* n */
+  double* t[5];      /* fast, scor, slow, vdwa, vdwb: 4*n each (a, b, c, d) */
+} NamdTables;
+
+static void namd_tables_build(NamdTables* T, double cutoff, double switchOn, double ewaldcof, double limitDist) {
+  const double cutoff2 = cutoff * cutoff;
+  const double switchOn2 = switchOn * switchOn;
+  const double c0 = (switchOn < cutoff) ? 1.0 / (cutoff2 - switchOn2) : 0.0;
+  const double c1 = c0 * c0 * c0;
+  const double c3 = 3.0 * (cutoff2 - switchOn2);
+  const double pi_ewaldcof = 1.12837916709551 * ewaldcof;
+  double r2_delta = 1.0;
+  int r2_delta_exp = 0;
+  while (r2_delta > 0.1) { r2_delta /= 2.0; r2_delta_exp += 1; }
+  double r2_tmp = 1.0;
+  int cutoff2_exp = 0;
+  while ((cutoff2 + r2_delta) > r2_tmp) { r2_tmp *= 2.0; cutoff2_exp += 1; }
+  const int n = (r2_delta_exp + cutoff2_exp) * 64 + 1;
+  T->n = n;
+  T->r2_delta = r2_delta;
+  T->r2_delta_exp = r2_delta_exp;
+  T->r2_table = (double*)calloc(n + 1, sizeof(double));
+  for (int k = 0; k < 5; ++k) T->t[k] = (double*)calloc(4 * (size_t)n + 8, sizeof(double));
+  double r2_limit = limitDist * limitDist;
+  if (r2_limit < r2_delta) r2_limit = r2_delta;
+  int r2_delta_i = 0;
+  T->r2_table[0] = r2_delta;
+  for (int i = 1; i < n; ++i) {
+    const double r2_base = r2_delta * (1 << (i / 64));
+    const double r2_del = r2_base / 64.0;
+    const double r2 = r2_base - r2_delta + r2_del * (i % 64);
+    const double r = sqrt(r2);
+    if (r2 <= r2_limit) r2_delta_i = i;
+    const double r_1 = 1.0 / r;
+    const double r_2 = 1.0 / r2;
+    const double tmp_a = r * ewaldcof;
+    const double tmp_b = erfc(tmp_a);
+    const double corr_energy = tmp_b;
+    const double corr_gradient = pi_ewaldcof * exp(-(tmp_a * tmp_a)) * r + tmp_b;
+    const double slow_energy = 0.5 / cutoff * (3.0 - (r2 / cutoff2));
+    double slow_gradient = -1.0 / cutoff2 * (r / cutoff);
+    const double scor_energy = slow_energy + (corr_energy - 1.0) / r;
+    double scor_gradient = slow_gradient - (corr_gradient - 1.0) / r2;
+    const double fast_energy = 1.0 / r - slow_energy;
+    double fast_gradient = -1.0 / r2 - slow_gradient;
+    fast_gradient *= 0.5 * r_1;
+    scor_gradient *= 0.5 * r_1;
+    slow_gradient *= 0.5 * r_1;
+    const double r_6 = r_2 * r_2 * r_2;
+    const double r_12 = r_6 * r_6;
+    const double c2 = cutoff2 - r2;
+    const double c4 = c2 * (c3 - 2.0 * c2);
+    const double switchVal = (r2 > switchOn2 ? c2 * c4 * c1 : 1.0);
+    const double dSwitchVal = (r2 > switchOn2 ? 2 * c1 * (c2 * c2 - c4) : 0.0);
+    const double vdwa_energy = switchVal * r_12;
+    const double vdwb_energy = switchVal * r_6;
+    const double vdwa_gradient = (dSwitchVal - 6.0 * switchVal * r_2) * r_12;
+    const double vdwb_gradient = (dSwitchVal - 3.0 * switchVal * r_2) * r_6;
+    const double ev[5] = {fast_energy, scor_energy, slow_energy, vdwa_energy, vdwb_energy};
+    const double gv[5] = {fast_gradient, scor_gradient, slow_gradient, vdwa_gradient, vdwb_gradient};
+    for (int k = 0; k < 5; ++k) {
+      T->t[k][4 * i + 0] = ev[k];
+      T->t[k][4 * i + 1] = gv[k];
+      T->t[k][4 * i + 2] = 0.0;
+      T->t[k][4 * i + 3] = 0.0;
+    }
+    T->r2_table[i] = r2 + r2_delta;
+  }
+  for (int k = 0; k < 5; ++k) {
+    double* t0 = T->t[k];
+    const int smooth_short = (k == 0 || k == 3 || k == 4);
+    t0[0] = t0[4] - t0[5] * (r2_delta / 64.0);
+    t0[1] = t0[5];
+    t0[2] = 0;
+    t0[3] = 0;
+    double energy0 = 0, gradient0 = 0, r20 = 0;
+    if (smooth_short) {
+      energy0 = t0[4 * r2_delta_i];
+      gradient0 = t0[4 * r2_delta_i + 1];
+      r20 = T->r2_table[r2_delta_i];
+      t0[0] = energy0 - gradient0 * (r20 - T->r2_table[0]);
+      t0[1] = gradient0;
+    }
+    double* t = t0;
+    for (int i = 0; i < n - 1; ++i, t += 4) {
+      const double x = (r2_delta * (1 << (i / 64))) / 64.0;
+      if (smooth_short && i + 1 < r2_delta_i) {
+        t[4] = energy0 - gradient0 * (r20 - T->r2_table[i + 1]);
+        t[5] = gradient0;
+      }
+      const double v1 = t[0], g1 = t[1], v2 = t[4], g2 = t[5];
+      double c = (3.0 * (v2 - v1) - x * (2.0 * g1 + g2)) / (x * x);
+      double d = (-2.0 * (v2 - v1) + x * (g1 + g2)) / (x * x * x);
+      for (int it = 0; it < 2; ++it) {
+        const double dv = (v1 - v2) + ((d * x + c) * x + g1) * x;
+        const double dg = (g1 - g2) + (3.0 * d * x + 2.0 * c) * x;
+        c -= (3.0 * dv - x * dg) / (x * x);
+        d -= (-2.0 * dv + x * dg) / (x * x * x);
+      }
+      t[2] = c;
+      t[3] = d;
+    }
+  }
+}
+
+static void namd_tables_free(NamdTables* T) {
+  free(T->r2_table);
+  for (int k = 0; k < 5; ++k) free(T->t[k]);
+}
+
+/* Table entries for tests: writes n, r2_table (n) and the 5 tables (5*4*n, order
+ * fast, scor, slow, vdwa, vdwb; each entry a,b,c,d) if the pointers are set. */
+int oracle_namd_tables(double cutoff, double switchOn, double ewaldcof, double limitDist, int* n, double* r2_table,
+                       double* tables) {
+  NamdTables T;
+  namd_tables_build(&T, cutoff, switchOn, ewaldcof, limitDist);
+  *n = T.n;
+  if (r2_table) memcpy(r2_table, T.r2_table, sizeof(double) * T.n);
+  if (tables)
+    for (int k = 0; k < 5; ++k) memcpy(tables + (size_t)k * 4 * T.n, T.t[k], sizeof(double) * 4 * T.n);
+  namd_tables_free(&T);
+  return 0;
+}
+
+/* The CPU pair path over normal (non-excluded) pairs with r2 + r2_delta <=
+ * cutoff2 + r2_delta.  Outputs like oracle_nb_fp64: f (vdw + fast), fs (scor),
+ * energies (vdw, fast elec, slow), virial, virialSlow, npairs. */
+int oracle_namd_cpu(int n, const double* pos, const double* q, const int32_t* type, int ntypes, const double* A,
+                    const double* B, const int64_t* eptr, const int32_t* eidx, const double L[3], double cutoff,
+                    double switchdist, double beta, double* f, double* fs, double* energies, double* virial,
+                    double* virialSlow, int64_t* npairs, int nthreads) {
+  NamdTables T;
+  namd_tables_build(&T, cutoff, switchdist, beta, 0.0);
+  const double r2_delta = T.r2_delta;
+  const int r2_delta_expc = 64 * (T.r2_delta_exp - 1023);
+  const double cutoff2_delta = cutoff * cutoff + r2_delta;
+  /* table_four: 16 per bin, -6d -4c -2b -a for vdwa, vdwb, fast, scor */
+  double* four = (double*)malloc(sizeof(double) * 16 * (size_t)T.n);
+  const int src[4] = {3, 4, 0, 1};
+  for (int i = 0; i < T.n; ++i)
+    for (int s = 0; s < 4; ++s) {
+      const double* t = T.t[src[s]] + 4 * i;
+      four[16 * i + 4 * s + 0] = -6. * t[3];
+      four[16 * i + 4 * s + 1] = -4. * t[2];
+      four[16 * i + 4 * s + 2] = -2. * t[1];
+      four[16 * i + 4 * s + 3] = -1. * t[0];
+    }
+  Cells C;
+  cells_build(&C, n, pos, L, cutoff);
+  const int ncell = C.nc[0] * C.nc[1] * C.nc[2];
+  int nt = 1;
+#ifdef _OPENMP
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+  double* fb = (double*)calloc((size_t)nt * 6 * n, sizeof(double));
+  double* acc = (double*)calloc((size_t)nt * 24, sizeof(double));
+  int64_t* np = (int64_t*)calloc((size_t)nt, sizeof(int64_t));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+#endif
+  for (int cell = 0; cell < ncell; ++cell) {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double* F = fb + (size_t)tid * 6 * n;
+    double* S = acc + (size_t)tid * 24;
+    int nb[27];
+    const int gz = cell % C.nc[2], gy = (cell / C.nc[2]) % C.nc[1], gx = cell / (C.nc[1] * C.nc[2]);
+    const int m = cell_neighbours(&C, gx, gy, gz, nb);
+    for (int i = C.head[cell]; i >= 0; i = C.next[i])
+      for (int k = 0; k < m; ++k)
+        for (int j = C.head[nb[k]]; j >= 0; j = C.next[j]) {
+          if (j <= i) continue;
+          /* p_ij = p_i - p_j, minimum image */
+          const double p_ij_x = -min_image(pos[3 * j] - pos[3 * i], L[0]);
+          const double p_ij_y = -min_image(pos[3 * j + 1] - pos[3 * i + 1], L[1]);
+          const double p_ij_z = -min_image(pos[3 * j + 2] - pos[3 * i + 2], L[2]);
+          double r2 = p_ij_x * p_ij_x + r2_delta;
+          r2 += p_ij_y * p_ij_y;
+          r2 += p_ij_z * p_ij_z;
+          if (!(r2 <= cutoff2_delta)) continue;
+          if (is_excluded(eptr, eidx, i, j)) continue;
+          uint64_t bits;
+          memcpy(&bits, &r2, sizeof bits);
+          const int table_i = (int)(bits >> 46) + r2_delta_expc;
+          const double diffa = r2 - T.r2_table[table_i];
+          const double* tf = four + 16 * table_i;
+          const int ab = type[i] * ntypes + type[j];
+          const double Aij = A[ab], Bij = B[ab];
+          const double vdw_d = Aij * tf[0] - Bij * tf[4];
+          const double vdw_c = Aij * tf[1] - Bij * tf[5];
+          const double vdw_b = Aij * tf[2] - Bij * tf[6];
+          const double vdw_a = Aij * tf[3] - Bij * tf[7];
+          const double kqq = q[i] * q[j];
+          const double fast_d = kqq * tf[8], fast_c = kqq * tf[9], fast_b = kqq * tf[10], fast_a = kqq * tf[11];
+          const double slow_d = kqq * tf[12], slow_c = kqq * tf[13], slow_b = kqq * tf[14], slow_a = kqq * tf[15];
+          const double vdw_val = ((diffa * vdw_d * (1 / 6.) + vdw_c * (1 / 4.)) * diffa + vdw_b * (1 / 2.)) * diffa + vdw_a;
+          const double fast_val = ((diffa * fast_d * (1 / 6.) + fast_c * (1 / 4.)) * diffa + fast_b * (1 / 2.)) * diffa + fast_a;
+          const double slow_val = ((diffa * slow_d * (1 / 6.) + slow_c * (1 / 4.)) * diffa + slow_b * (1 / 2.)) * diffa + slow_a;
+          S[18] -= vdw_val;
+          S[19] -= fast_val;
+          S[20] -= slow_val;
+          const double force_r = ((diffa * (vdw_d + fast_d) + (vdw_c + fast_c)) * diffa) + (vdw_b + fast_b);
+          const double fullforce_r = (diffa * slow_d + slow_c) * diffa + slow_b;
+          const double pij[3] = {p_ij_x, p_ij_y, p_ij_z};
+          for (int u = 0; u < 3; ++u) {
+            F[3 * i + u] += force_r * pij[u];
+            F[3 * j + u] -= force_r * pij[u];
+            F[3 * n + 3 * i + u] += fullforce_r * pij[u];
+            F[3 * n + 3 * j + u] -= fullforce_r * pij[u];
+          }
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+              S[a * 3 + b] += force_r * pij[a] * pij[b];
+              S[9 + a * 3 + b] += fullforce_r * pij[a] * pij[b];
+            }
+          np[tid] += 1;
+        }
+  }
+  if (f) memset(f, 0, sizeof(double) * 3 * n);
+  if (fs) memset(fs, 0, sizeof(double) * 3 * n);
+  double tot[24] = {0};
+  int64_t ntot = 0;
+  for (int t = 0; t < nt; ++t) {
+    const double* F = fb + (size_t)t * 6 * n;
+    if (f)
+      for (int k = 0; k < 3 * n; ++k) f[k] += F[k];
+    if (fs)
+      for (int k = 0; k < 3 * n; ++k) fs[k] += F[3 * n + k];
+    for (int k = 0; k < 24; ++k) tot[k] += acc[(size_t)t * 24 + k];
+    ntot += np[t];
+  }
+  if (virial) memcpy(virial, tot, 9 * sizeof(double));
+  if (virialSlow) memcpy(virialSlow, tot + 9, 9 * sizeof(double));
+  if (energies) memcpy(energies, tot + 18, 3 * sizeof(double));
+  if (npairs) *npairs = ntot;
+  free(fb);
+  free(acc);
+  free(np);
+  free(four);
+  cells_free(&C);
+  namd_tables_free(&T);
   return 0;
 }

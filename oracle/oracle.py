@@ -41,6 +41,11 @@ def lib():
         L.oracle_pme_spread_cpu.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
         L.oracle_pme_gather_cpu.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
         L.oracle_bspline_gpu.argtypes = [C.c_double, C.c_int, dp, dp]
+        L.oracle_pairs_exact.argtypes = [C.c_int, dp, i64p, i32p, dp, C.c_double, C.c_double, i64p, u64p, u64p,
+                                         C.c_void_p, C.c_int64]
+        L.oracle_namd_tables.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int),
+                                         C.c_void_p, C.c_void_p]
+        L.oracle_namd_cpu.argtypes = L.oracle_nb_fp64.argtypes
     return _LIB
 
 
@@ -87,6 +92,57 @@ def pairs_fp32(pos, excl_ptr, excl_idx, box, cutoff2f, plcutoff2f, want_pairs=Fa
             raise ValueError("pair buffer too small")
         res["pairs"] = buf[: out[3]].copy()
     return res
+
+
+def pairs_exact(pos, excl_ptr, excl_idx, box, cutoff, pairlistdist, want_pairs=False, maxpairs=None):
+    """NAMD CPU membership rule (r2 <= rc^2 in FP64, exact on effective positions); same outputs as pairs_fp32."""
+    n = pos.shape[0]
+    out = np.zeros(4, np.int64)
+    sums = np.zeros(2, np.uint64)
+    xors = np.zeros(2, np.uint64)
+    buf = None
+    cap = 0
+    if want_pairs:
+        cap = int(maxpairs) if maxpairs else 200 * n
+        buf = np.zeros((cap, 2), np.int32)
+    lib().oracle_pairs_exact(n, _c(pos, np.float64).reshape(-1), _c(excl_ptr, np.int64), _c(excl_idx, np.int32),
+                             _c(box, np.float64), float(cutoff) ** 2, float(pairlistdist) ** 2, out, sums, xors,
+                             buf.ctypes.data if buf is not None else None, cap)
+    res = dict(n_cut=int(out[0]), n_list=int(out[1]), n_excl_list=int(out[2]),
+               sum_cut=int(sums[0]), xor_cut=int(xors[0]), sum_list=int(sums[1]), xor_list=int(xors[1]))
+    if want_pairs:
+        if out[3] < out[1]:
+            raise ValueError("pair buffer too small")
+        res["pairs"] = buf[: out[3]].copy()
+    return res
+
+
+def namd_tables(cutoff, switchdist, ewaldcof, limitdist=0.0):
+    """NAMD's FP64 spline tables: (r2_table[n], tables[5, n, 4]) for fast, scor, slow, vdwa, vdwb."""
+    n = C.c_int(0)
+    lib().oracle_namd_tables(cutoff, switchdist, ewaldcof, limitdist, C.byref(n), None, None)
+    r2 = np.zeros(n.value)
+    t = np.zeros(5 * 4 * n.value)
+    lib().oracle_namd_tables(cutoff, switchdist, ewaldcof, limitdist, C.byref(n), r2.ctypes.data, t.ctypes.data)
+    return r2, t.reshape(5, n.value, 4)
+
+
+def namd_cpu(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff, switchdist, beta, nthreads=0):
+    """NAMD CPU nonbonded path restatement (spline tables); same outputs as nb_fp64."""
+    n = len(q_scaled)
+    ntypes = int(round(np.sqrt(np.asarray(A).size)))
+    f = np.zeros(3 * n)
+    fs = np.zeros(3 * n)
+    en = np.zeros(3)
+    vir = np.zeros(9)
+    virs = np.zeros(9)
+    npairs = np.zeros(1, np.int64)
+    lib().oracle_namd_cpu(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32),
+                          ntypes, _c(A, np.float64).reshape(-1), _c(B, np.float64).reshape(-1),
+                          _c(excl_ptr, np.int64), _c(excl_idx, np.int32), _c(box, np.float64), cutoff, switchdist,
+                          beta, f, fs, en, vir, virs, npairs, int(nthreads))
+    return dict(f=f.reshape(n, 3), fs=fs.reshape(n, 3), e_vdw=en[0], e_elec=en[1], e_slow=en[2],
+                virial=vir.reshape(3, 3), virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]))
 
 
 def pair_hash(i, j, n):
