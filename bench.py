@@ -49,6 +49,7 @@ def main():
     ap.add_argument("--pairlist-every", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default pair-kernel launch shape")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -114,6 +115,15 @@ def main():
     for k in range(max(1, args.warmup)):
         step(k)
     torch.cuda.synchronize()
+    tuned = None
+    if not args.no_autotune:
+        from mi355nb.autotune import tune_pair_kernel
+        best, table = tune_pair_kernel(c)
+        tuned = {"workgroups_per_cu": best, "ms_by_workgroups_per_cu": {str(k): round(v, 4) for k, v in table.items()}}
+        log(f"autotune: {tuned}")
+        for k in range(max(1, args.warmup)):
+            step(k)
+        torch.cuda.synchronize()
     r0 = c.finish(forces=False)
     npairs = int(r0["stat"]["numPairsCutoff"])
     nlists = int(r0["stat"]["numTileLists"])
@@ -166,6 +176,7 @@ def main():
             "dt_fs": 2.0,
         },
         "gpairs_per_s": round(gpairs, 2),
+        "autotune": tuned,
         "pairs_in_cutoff": npairs,
         "kernel_ms": {"nonbonded_force": round(t_force_ms, 4), "nonbonded_pairlist": round(t_pl_ms, 4),
                       "pme_spread": round(float(spread_ms), 4), "pme_gather": round(float(gather_ms), 4)},

@@ -173,6 +173,12 @@ int mi355nb_compute_finish(mi355nb_compute* c, double* force, double* forceSlow,
  * get_kernel_time synchronises and returns the summed milliseconds and
  * launch counts since the last reset, split by pairlist / non-pairlist. */
 int mi355nb_compute_set_timing(mi355nb_compute* c, int enable);
+
+/* Launch shape of the pair kernel: persistent workgroups (4 waves each) per
+ * CU, 1..64 (default 8).  Chosen by the autotuner in bench.py; no counterpart
+ * in the reference (NAMD fixes its grid at compile time,
+ * src/CudaComputeNonbondedKernel.cu:2164-2167). */
+int mi355nb_compute_set_launch_config(mi355nb_compute* c, int workgroupsPerCU);
 int mi355nb_compute_get_kernel_time(mi355nb_compute* c, double* msPairlist, int32_t* nPairlist, double* msForce,
                                     int32_t* nForce);
 

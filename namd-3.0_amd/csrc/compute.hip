@@ -57,6 +57,7 @@ struct mi355nb_compute {
 
   PatchLayout L;
   int numCandLists = 0;
+  int wgPerCU = 8;     // persistent pair-kernel workgroups per CU
 
   // device buffers
   DevBuf<F4> xyzq, force, forceSlow;
@@ -203,6 +204,8 @@ int mi355nb_compute_create(const mi355nb_nb_params* params, int device, void* st
   CHECK_ARG(params->cutoff > 0 && params->switchdist > 0 && params->switchdist <= params->cutoff &&
                 params->pairlistdist >= params->cutoff && params->ewaldcof >= 0,
             "inconsistent cutoff / switchdist / pairlistdist / ewaldcof");
+  CHECK_ARG(params->ewaldcof * params->cutoff <= 4.0,
+            "ewaldcof * cutoff above 4 (outside the fitted range of the slow-force factor)");
   int ndev = 0;
   MI355NB_HIP(hipGetDeviceCount(&ndev));
   CHECK_ARG(device >= 0 && device < ndev, "device index out of range");
@@ -230,6 +233,10 @@ int mi355nb_compute_set_molecule(mi355nb_compute* c, int natoms, const int32_t* 
                                  const int32_t* excl_idx) {
   CHECK_ARG(c && natoms > 0 && vdw_type && charge && ntypes > 0 && lj_A && lj_B && excl_ptr,
             "bad molecule arguments");
+  if (ntypes > 64) {
+    setLastError("more than 64 LJ types: the pair kernel stages the A/B table in LDS");
+    return MI355NB_ERR_CAPACITY;
+  }
   for (int i = 0; i < natoms; ++i)
     CHECK_ARG(vdw_type[i] >= 0 && vdw_type[i] < ntypes, "vdw type out of range");
   for (int i = 0; i < natoms; ++i)
@@ -385,6 +392,7 @@ int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int d
   a.fsoa = c->fsoa.ptr;
   a.stride = n;
   a.virialEnergy = c->tlve.ptr;
+  a.wgPerCU = c->wgPerCU;
 
   const int* numListsForReduce;
   if (doPairlist) {
@@ -481,6 +489,12 @@ int mi355nb_compute_finish(mi355nb_compute* c, double* forceOut, double* forceSl
     static_assert(sizeof(TileListStat) == sizeof(mi355nb_tile_list_stat), "TileListStat layout");
     MI355NB_HIP(hipMemcpy(statOut, c->stat.ptr, sizeof(TileListStat), hipMemcpyDeviceToHost));
   }
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_set_launch_config(mi355nb_compute* c, int workgroupsPerCU) {
+  CHECK_ARG(c && workgroupsPerCU >= 1 && workgroupsPerCU <= 64, "workgroupsPerCU must be in 1..64");
+  c->wgPerCU = workgroupsPerCU;
   return MI355NB_OK;
 }
 

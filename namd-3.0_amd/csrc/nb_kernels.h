@@ -29,6 +29,7 @@ struct ForceArgsHost {
   float* fsoa;
   int stride;
   TileListVirialEnergy* virialEnergy;
+  int wgPerCU;            // persistent workgroups per CU (4 waves each)
 };
 
 hipError_t launchBoundingBoxes(int numTiles, const F4* xyzq, BoundingBox* bbox, hipStream_t s);

@@ -15,7 +15,10 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "nb_kernels.h"
+#include "slowforce.h"
 
 namespace mi355nb {
 
@@ -108,42 +111,14 @@ __global__ void buildJtilesKernel(int numLists, TileList* __restrict__ lists,
 // (cudaNBForceMagCalc_VdwEnergySwitch_PMEC1, src/CudaComputeNonbondedInteractions.h:37-214).
 // f and fSlow are (1/r) dE/dr, so F_i = f * (x_j - x_i).
 //
-// The slow force is written through h(a) = 1 - erfc(a) - (2/sqrt(pi)) a e^{-a^2}
-// (a = beta r), which is NAMD's (1 - corr_grad):  fSlow = qq (h/r^3 - 1/rc^3).
-//  * a >= 0.8: erfc from Abramowitz & Stegun 7.1.26 (Hastings),
-//    erfc(x) = t P(t) e^{-x^2}, t = 1/(1+p x), |error| <= 1.5e-7, sharing
-//    e^{-x^2} with the gradient term; h >= 0.266 there, so the absolute
-//    error stays below 6e-7 relative.
-//  * a < 0.8: h(a) = a^3 H(a^2) with the Taylor series
-//    H(u) = (4/sqrt(pi)) sum_n (-u)^n / (n! (2n+3)), n = 0..9 (truncation
-//    below 4e-9), which avoids the cancellation of 1 - erfc - ... at short
-//    range.
+// Slow force: fSlow = qq (beta^3 H(beta^2 r^2) - 1/rc^3) with the rational
+// H of slowforce.h; this is NAMD's qq ((1 - corr_grad)/r^3 - 1/rc^3) without
+// exp, erfc or a branch.
 // Energies (only on energy steps) use the cancellation-free form
 //    E_fast = qq (1 - x)^2 (1 + x/2) / r, x = r/rc   (= qq (1/r - 3/(2rc) + r^2/(2rc^3)))
-//    E_slow = qq erfc(a)/r - E_fast
+//    E_slow = qq erfc(a)/r - E_fast,  a = beta r
 // with the device library's erfcf, and accumulate per lane in FP64.
 // ---------------------------------------------------------------------------
-#define AS_P 0.3275911f
-#define AS_A1 0.254829592f
-#define AS_A2 -0.284496736f
-#define AS_A3 1.421413741f
-#define AS_A4 -1.453152027f
-#define AS_A5 1.061405429f
-#define LOG2E_F 1.4426950408889634f
-#define SERIES_A0 0.8f
-
-constexpr double FOUR_OVER_SQRT_PI = 2.2567583341910251478;
-constexpr float H0 = (float)(FOUR_OVER_SQRT_PI / 3.0);
-constexpr float H1 = (float)(-FOUR_OVER_SQRT_PI / 5.0);
-constexpr float H2 = (float)(FOUR_OVER_SQRT_PI / (2.0 * 7.0));
-constexpr float H3 = (float)(-FOUR_OVER_SQRT_PI / (6.0 * 9.0));
-constexpr float H4 = (float)(FOUR_OVER_SQRT_PI / (24.0 * 11.0));
-constexpr float H5 = (float)(-FOUR_OVER_SQRT_PI / (120.0 * 13.0));
-constexpr float H6 = (float)(FOUR_OVER_SQRT_PI / (720.0 * 15.0));
-constexpr float H7 = (float)(-FOUR_OVER_SQRT_PI / (5040.0 * 17.0));
-constexpr float H8 = (float)(FOUR_OVER_SQRT_PI / (40320.0 * 19.0));
-constexpr float H9 = (float)(-FOUR_OVER_SQRT_PI / (362880.0 * 21.0));
-
 template <bool doEnergy>
 struct EnergyAcc { float v = 0.f; };
 template <>
@@ -151,7 +126,7 @@ struct EnergyAcc<true> { double v = 0.0; };
 
 template <bool doEnergy, bool doSlow, typename Acc>
 __device__ __forceinline__ void pairForce(const float r2, const float qq, const float A, const float B,
-                                          const NbConstants& c, float& f, float& fSlow,
+                                          const NbConstants& c, const float beta3, float& f, float& fSlow,
                                           Acc& eVdw, Acc& eElec, Acc& eSlow) {
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float rinv2 = rinv * rinv;
@@ -172,30 +147,20 @@ __device__ __forceinline__ void pairForce(const float r2, const float qq, const 
   // fast electrostatics (PMEFast_C1, :107-123)
   const float elec_fast = -1.0f * rinv3 + c.e_0;
   f = qq * elec_fast + f_vdw;
-  const float r = r2 * rinv;
-  const float a = r * c.ewald_0;
   if (doSlow) {
-    // slow electrostatics (PMESlow_C1, :131-159) through h(a)
-    float h;
-    if (a < SERIES_A0) {
-      const float u = a * a;
-      const float H = H0 + u * (H1 + u * (H2 + u * (H3 + u * (H4 + u * (H5 + u * (H6 + u * (H7 + u * (H8 + u * H9))))))));
-      h = a * u * H;
-    } else {
-      const float ex = __builtin_amdgcn_exp2f(-(a * a) * LOG2E_F);
-      const float t = __builtin_amdgcn_rcpf(1.0f + AS_P * a);
-      const float poly = t * (AS_A1 + t * (AS_A2 + t * (AS_A3 + t * (AS_A4 + t * AS_A5))));
-      h = (1.0f - poly * ex) - c.ewald_1 * r * ex;
-    }
-    fSlow = qq * (h * rinv3 - c.e_0);
+    const float u = r2 * c.ewald_2;
+    const float P = HP0 + u * (HP1 + u * (HP2 + u * (HP3 + u * (HP4 + u * (HP5 + u * HP6)))));
+    const float Q = 1.0f + u * (HQ1 + u * (HQ2 + u * (HQ3 + u * (HQ4 + u * (HQ5 + u * HQ6)))));
+    fSlow = qq * (beta3 * (P * __builtin_amdgcn_rcpf(Q)) - c.e_0);
   }
   if (doEnergy) {
+    const float r = r2 * rinv;
     const float x = r * c.e_2;
     const float omx = 1.0f - x;
     const float fastE = omx * omx * (1.0f + 0.5f * x) * rinv;
     eVdw.v += w * s;
     eElec.v += qq * fastE;
-    if (doSlow) eSlow.v += qq * (erfcf(a) * rinv - fastE);
+    if (doSlow) eSlow.v += qq * (erfcf(r * c.ewald_0) * rinv - fastE);
   }
 }
 
@@ -208,10 +173,20 @@ __device__ __forceinline__ bool within(float dx, float dy, float dz, float r2, c
   return x * x + y * y + z * z <= t.c2;
 }
 
+// Orders LDS accesses between the lanes of one wave: a compiler barrier plus
+// wavefront-scope fences (a wave's LDS operations complete in order).
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // the pair kernel
 //
-// One workgroup = one wave64 = one tile list.  Lane l holds i-atom (l & 31);
+// Persistent workgroups of NB_WAVES wave64s; the LJ A/B table is staged in
+// LDS once per workgroup and every wave walks tile lists w, w + W, w + 2W ...
+// (lists are ordered by descending depth).  Lane l holds i-atom (l & 31);
 // half-wave h = l >> 5.  For every j-tile of the list both halves sweep the
 // same 32x32 tile pair: at step t (0..15) lane l handles
 //     j = (i + t + 16 h) mod 32,
@@ -219,9 +194,13 @@ __device__ __forceinline__ bool within(float dx, float dy, float dz, float r2, c
 // j-force accumulates in LDS without atomics (one LDS copy per half).  After
 // the sweep the two copies are summed and added to the SOA force arrays with
 // coalesced float atomics; i-forces are summed across halves at the end of
-// the list.  On pairlist steps the kernel evaluates every slot against the
-// pairlist cutoff and the exclusion bit table and writes the 32-bit masks.
+// the list.  The next j-tile's coordinates, types and masks are loaded into
+// registers while the current one is computed.  On pairlist steps the kernel
+// evaluates every slot against the pairlist cutoff and the exclusion bit
+// table and writes the 32-bit masks.
 // ---------------------------------------------------------------------------
+constexpr int NB_WAVES = 4;
+
 struct ForceArgs {
   const int* numLists;               // device: number of lists to process
   const TileList* tileLists;
@@ -249,178 +228,221 @@ struct ForceArgs {
 };
 
 template <bool doEnergy, bool doVirial, bool doSlow, bool doPairlist>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64 * NB_WAVES)
 nonbondedForceKernel(const ForceArgs args) {
-  const int list = blockIdx.x;
-  if (list >= *args.numLists) return;
-  const int lane = threadIdx.x;
+  extern __shared__ float2 s_lj[];   // vdwWidth^2 (A, B)
+  __shared__ F4 s_xyzq[NB_WAVES][TILE];
+  __shared__ int s_type[NB_WAVES][TILE];
+  __shared__ int s_index[NB_WAVES][TILE];
+  __shared__ F4 s_jf[NB_WAVES][2][TILE];
+  __shared__ F4 s_jfs[NB_WAVES][2][TILE];
+
+  const int nlj = args.vdwWidth * args.vdwWidth;
+  for (int k = threadIdx.x; k < nlj; k += blockDim.x) s_lj[k] = args.vdwCoef[k];
+  __syncthreads();
+
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int i = lane & 31;
   const int h = lane >> 5;
+  const int numLists = *args.numLists;
+  const float beta3 = args.c.ewald_2 * args.c.ewald_0;
+  F4* const sx = s_xyzq[wv];
+  int* const st = s_type[wv];
+  int* const sidx = s_index[wv];
+  F4* const sjf = s_jf[wv][h];
+  F4* const sjfs = s_jfs[wv][h];
 
-  __shared__ F4 s_xyzq[TILE];
-  __shared__ int s_type[TILE];
-  __shared__ int s_index[TILE];
-  __shared__ F4 s_jf[2][TILE];
-  __shared__ F4 s_jfs[2][TILE];
+  for (int list = blockIdx.x * NB_WAVES + wv; list < numLists; list += gridDim.x * NB_WAVES) {
+    const TileList tl = args.tileLists[list];
+    const int iatom = tl.iatomStart + i;
+    F4 xi = args.xyzq[iatom];
+    xi.x += tl.offx;
+    xi.y += tl.offy;
+    xi.z += tl.offz;
+    const int vdwRow = args.vdwTypes[iatom] * args.vdwWidth;
 
-  const TileList tl = args.tileLists[list];
-  const int iatom = tl.iatomStart + i;
-  F4 xi = args.xyzq[iatom];
-  xi.x += tl.offx;
-  xi.y += tl.offy;
-  xi.z += tl.offz;
-  const int vdwRow = args.vdwTypes[iatom] * args.vdwWidth;
+    float ifx = 0.f, ify = 0.f, ifz = 0.f;
+    float ifsx = 0.f, ifsy = 0.f, ifsz = 0.f;
+    EnergyAcc<doEnergy> eVdw, eElec, eSlow;
+    // FP64 copies of the list's i-force for the virial's shift term
+    EnergyAcc<doVirial> vfx, vfy, vfz, vfsx, vfsy, vfsz;
 
-  float ifx = 0.f, ify = 0.f, ifz = 0.f;
-  float ifsx = 0.f, ifsy = 0.f, ifsz = 0.f;
-  EnergyAcc<doEnergy> eVdw, eElec, eSlow;
-
-  // pairlist-step state
-  bool iValid = true;
-  int iIndex = 0;
-  I2 iExcl{0, 0};
-  int jEnd = 0;
-  int nPairsCut = 0, nPairsList = 0, nExcl = 0, depth = 0;
-  if (doPairlist) {
-    const PatchRecord pa = args.patches[tl.patchInd[0]];
-    const PatchRecord pb = args.patches[tl.patchInd[1]];
-    iValid = iatom < pa.atomStart + pa.numAtoms;
-    jEnd = pb.atomStart + pb.numAtoms;
-    iIndex = args.atomIndex[iatom];
-    iExcl = args.exclStartMaxdiff[iatom];
-  }
-
-  for (int jt = tl.jtileStart; jt <= tl.jtileEnd; ++jt) {
-    const int jatomStart = args.tileJatomStart[jt];
-    if (h == 0) {
-      s_xyzq[i] = args.xyzq[jatomStart + i];
-      s_type[i] = args.vdwTypes[jatomStart + i];
-      if (doPairlist) s_index[i] = args.atomIndex[jatomStart + i];
+    // pairlist-step state
+    bool iValid = true;
+    int iIndex = 0;
+    I2 iExcl{0, 0};
+    int jEnd = 0;
+    int nPairsCut = 0, nPairsList = 0, nExcl = 0, depth = 0;
+    if (doPairlist) {
+      const PatchRecord pa = args.patches[tl.patchInd[0]];
+      const PatchRecord pb = args.patches[tl.patchInd[1]];
+      iValid = iatom < pa.atomStart + pa.numAtoms;
+      jEnd = pb.atomStart + pb.numAtoms;
+      iIndex = args.atomIndex[iatom];
+      iExcl = args.exclStartMaxdiff[iatom];
     }
-    s_jf[h][i] = F4{0.f, 0.f, 0.f, 0.f};
-    if (doSlow) s_jfs[h][i] = F4{0.f, 0.f, 0.f, 0.f};
-    unsigned mask = doPairlist ? 0u : args.tileExcls[jt].excl[i];
-    const bool selfTile = doPairlist && tl.self && (jatomStart == tl.iatomStart);
-    __syncthreads();
+
+    // j-tile starts of the list, 64 at a time, one per lane (broadcast by shuffle)
+    const int nJt = tl.jtileEnd - tl.jtileStart + 1;
+    int myJ = (lane < nJt) ? args.tileJatomStart[tl.jtileStart + lane] : 0;
+    // prefetch registers for the first j-tile (all lanes load; halves share lines)
+    float nx = 0.f, ny = 0.f, nz = 0.f, nq = 0.f;
+    int nType = 0, nIdx = 0;
+    unsigned nMask = 0u;
+    if (nJt > 0) {
+      const int ja0 = __shfl(myJ, 0, 64);
+      const F4 p = args.xyzq[ja0 + i];
+      nx = p.x; ny = p.y; nz = p.z; nq = p.w;
+      nType = args.vdwTypes[ja0 + i];
+      if (doPairlist) nIdx = args.atomIndex[ja0 + i];
+      if (!doPairlist) nMask = args.tileExcls[tl.jtileStart].excl[i];
+    }
+
+    for (int k = 0; k < nJt; ++k) {
+      const int jt = tl.jtileStart + k;
+      const int jatomStart = __shfl(myJ, k & 63, 64);
+      if (h == 0) {
+        sx[i] = F4{nx, ny, nz, nq};
+        st[i] = nType;
+        if (doPairlist) sidx[i] = nIdx;
+      }
+      sjf[i] = F4{0.f, 0.f, 0.f, 0.f};
+      if (doSlow) sjfs[i] = F4{0.f, 0.f, 0.f, 0.f};
+      unsigned mask = nMask;
+      // prefetch the next j-tile
+      if (k + 1 < nJt) {
+        if (((k + 1) & 63) == 0) myJ = (k + 1 + lane < nJt) ? args.tileJatomStart[jt + 1 + lane] : 0;
+        const int jn = __shfl(myJ, (k + 1) & 63, 64);
+        const F4 p = args.xyzq[jn + i];
+        nx = p.x; ny = p.y; nz = p.z; nq = p.w;
+        nType = args.vdwTypes[jn + i];
+        if (doPairlist) nIdx = args.atomIndex[jn + i];
+        if (!doPairlist) nMask = args.tileExcls[jt + 1].excl[i];
+      }
+      const bool selfTile = doPairlist && tl.self && (jatomStart == tl.iatomStart);
+      waveSync();
 
 #pragma unroll 2
-    for (int t = 0; t < 16; ++t) {
-      const int j = (i + t + 16 * h) & 31;
-      const F4 xj = s_xyzq[j];
-      const float dx = xj.x - xi.x;
-      const float dy = xj.y - xi.y;
-      const float dz = xj.z - xi.z;
-      const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-      bool active;
-      if (doPairlist) {
-        bool inList = iValid && (jatomStart + j < jEnd) && (!selfTile || j > i) && within(dx, dy, dz, r2, args.plcut);
-        bool excluded = false;
-        if (inList) {
-          const int diff = s_index[j] - iIndex;
-          if (diff >= -iExcl.y && diff <= iExcl.y) {
-            const unsigned b = (unsigned)(iExcl.x + diff);
-            excluded = (args.exclBits[b >> 5] >> (b & 31)) & 1u;
+      for (int t = 0; t < 16; ++t) {
+        const int j = (i + t + 16 * h) & 31;
+        const F4 xj = sx[j];
+        const float dx = xj.x - xi.x;
+        const float dy = xj.y - xi.y;
+        const float dz = xj.z - xi.z;
+        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        bool active;
+        if (doPairlist) {
+          bool inList = iValid && (jatomStart + j < jEnd) && (!selfTile || j > i) && within(dx, dy, dz, r2, args.plcut);
+          bool excluded = false;
+          if (inList) {
+            const int diff = sidx[j] - iIndex;
+            if (diff >= -iExcl.y && diff <= iExcl.y) {
+              const unsigned b = (unsigned)(iExcl.x + diff);
+              excluded = (args.exclBits[b >> 5] >> (b & 31)) & 1u;
+            }
+          }
+          if (inList && excluded) ++nExcl;
+          inList = inList && !excluded;
+          if (inList) { mask |= 1u << j; ++nPairsList; }
+          active = inList && within(dx, dy, dz, r2, args.cut);
+          if (active) ++nPairsCut;
+        } else {
+          active = ((mask >> j) & 1u) && within(dx, dy, dz, r2, args.cut);
+        }
+        if (active) {
+          const float2 ab = s_lj[vdwRow + st[j]];
+          float f, fs = 0.f;
+          pairForce<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, eVdw, eElec, eSlow);
+          const float fx = dx * f, fy = dy * f, fz = dz * f;
+          ifx += fx; ify += fy; ifz += fz;
+          if (doVirial) { vfx.v += fx; vfy.v += fy; vfz.v += fz; }
+          F4 jf = sjf[j];
+          jf.x -= fx; jf.y -= fy; jf.z -= fz;
+          sjf[j] = jf;
+          if (doSlow) {
+            const float fsx = dx * fs, fsy = dy * fs, fsz = dz * fs;
+            ifsx += fsx; ifsy += fsy; ifsz += fsz;
+            if (doVirial) { vfsx.v += fsx; vfsy.v += fsy; vfsz.v += fsz; }
+            F4 jfs = sjfs[j];
+            jfs.x -= fsx; jfs.y -= fsy; jfs.z -= fsz;
+            sjfs[j] = jfs;
           }
         }
-        if (inList && excluded) ++nExcl;
-        inList = inList && !excluded;
-        if (inList) { mask |= 1u << j; ++nPairsList; }
-        active = inList && within(dx, dy, dz, r2, args.cut);
-        if (active) ++nPairsCut;
-      } else {
-        active = ((mask >> j) & 1u) && within(dx, dy, dz, r2, args.cut);
       }
-      if (active) {
-        const float2 ab = args.vdwCoef[vdwRow + s_type[j]];
-        float f, fs = 0.f;
-        pairForce<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, f, fs, eVdw, eElec, eSlow);
-        const float fx = dx * f, fy = dy * f, fz = dz * f;
-        ifx += fx; ify += fy; ifz += fz;
-        F4 jf = s_jf[h][j];
-        jf.x -= fx; jf.y -= fy; jf.z -= fz;
-        s_jf[h][j] = jf;
+      waveSync();
+
+      // j-forces: sum the two half-wave copies, coalesced atomics to SOA
+      if (h == 0) {
+        const int ja = jatomStart + i;
+        const F4 a = s_jf[wv][0][i], b = s_jf[wv][1][i];
+        atomicAdd(&args.fsoa[ja], a.x + b.x);
+        atomicAdd(&args.fsoa[args.stride + ja], a.y + b.y);
+        atomicAdd(&args.fsoa[2 * args.stride + ja], a.z + b.z);
         if (doSlow) {
-          const float fsx = dx * fs, fsy = dy * fs, fsz = dz * fs;
-          ifsx += fsx; ifsy += fsy; ifsz += fsz;
-          F4 jfs = s_jfs[h][j];
-          jfs.x -= fsx; jfs.y -= fsy; jfs.z -= fsz;
-          s_jfs[h][j] = jfs;
+          const F4 c = s_jfs[wv][0][i], d = s_jfs[wv][1][i];
+          atomicAdd(&args.fsoa[3 * args.stride + ja], c.x + d.x);
+          atomicAdd(&args.fsoa[4 * args.stride + ja], c.y + d.y);
+          atomicAdd(&args.fsoa[5 * args.stride + ja], c.z + d.z);
         }
       }
+      if (doPairlist) {
+        const unsigned full = mask | __shfl_down(mask, 32, 64);
+        const bool any = __any(full != 0u);
+        if (h == 0) args.tileExcls[jt].excl[i] = full;
+        if (lane == 0) args.jtileFlags[jt] = any ? 1 : 0;
+        depth += any ? 1 : 0;
+      }
+      waveSync();
     }
-    __syncthreads();
 
-    // j-forces: sum the two half-wave copies, coalesced atomics to SOA
+    // i-forces: combine halves, atomics to SOA
+    ifx += __shfl_down(ifx, 32, 64);
+    ify += __shfl_down(ify, 32, 64);
+    ifz += __shfl_down(ifz, 32, 64);
+    if (doSlow) {
+      ifsx += __shfl_down(ifsx, 32, 64);
+      ifsy += __shfl_down(ifsy, 32, 64);
+      ifsz += __shfl_down(ifsz, 32, 64);
+    }
     if (h == 0) {
-      const int ja = jatomStart + i;
-      const F4 a = s_jf[0][i], b = s_jf[1][i];
-      atomicAdd(&args.fsoa[ja], a.x + b.x);
-      atomicAdd(&args.fsoa[args.stride + ja], a.y + b.y);
-      atomicAdd(&args.fsoa[2 * args.stride + ja], a.z + b.z);
+      atomicAdd(&args.fsoa[iatom], ifx);
+      atomicAdd(&args.fsoa[args.stride + iatom], ify);
+      atomicAdd(&args.fsoa[2 * args.stride + iatom], ifz);
       if (doSlow) {
-        const F4 c = s_jfs[0][i], d = s_jfs[1][i];
-        atomicAdd(&args.fsoa[3 * args.stride + ja], c.x + d.x);
-        atomicAdd(&args.fsoa[4 * args.stride + ja], c.y + d.y);
-        atomicAdd(&args.fsoa[5 * args.stride + ja], c.z + d.z);
+        atomicAdd(&args.fsoa[3 * args.stride + iatom], ifsx);
+        atomicAdd(&args.fsoa[4 * args.stride + iatom], ifsy);
+        atomicAdd(&args.fsoa[5 * args.stride + iatom], ifsz);
       }
     }
+
     if (doPairlist) {
-      const unsigned full = mask | __shfl_down(mask, 32, 64);
-      const bool any = __any(full != 0u);
-      if (h == 0) args.tileExcls[jt].excl[i] = full;
-      if (lane == 0) args.jtileFlags[jt] = any ? 1 : 0;
-      depth += any ? 1 : 0;
+      const int a = waveSumInt(nPairsCut);
+      const int b = waveSumInt(nPairsList);
+      const int e = waveSumInt(nExcl);
+      if (lane == 0) {
+        args.listDepth[list] = depth;
+        args.listStat[list] = make_int4(depth, a, b, e);
+      }
     }
-    __syncthreads();
-  }
-
-  // i-forces: combine halves, atomics to SOA
-  ifx += __shfl_down(ifx, 32, 64);
-  ify += __shfl_down(ify, 32, 64);
-  ifz += __shfl_down(ifz, 32, 64);
-  if (doSlow) {
-    ifsx += __shfl_down(ifsx, 32, 64);
-    ifsy += __shfl_down(ifsy, 32, 64);
-    ifsz += __shfl_down(ifsz, 32, 64);
-  }
-  if (h == 0) {
-    atomicAdd(&args.fsoa[iatom], ifx);
-    atomicAdd(&args.fsoa[args.stride + iatom], ify);
-    atomicAdd(&args.fsoa[2 * args.stride + iatom], ifz);
-    if (doSlow) {
-      atomicAdd(&args.fsoa[3 * args.stride + iatom], ifsx);
-      atomicAdd(&args.fsoa[4 * args.stride + iatom], ifsy);
-      atomicAdd(&args.fsoa[5 * args.stride + iatom], ifsz);
+    if (doVirial) {
+      // per-list i-force sums over all 64 lanes, FP64
+      const double sx2 = waveSumD(vfx.v), sy2 = waveSumD(vfy.v), sz2 = waveSumD(vfz.v);
+      double ssx = 0.0, ssy = 0.0, ssz = 0.0;
+      if (doSlow) { ssx = waveSumD(vfsx.v); ssy = waveSumD(vfsy.v); ssz = waveSumD(vfsz.v); }
+      if (lane == 0) {
+        TileListVirialEnergy& ve = args.virialEnergy[list];
+        ve.shx = tl.offx; ve.shy = tl.offy; ve.shz = tl.offz;
+        ve.forcex = sx2; ve.forcey = sy2; ve.forcez = sz2;
+        ve.forceSlowx = ssx; ve.forceSlowy = ssy; ve.forceSlowz = ssz;
+      }
     }
-  }
-
-  if (doPairlist) {
-    const int a = waveSumInt(nPairsCut);
-    const int b = waveSumInt(nPairsList);
-    const int e = waveSumInt(nExcl);
-    if (lane == 0) {
-      args.listDepth[list] = depth;
-      args.listStat[list] = make_int4(depth, a, b, e);
-    }
-  }
-  if (doVirial) {
-    // per-list i-force sums (lanes 0..31 hold the combined values; zero the rest)
-    const float m = (h == 0) ? 1.0f : 0.0f;
-    const float sx = waveSum(ifx * m), sy = waveSum(ify * m), sz = waveSum(ifz * m);
-    float ssx = 0.f, ssy = 0.f, ssz = 0.f;
-    if (doSlow) { ssx = waveSum(ifsx * m); ssy = waveSum(ifsy * m); ssz = waveSum(ifsz * m); }
-    if (lane == 0) {
-      TileListVirialEnergy& ve = args.virialEnergy[list];
-      ve.shx = tl.offx; ve.shy = tl.offy; ve.shz = tl.offz;
-      ve.forcex = sx; ve.forcey = sy; ve.forcez = sz;
-      ve.forceSlowx = ssx; ve.forceSlowy = ssy; ve.forceSlowz = ssz;
-    }
-  }
-  if (doEnergy) {
-    const double a = waveSumD(eVdw.v), b = waveSumD(eElec.v), e = doSlow ? waveSumD(eSlow.v) : 0.0;
-    if (lane == 0) {
-      TileListVirialEnergy& ve = args.virialEnergy[list];
-      ve.energyVdw = a; ve.energyElec = b; ve.energySlow = e;
+    if (doEnergy) {
+      const double a = waveSumD(eVdw.v), b = waveSumD(eElec.v), e = doSlow ? waveSumD(eSlow.v) : 0.0;
+      if (lane == 0) {
+        TileListVirialEnergy& ve = args.virialEnergy[list];
+        ve.energyVdw = a; ve.energyElec = b; ve.energySlow = e;
+      }
     }
   }
 }
@@ -705,12 +727,22 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
   a.fsoa = h.fsoa;
   a.stride = h.stride;
   a.virialEnergy = h.virialEnergy;
-  dim3 g(maxLists), b(64);
+  // persistent grid: enough workgroups to fill every CU several times over
+  static int numCU = 0;
+  if (numCU == 0) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return hipErrorUnknown;
+    numCU = prop.multiProcessorCount;
+  }
+  const int wgNeeded = (maxLists + NB_WAVES - 1) / NB_WAVES;
+  dim3 g(std::min(wgNeeded, numCU * h.wgPerCU)), b(64 * NB_WAVES);
+  const size_t shmem = sizeof(float2) * (size_t)h.vdwWidth * h.vdwWidth;
   const int key = (doEnergy ? 8 : 0) | (doVirial ? 4 : 0) | (doSlow ? 2 : 0) | (doPairlist ? 1 : 0);
   switch (key) {
 #define NB_CASE(E, V, S, P)                                                                  \
   case (E * 8 + V * 4 + S * 2 + P):                                                          \
-    hipLaunchKernelGGL((nonbondedForceKernel<E, V, S, P>), g, b, 0, s, a);                   \
+    hipLaunchKernelGGL((nonbondedForceKernel<E, V, S, P>), g, b, shmem, s, a);               \
     break;
     NB_CASE(0, 0, 0, 0) NB_CASE(0, 0, 0, 1) NB_CASE(0, 0, 1, 0) NB_CASE(0, 0, 1, 1)
     NB_CASE(0, 1, 0, 0) NB_CASE(0, 1, 0, 1) NB_CASE(0, 1, 1, 0) NB_CASE(0, 1, 1, 1)

@@ -99,12 +99,15 @@ struct TileListStat {
 };
 
 // TileListVirialEnergy (src/CudaTileListKernel.h:72-94): the per-list partials
-// that the FP64 reduction consumes.  Per-list i-force sums are FP32 (as in
-// NAMD); per-list energies are FP64 (accumulated per lane in FP64).
+// that the FP64 reduction consumes.  NAMD keeps the per-list i-force sums in
+// FP32; here they are accumulated per lane in FP64 on virial steps, because
+// they are multiplied by image shifts of up to a box length and the slow
+// virial is a strongly cancelling sum.  Per-list energies are FP64 too.
 struct TileListVirialEnergy {
   float shx, shy, shz;
-  float forcex, forcey, forcez;
-  float forceSlowx, forceSlowy, forceSlowz;
+  float pad;
+  double forcex, forcey, forcez;
+  double forceSlowx, forceSlowy, forceSlowz;
   double energyVdw, energyElec, energySlow;
 };
 

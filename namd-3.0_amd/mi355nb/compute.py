@@ -109,6 +109,9 @@ class ComputeNonbonded:
     def set_timing(self, enable=True):
         L.check(L.lib().mi355nb_compute_set_timing(self._h, int(enable)), "set_timing")
 
+    def set_launch_config(self, workgroups_per_cu):
+        L.check(L.lib().mi355nb_compute_set_launch_config(self._h, int(workgroups_per_cu)), "set_launch_config")
+
     def kernel_time(self):
         """(ms_pairlist_total, n_pairlist, ms_force_total, n_force) since the last call; resets."""
         a, b = C.c_double(0), C.c_double(0)

@@ -57,7 +57,7 @@ EXPORTED = [
     "mi355nb_get_do_table", "mi355nb_lj_mix", "mi355nb_build_exclusions", "mi355nb_tile_excl_to_namd",
     "mi355nb_compute_create", "mi355nb_compute_destroy", "mi355nb_compute_set_molecule",
     "mi355nb_compute_assign_patches", "mi355nb_compute_update_positions", "mi355nb_compute_launch",
-    "mi355nb_compute_finish", "mi355nb_compute_set_timing", "mi355nb_compute_get_kernel_time",
+    "mi355nb_compute_finish", "mi355nb_compute_set_timing", "mi355nb_compute_set_launch_config", "mi355nb_compute_get_kernel_time",
     "mi355nb_compute_get_info", "mi355nb_compute_get_device_buffers",
     "mi355nb_compute_get_nonbonded_coef", "mi355nb_compute_get_effective_positions",
     "mi355nb_compute_get_tile_lists", "mi355nb_spread_charge", "mi355nb_gather_force", "mi355nb_self_energy",
@@ -94,6 +94,7 @@ def lib():
     L.mi355nb_compute_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.mi355nb_compute_finish.argtypes = [vp, vp, vp, C.POINTER(VirialEnergy), C.POINTER(TileListStat)]
     L.mi355nb_compute_set_timing.argtypes = [vp, C.c_int]
+    L.mi355nb_compute_set_launch_config.argtypes = [vp, C.c_int]
     L.mi355nb_compute_get_kernel_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32),
                                                   C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.mi355nb_compute_get_info.argtypes = [vp, C.POINTER(LayoutInfo)]
