@@ -50,6 +50,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default pair-kernel launch shape")
+    ap.add_argument("--wg-per-cu", type=int, default=0,
+                    help="fix the pair kernel's persistent workgroups per CU instead of autotuning")
+    ap.add_argument("--workload", choices=("solvated96k", "batch64"), default="solvated96k",
+                    help="solvated96k (default): the 96k-atom step; batch64: 64 uneven alchemical systems in one "
+                         "launch against the same systems run one at a time")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -65,6 +70,9 @@ def main():
     if not torch.cuda.is_available():
         log("no GPU visible")
         sys.exit(1)
+    if args.workload == "batch64":
+        run_batch(args)
+        return
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
@@ -116,7 +124,10 @@ def main():
         step(k)
     torch.cuda.synchronize()
     tuned = None
-    if not args.no_autotune:
+    if args.wg_per_cu:
+        c.set_launch_config(args.wg_per_cu)
+        tuned = {"workgroups_per_cu": args.wg_per_cu, "fixed": True}
+    elif not args.no_autotune:
         from mi355nb.autotune import tune_pair_kernel
         best, table = tune_pair_kernel(c)
         tuned = {"workgroups_per_cu": best, "ms_by_workgroups_per_cu": {str(k): round(v, 4) for k, v in table.items()}}
@@ -201,6 +212,119 @@ def main():
     if not args.no_cpu_baseline:
         out.update(cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args, ms_step))
     print(json.dumps(out), flush=True)
+    c.close()
+
+
+def run_batch(args):
+    """64 independent water + chain systems of 3k-40k atoms (log-uniform), each
+    with its own box and a 50-atom alchemical solute at its own lambda.
+    Batched: one compute object, every non-pairlist step replayed from a HIP
+    graph (one pair-kernel launch for all systems).  One at a time: one
+    compute object per system, launched back to back on the same stream
+    (eager, and replayed from per-system graphs); the faster of the two is the
+    reference.  Pairlists + energies every pairlist_every steps in both."""
+    import torch
+    from mi355nb import compute as mc
+    from mi355nb import lib as L
+    from mi355nb import systems
+
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    log("building 64 batched systems")
+    members = systems.batch_systems(64)
+    sysl = [m.system for m in members]
+    c = mc.ComputeNonbonded.from_systems(sysl, stream=sp)
+    part = np.concatenate([m.partition for m in members])
+    alchs = [L.alch_data(c.params, m.lam, min(1.0, m.lam + 0.05)) for m in members]
+    c.set_alch(part, alchs)
+    every = args.pairlist_every
+
+    def bstep(k):
+        if k % every == 0:
+            c.launch_work(True, True, True, True)
+        else:
+            c.launch_graph(False, False, True)
+
+    def timed(step, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(n):
+            step(k)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    for k in range(max(1, args.warmup)):
+        bstep(k)
+    torch.cuda.synchronize()
+    tuned = None
+    if not args.no_autotune:
+        from mi355nb.autotune import tune_pair_kernel
+        best, table = tune_pair_kernel(c)
+        tuned = {"workgroups_per_cu": best, "ms_by_workgroups_per_cu": {str(k): round(v, 4) for k, v in table.items()}}
+        log(f"autotune: {tuned}")
+        for k in range(max(1, args.warmup)):
+            bstep(k)
+    t_batch = timed(bstep, args.steps)
+    per, stat, _, _ = c.finish_batch(forces=False)
+    npairs = int(stat["numPairsCutoff"])
+    c.set_timing(True)
+    for k in range(1, every):
+        bstep(k)
+    _, _, ms_f, n_f = c.kernel_time()
+    c.set_timing(False)
+
+    log("building 64 single-system computes")
+    singles = []
+    for m, a in zip(members, alchs):
+        cs = mc.ComputeNonbonded.from_system(m.system, stream=sp)
+        cs.set_alch(m.partition, [a])
+        singles.append(cs)
+
+    def sstep_eager(k):
+        pl = k % every == 0
+        for cs in singles:
+            cs.launch_work(do_energy=pl, do_virial=pl, do_slow=True, do_pairlist=pl)
+
+    def sstep_graph(k):
+        pl = k % every == 0
+        for cs in singles:
+            if pl:
+                cs.launch_work(True, True, True, True)
+            else:
+                cs.launch_graph(False, False, True)
+
+    for k in range(max(1, args.warmup)):
+        sstep_eager(k)
+    t_eager = timed(sstep_eager, args.steps)
+    for k in range(max(1, args.warmup)):
+        sstep_graph(k)
+    t_graph = timed(sstep_graph, args.steps)
+    t_single = min(t_eager, t_graph)
+    n_atoms = int(sum(s.natoms for s in sysl))
+    agg_ns_day = len(sysl) * 2e-6 * 86400.0 / t_batch
+    out = {
+        "metric": "batched-replica throughput: 64 uneven alchemical systems per launch vs one at a time",
+        "value": round(agg_ns_day, 2),
+        "unit": "aggregate ns/day (sum over the 64 systems)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_batch * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "batch64: 64 water + chain boxes, 3k-40k atoms log-uniform (seed 7), density "
+                               "0.102 A^-3, rc 12 A, switch 10 A, pairlist 13.5 A; 50-atom soft-core solute per "
+                               "system, lambda ~ U[0,1], lambda2 = lambda + 0.05; step = nonbonded fast + slow "
+                               "forces (force-only steps replayed from a HIP graph), pairlist + energy/virial "
+                               "every 20 steps",
+                   "natoms": n_atoms, "systems": len(sysl), "pairlist_every": every},
+        "speedup_vs_one_at_a_time": round(t_single / t_batch, 3),
+        "one_at_a_time_ms_per_step": {"eager": round(t_eager * 1e3, 4), "graph": round(t_graph * 1e3, 4)},
+        "pair_kernel_ms": round(ms_f / max(n_f, 1), 4),
+        "gpairs_per_s": round(npairs / (ms_f / max(n_f, 1) * 1e-3) / 1e9, 2),
+        "pairs_in_cutoff": npairs,
+        "autotune": tuned,
+    }
+    print(json.dumps(out), flush=True)
+    for cs in singles:
+        cs.close()
     c.close()
 
 

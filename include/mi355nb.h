@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MI355NB_ABI_VERSION 1
+#define MI355NB_ABI_VERSION 2
 
 #define MI355NB_OK 0
 #define MI355NB_ERR_ARG -1
@@ -85,8 +85,29 @@ typedef struct {
   int32_t numCandidateLists;    /* (compute, i-tile) lists before refinement */
   int32_t patchDims[3];
   int64_t jtileCapacity;
-  double box[3];                /* edge lengths after quantisation to 2^-17 A */
+  double box[3];                /* edge lengths after quantisation to 2^-17 A (system 0) */
+  int32_t numSystems;           /* independent systems of a batch (1 otherwise) */
 } mi355nb_layout_info;
+
+/* AlchData (src/CudaComputeNonbondedKernel.h:24-50), same fields and order;
+ * alchDecouple is an int32 here.  Filled per lambda window by
+ * mi355nb_alch_data_fill as CudaComputeNonbonded::launchWork does
+ * (src/CudaComputeNonbonded.C:1424-1455). */
+typedef struct {
+  float scaling;
+  float switchdist2;
+  float cutoff2;
+  float switchfactor;
+  float alchVdwShiftCoeff;
+  float lambdaUp, lambdaDown;
+  float elecLambdaUp, elecLambdaDown;
+  float vdwLambdaUp, vdwLambdaDown;
+  float lambda2Up, lambda2Down;
+  float elecLambda2Up, elecLambda2Down;
+  float vdwLambda2Up, vdwLambda2Down;
+  float vdwShiftUp, vdwShift2Up, vdwShiftDown, vdwShift2Down;
+  int32_t alchDecouple;
+} mi355nb_alch_data;
 
 /* Device pointers owned by a compute object (valid until the next
  * mi355nb_compute_assign_patches or destroy). */
@@ -150,6 +171,33 @@ int mi355nb_compute_set_molecule(mi355nb_compute* c, int natoms, const int32_t* 
  * computes, ORB-sorted padded storage, and uploads.  pos: natoms*3 doubles. */
 int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], const double* pos);
 
+/* Batched systems (no counterpart in the reference, which runs one system
+ * per process): numSystems independent periodic boxes whose atoms are
+ * concatenated in the molecule given to set_molecule; system s owns atoms
+ * [sysAtomStart[s], sysAtomStart[s+1]) (numSystems+1 entries, first 0, last
+ * natoms) and has box edges boxes[3s..3s+2].  Each system gets its own patch
+ * grid and computes, as assign_patches would build them; one launch then
+ * evaluates all systems, and finish_batch returns one virial/energy record
+ * per system.  Exclusions must not pair atoms of two systems. */
+int mi355nb_compute_assign_patches_batch(mi355nb_compute* c, int numSystems, const int32_t* sysAtomStart,
+                                         const double* boxes, const double* pos);
+
+/* Alchemical FEP factors for one lambda window (replaces the AlchData fill of
+ * src/CudaComputeNonbonded.C:1424-1455 with SimParameters::getElecLambda /
+ * getVdwLambda, src/SimParameters.C:8182-8210; alchWCA off).  p supplies
+ * cutoff and switchdist. */
+int mi355nb_alch_data_fill(const mi355nb_nb_params* p, double lambda, double lambda2, double elecLambdaStart,
+                           double vdwLambdaEnd, double vdwShiftCoeff, int decouple, mi355nb_alch_data* out);
+
+/* Alchemical partitions (0 environment, 1 appearing, 2 disappearing; int8 per
+ * atom, the char part[] of CudaComputeNonbonded::updateVdwTypesExcl,
+ * src/CudaComputeNonbonded.C:828-896) and one AlchData per system: later
+ * launches use the FEP pair math of calcForceEnergyFEP
+ * (src/CudaComputeNonbondedKernel.cu:163-354; potential switching) and fill the
+ * energy*_s (lambda2) fields.  partition == NULL switches it off. */
+int mi355nb_compute_set_alch(mi355nb_compute* c, const int8_t* partition, int numSystems,
+                             const mi355nb_alch_data* perSystem);
+
 /* Per-step xyzq staging for the current assignment (openBoxesOnPe /
  * copyAtomsSubset, src/CudaComputeNonbonded.C:903-1052) and upload. */
 int mi355nb_compute_update_positions(mi355nb_compute* c, const double* pos);
@@ -167,6 +215,18 @@ int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int d
 int mi355nb_compute_finish(mi355nb_compute* c, double* force, double* forceSlow,
                            mi355nb_virial_energy* ve, mi355nb_tile_list_stat* stat);
 
+/* finish for a batch: ve receives numSystems records (system order). */
+int mi355nb_compute_finish_batch(mi355nb_compute* c, double* force, double* forceSlow,
+                                 mi355nb_virial_energy* vePerSystem, mi355nb_tile_list_stat* stat);
+
+/* A non-pairlist launch replayed from a HIP graph: the first call for each
+ * (doEnergy, doVirial, doSlow) captures the launch sequence on the compute's
+ * stream (which must not be NULL); later calls replay it.  Graphs are dropped
+ * whenever the layout, molecule, alchemical data or launch shape change.  No
+ * counterpart in this form in the reference (NAMD's GPU-resident integrator
+ * captures its own step, src/SequencerCUDA.C). */
+int mi355nb_compute_launch_graph(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow);
+
 /* Kernel timing with HIP events recorded on the compute's stream around every
  * pair-kernel launch (no counterpart in the reference; NAMD brackets the
  * kernel with Projections events, src/CudaComputeNonbonded.C:1957-1973).
@@ -174,8 +234,9 @@ int mi355nb_compute_finish(mi355nb_compute* c, double* force, double* forceSlow,
  * launch counts since the last reset, split by pairlist / non-pairlist. */
 int mi355nb_compute_set_timing(mi355nb_compute* c, int enable);
 
-/* Launch shape of the pair kernel: persistent workgroups (4 waves each) per
- * CU, 1..64 (default 8).  Chosen by the autotuner in bench.py; no counterpart
+/* Launch shape of the pair kernel on non-pairlist steps: persistent
+ * workgroups (4 waves each) per CU, 1..64, or 0 (default) for one wave per
+ * tile list.  Chosen by mi355nb.autotune; no counterpart
  * in the reference (NAMD fixes its grid at compile time,
  * src/CudaComputeNonbondedKernel.cu:2164-2167). */
 int mi355nb_compute_set_launch_config(mi355nb_compute* c, int workgroupsPerCU);

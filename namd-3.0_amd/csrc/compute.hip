@@ -57,7 +57,23 @@ struct mi355nb_compute {
 
   PatchLayout L;
   int numCandLists = 0;
-  int wgPerCU = 8;     // persistent pair-kernel workgroups per CU
+  int wgPerCU = 0;     // persistent pair-kernel workgroups per CU; 0: one wave per list
+
+  // alchemical partitions (optional)
+  bool alchOn = false;
+  std::vector<signed char> partByAtom;
+  std::vector<AlchParams> alchHost;
+
+  // per-system reduction layout
+  std::vector<int4> redBlocksHost;
+  std::vector<int> redSysStartHost;
+
+  // captured force-only steps, keyed by (doEnergy, doVirial, doSlow)
+  hipGraphExec_t graphs[8] = {};
+  void dropGraphs() {
+    for (auto& g : graphs)
+      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  }
 
   // device buffers
   DevBuf<F4> xyzq, force, forceSlow;
@@ -81,6 +97,10 @@ struct mi355nb_compute {
   DevBuf<VirialEnergy> ve;
   DevBuf<TileListStat> stat;
   DevBuf<int> numCand;
+  DevBuf<AlchParams> alch;
+  DevBuf<signed char> part;
+  DevBuf<int4> redBlocks;
+  DevBuf<int> redSysStart;
 
   // host staging
   std::vector<F4> hForce, hForceSlow;
@@ -108,6 +128,8 @@ struct mi355nb_compute {
   }
 
   void releaseAll() {
+    dropGraphs();
+    alch.release(); part.release(); redBlocks.release(); redSysStart.release();
     xyzq.release(); force.release(); forceSlow.release();
     vdwTypes.release(); atomIndex.release(); excl.release(); bits.release(); vdwCoef.release();
     patches.release(); bbox.release(); listsCand.release(); listsAct.release();
@@ -270,6 +292,10 @@ int mi355nb_compute_set_molecule(mi355nb_compute* c, int natoms, const int32_t* 
   c->haveMolecule = true;
   c->haveLayout = false;
   c->haveLists = false;
+  c->alchOn = false;
+  c->alchHost.clear();
+  c->partByAtom.clear();
+  c->dropGraphs();
   return MI355NB_OK;
 }
 
@@ -279,22 +305,42 @@ static int uploadSlots(mi355nb_compute* c) {
   return MI355NB_OK;
 }
 
-int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], const double* pos) {
-  CHECK_ARG(c && box && pos, "null argument");
-  CHECK_STATE(c->haveMolecule, "set_molecule must precede assign_patches");
-  for (int d = 0; d < 3; ++d)
-    CHECK_ARG(box[d] >= 2.0 * c->p.pairlistdist, "box edge must be at least twice pairlistdist");
+static int uploadPartitions(mi355nb_compute* c) {
+  const int n = c->L.atomStorageSize;
+  std::vector<signed char> slotPart(n, 0);
+  for (int k = 0; k < n; ++k) {
+    const int a = c->L.atomIndex[k];
+    if (a >= 0) slotPart[k] = c->partByAtom[a];
+  }
+  MI355NB_HIP(c->part.reserve(n));
+  MI355NB_HIP(hipMemcpy(c->part.ptr, slotPart.data(), n, hipMemcpyHostToDevice));
+  MI355NB_HIP(c->alch.reserve(c->alchHost.size()));
+  MI355NB_HIP(hipMemcpy(c->alch.ptr, c->alchHost.data(), c->alchHost.size() * sizeof(AlchParams),
+                        hipMemcpyHostToDevice));
+  return MI355NB_OK;
+}
+
+static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomStart, const double* boxes,
+                             const double* pos) {
+  for (int s = 0; s < nsys; ++s)
+    for (int d = 0; d < 3; ++d)
+      CHECK_ARG(boxes[3 * s + d] >= 2.0 * c->p.pairlistdist, "box edge must be at least twice pairlistdist");
   for (int i = 0; i < 3 * c->natoms; ++i) CHECK_ARG(std::isfinite(pos[i]), "non-finite position");
   std::string err;
-  if (!c->L.build(c->natoms, pos, box, c->p.pairlistdist + c->p.margin, err)) {
+  std::vector<int> start(atomStart, atomStart + nsys + 1);
+  if (!c->L.buildBatch(nsys, start.data(), pos, boxes, c->p.pairlistdist + c->p.margin, err)) {
     setLastError(err);
     return MI355NB_ERR_ARG;
   }
+  c->dropGraphs();
   c->L.pack(pos, c->qScaled.data(), c->vdwType.data(), c->exclByAtom.data());
   const int n = c->L.atomStorageSize;
   const int nl = (int)c->L.tileListSkeleton.size();
   const long long cap = std::max<long long>(1, c->L.jtileCapacity);
   c->numCandLists = nl;
+  buildReduceBlocks(c->L.sysSlotStart.data(), c->L.sysListStart.data(), nsys, c->redBlocksHost,
+                    c->redSysStartHost);
+  const int nrb = (int)c->redBlocksHost.size();
   MI355NB_HIP(hipSetDevice(c->device));
   MI355NB_HIP(c->xyzq.reserve(n));
   MI355NB_HIP(c->force.reserve(n));
@@ -321,11 +367,12 @@ int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], cons
   MI355NB_HIP(c->sortValsOut.reserve(nl + 1));
   MI355NB_HIP(c->newStart.reserve(nl + 1));
   MI355NB_HIP(c->tlve.reserve(nl + 1));
-  MI355NB_HIP(c->partials.reserve((size_t)reducePartialBlocks(n, nl) * 21 + 21));
-  MI355NB_HIP(c->ve.reserve(1));
+  MI355NB_HIP(c->redBlocks.reserve(std::max(nrb, 1)));
+  MI355NB_HIP(c->redSysStart.reserve(nsys + 1));
+  MI355NB_HIP(c->partials.reserve((size_t)std::max(nrb, 1) * RED_VALUES));
+  MI355NB_HIP(c->ve.reserve(nsys));
   MI355NB_HIP(c->stat.reserve(1));
   MI355NB_HIP(c->numCand.reserve(1));
-  const size_t fsoaOld = c->fsoa.cap;
   MI355NB_HIP(c->fsoa.reserve(6 * (size_t)n));
   // scratch for the scan and the radix sort
   size_t b1 = 0, b2 = 0;
@@ -343,13 +390,132 @@ int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], cons
   MI355NB_HIP(hipMemcpyAsync(c->listsCand.ptr, c->L.tileListSkeleton.data(), nl * sizeof(TileList),
                              hipMemcpyHostToDevice, c->stream));
   MI355NB_HIP(hipMemcpyAsync(c->numCand.ptr, &c->numCandLists, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  MI355NB_HIP(hipMemcpyAsync(c->redBlocks.ptr, c->redBlocksHost.data(), nrb * sizeof(int4), hipMemcpyHostToDevice,
+                             c->stream));
+  MI355NB_HIP(hipMemcpyAsync(c->redSysStart.ptr, c->redSysStartHost.data(), (nsys + 1) * sizeof(int),
+                             hipMemcpyHostToDevice, c->stream));
   MI355NB_HIP(launchFill(c->fsoa.cap, c->fsoa.ptr, 0.0f, c->stream));
-  (void)fsoaOld;
   int rc = uploadSlots(c);
   if (rc) return rc;
   MI355NB_HIP(hipStreamSynchronize(c->stream));
+  if (c->alchOn) {
+    if ((int)c->alchHost.size() != nsys) {
+      c->alchOn = false;
+      c->alchHost.clear();
+    } else {
+      rc = uploadPartitions(c);
+      if (rc) return rc;
+    }
+  }
   c->haveLayout = true;
   c->haveLists = false;
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], const double* pos) {
+  CHECK_ARG(c && box && pos, "null argument");
+  CHECK_STATE(c->haveMolecule, "set_molecule must precede assign_patches");
+  const int32_t start[2] = {0, c->natoms};
+  return assignPatchesImpl(c, 1, start, box, pos);
+}
+
+int mi355nb_compute_assign_patches_batch(mi355nb_compute* c, int numSystems, const int32_t* sysAtomStart,
+                                         const double* boxes, const double* pos) {
+  CHECK_ARG(c && sysAtomStart && boxes && pos && numSystems >= 1, "bad batch arguments");
+  CHECK_STATE(c->haveMolecule, "set_molecule must precede assign_patches_batch");
+  CHECK_ARG(sysAtomStart[0] == 0 && sysAtomStart[numSystems] == c->natoms,
+            "system ranges must start at 0 and end at natoms");
+  std::vector<int> sysOf(c->natoms, 0);
+  for (int s = 0; s < numSystems; ++s) {
+    CHECK_ARG(sysAtomStart[s + 1] > sysAtomStart[s], "every system needs at least one atom");
+    for (int a = sysAtomStart[s]; a < sysAtomStart[s + 1]; ++a) sysOf[a] = s;
+  }
+  // exclusions may not pair atoms of two systems: the bit table is indexed by
+  // atom-index difference, which the batch keeps within one system
+  for (int a = 0; a < c->natoms; ++a) {
+    const I2 sm = c->exclByAtom[a];
+    const int lo = std::max(0, a - sm.y), hi = std::min(c->natoms - 1, a + sm.y);
+    for (int b = lo; b <= hi; ++b)
+      if (b != a && sysOf[b] != sysOf[a] && isExcluded(sm, c->exclBitsHost.data(), a, b))
+        CHECK_ARG(false, "an exclusion pairs atoms of two different systems");
+  }
+  return assignPatchesImpl(c, numSystems, sysAtomStart, boxes, pos);
+}
+
+int mi355nb_alch_data_fill(const mi355nb_nb_params* p, double lambda, double lambda2, double elecLambdaStart,
+                           double vdwLambdaEnd, double vdwShiftCoeff, int decouple, mi355nb_alch_data* out) {
+  CHECK_ARG(p && out, "null argument");
+  CHECK_ARG(lambda >= 0 && lambda <= 1 && lambda2 >= 0 && lambda2 <= 1, "lambda outside [0, 1]");
+  CHECK_ARG(elecLambdaStart >= 0 && elecLambdaStart < 1 && vdwLambdaEnd > 0 && vdwLambdaEnd <= 1 &&
+                vdwShiftCoeff >= 0,
+            "bad alchemical schedule");
+  // SimParameters::getElecLambda / getVdwLambda (src/SimParameters.C:8182-8210), no WCA
+  auto elecL = [&](double l) { return l <= elecLambdaStart ? 0.0 : (l - elecLambdaStart) / (1.0 - elecLambdaStart); };
+  auto vdwL = [&](double l) { return l >= vdwLambdaEnd ? 1.0 : l / vdwLambdaEnd; };
+  // CudaComputeNonbonded::launchWork (src/CudaComputeNonbonded.C:1424-1455)
+  std::memset(out, 0, sizeof(*out));
+  out->scaling = 1.0f;
+  out->cutoff2 = (float)(p->cutoff * p->cutoff);
+  out->switchdist2 = (float)(p->switchdist * p->switchdist);
+  const double factor = p->cutoff * p->cutoff - p->switchdist * p->switchdist;
+  out->switchfactor = factor > 0 ? (float)(1.0 / (factor * factor * factor)) : 0.0f;
+  out->alchVdwShiftCoeff = (float)vdwShiftCoeff;
+  out->lambdaUp = (float)lambda;
+  out->lambdaDown = (float)(1.0 - lambda);
+  out->elecLambdaUp = (float)elecL(lambda);
+  out->elecLambdaDown = (float)elecL(1.0 - lambda);
+  out->vdwLambdaUp = (float)vdwL(lambda);
+  out->vdwLambdaDown = (float)vdwL(1.0 - lambda);
+  out->lambda2Up = (float)lambda2;
+  out->lambda2Down = (float)(1.0 - lambda2);
+  out->elecLambda2Up = (float)elecL(lambda2);
+  out->elecLambda2Down = (float)elecL(1.0 - lambda2);
+  out->vdwLambda2Up = (float)vdwL(lambda2);
+  out->vdwLambda2Down = (float)vdwL(1.0 - lambda2);
+  out->vdwShiftUp = (float)(vdwShiftCoeff * (1.0 - out->vdwLambdaUp));
+  out->vdwShift2Up = (float)(vdwShiftCoeff * (1.0 - out->vdwLambda2Up));
+  out->vdwShiftDown = (float)(vdwShiftCoeff * (1.0 - out->vdwLambdaDown));
+  out->vdwShift2Down = (float)(vdwShiftCoeff * (1.0 - out->vdwLambda2Down));
+  out->alchDecouple = decouple ? 1 : 0;
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_set_alch(mi355nb_compute* c, const int8_t* partition, int numSystems,
+                             const mi355nb_alch_data* perSystem) {
+  CHECK_ARG(c, "null compute");
+  CHECK_STATE(c->haveMolecule, "set_molecule must precede set_alch");
+  c->dropGraphs();
+  if (!partition) {
+    c->alchOn = false;
+    c->alchHost.clear();
+    c->partByAtom.clear();
+    return MI355NB_OK;
+  }
+  CHECK_ARG(perSystem && numSystems >= 1, "bad alchemical arguments");
+  if (c->haveLayout) CHECK_ARG(numSystems == c->L.numSystems, "one alchemical record per system");
+  for (int i = 0; i < c->natoms; ++i) CHECK_ARG(partition[i] >= 0 && partition[i] <= 2, "partition must be 0, 1 or 2");
+  const float c2 = (float)(c->p.cutoff * c->p.cutoff), s2 = (float)(c->p.switchdist * c->p.switchdist);
+  std::vector<AlchParams> ap(numSystems);
+  for (int s = 0; s < numSystems; ++s) {
+    const mi355nb_alch_data& d = perSystem[s];
+    CHECK_ARG(std::fabs(d.cutoff2 - c2) <= 1e-6f * c2 && std::fabs(d.switchdist2 - s2) <= 1e-6f * c2 &&
+                  d.scaling == 1.0f,
+              "alchemical record does not match the compute's cutoff / switchdist / scaling");
+    AlchParams& a = ap[s];
+    std::memset(&a, 0, sizeof(a));
+    a.elecLambdaUp = d.elecLambdaUp; a.elecLambdaDown = d.elecLambdaDown;
+    a.vdwLambdaUp = d.vdwLambdaUp; a.vdwLambdaDown = d.vdwLambdaDown;
+    a.elecLambda2Up = d.elecLambda2Up; a.elecLambda2Down = d.elecLambda2Down;
+    a.vdwLambda2Up = d.vdwLambda2Up; a.vdwLambda2Down = d.vdwLambda2Down;
+    a.vdwShiftUp = d.vdwShiftUp; a.vdwShiftDown = d.vdwShiftDown;
+    a.vdwShift2Up = d.vdwShift2Up; a.vdwShift2Down = d.vdwShift2Down;
+    a.decouple = d.alchDecouple ? 1 : 0;
+  }
+  c->partByAtom.assign(partition, partition + c->natoms);
+  c->alchHost = ap;
+  c->alchOn = true;
+  MI355NB_HIP(hipSetDevice(c->device));
+  if (c->haveLayout) return uploadPartitions(c);
   return MI355NB_OK;
 }
 
@@ -363,16 +529,15 @@ int mi355nb_compute_update_positions(mi355nb_compute* c, const double* pos) {
   return uploadSlots(c);
 }
 
-int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow, int doPairlist) {
-  CHECK_ARG(c, "null compute");
-  CHECK_STATE(c->haveLayout, "assign_patches must precede launch");
-  CHECK_STATE(doPairlist || c->haveLists, "the first launch after assign_patches must build pairlists");
-  MI355NB_HIP(hipSetDevice(c->device));
+// Enqueues one evaluation on the compute's stream.  With capturing set no
+// timing events are recorded (the stream is being captured into a graph).
+static int launchImpl(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow, int doPairlist, bool capturing) {
   const int n = c->L.atomStorageSize;
   const int nl = c->numCandLists;
   const CutTest cut = makeCutTest(c->p.cutoff);
   const CutTest plcut = makeCutTest(c->p.pairlistdist);
   hipStream_t s = c->stream;
+  const bool timing = c->timing && !capturing;
 
   ForceArgsHost a;
   a.xyzq = c->xyzq.ptr;
@@ -393,8 +558,9 @@ int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int d
   a.stride = n;
   a.virialEnergy = c->tlve.ptr;
   a.wgPerCU = c->wgPerCU;
+  a.alch = c->alchOn ? c->alch.ptr : nullptr;
+  a.part = c->alchOn ? c->part.ptr : nullptr;
 
-  const int* numListsForReduce;
   if (doPairlist) {
     const float cull = (float)(c->p.pairlistdist + 0.01);
     MI355NB_HIP(launchBoundingBoxes(n / TILE, c->xyzq.ptr, c->bbox.ptr, s));
@@ -409,12 +575,12 @@ int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int d
     a.tileJatomStart = c->jCand.ptr;
     a.tileExcls = c->exCand.ptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->timing) {
+    if (timing) {
       e0 = c->takeEvent(); e1 = c->takeEvent();
       if (e0 && e1) MI355NB_HIP(hipEventRecord(e0, s));
     }
     MI355NB_HIP(launchNonbondedForce(nl, a, doEnergy, doVirial, doSlow, true, s));
-    if (c->timing && e0 && e1) {
+    if (timing && e0 && e1) {
       MI355NB_HIP(hipEventRecord(e1, s));
       c->evPairlist.emplace_back(e0, e1);
     }
@@ -429,28 +595,31 @@ int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int d
     MI355NB_HIP(launchRepack(nl, &c->stat.ptr->numTileLists, c->sortValsOut.ptr, c->newStart.ptr,
                              c->listsCand.ptr, c->jCand.ptr, c->exCand.ptr, c->jtileFlags.ptr, c->listsAct.ptr,
                              c->jAct.ptr, c->exAct.ptr, s));
-    numListsForReduce = c->numCand.ptr;
     c->haveLists = true;
   } else {
+    // lists dropped by the last refinement write no record: clear them all
+    if (doEnergy || doVirial)
+      MI355NB_HIP(launchFill((size_t)nl * sizeof(TileListVirialEnergy) / sizeof(float),
+                             reinterpret_cast<float*>(c->tlve.ptr), 0.0f, s));
     a.numLists = &c->stat.ptr->numTileLists;
     a.tileLists = c->listsAct.ptr;
     a.tileJatomStart = c->jAct.ptr;
     a.tileExcls = c->exAct.ptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->timing) {
+    if (timing) {
       e0 = c->takeEvent(); e1 = c->takeEvent();
       if (e0 && e1) MI355NB_HIP(hipEventRecord(e0, s));
     }
     MI355NB_HIP(launchNonbondedForce(nl, a, doEnergy, doVirial, doSlow, false, s));
-    if (c->timing && e0 && e1) {
+    if (timing && e0 && e1) {
       MI355NB_HIP(hipEventRecord(e1, s));
       c->evForce.emplace_back(e0, e1);
     }
-    numListsForReduce = &c->stat.ptr->numTileLists;
   }
   MI355NB_HIP(launchTransposeForces(n, n, c->fsoa.ptr, c->force.ptr, c->forceSlow.ptr, doSlow != 0, s));
   if (doEnergy || doVirial) {
-    MI355NB_HIP(launchReduceVirialEnergy(n, nl, c->xyzq.ptr, c->force.ptr, c->forceSlow.ptr, numListsForReduce,
+    MI355NB_HIP(launchReduceVirialEnergy((int)c->redBlocksHost.size(), c->L.numSystems, c->redBlocks.ptr,
+                                         c->redSysStart.ptr, c->xyzq.ptr, c->force.ptr, c->forceSlow.ptr,
                                          c->tlve.ptr, doEnergy != 0, doVirial != 0, doSlow != 0,
                                          c->partials.ptr, c->ve.ptr, s));
   }
@@ -458,12 +627,41 @@ int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int d
   return MI355NB_OK;
 }
 
-int mi355nb_compute_finish(mi355nb_compute* c, double* forceOut, double* forceSlowOut, mi355nb_virial_energy* veOut,
-                           mi355nb_tile_list_stat* statOut) {
+int mi355nb_compute_launch(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow, int doPairlist) {
   CHECK_ARG(c, "null compute");
-  CHECK_STATE(c->haveLayout, "nothing launched");
+  CHECK_STATE(c->haveLayout, "assign_patches must precede launch");
+  CHECK_STATE(doPairlist || c->haveLists, "the first launch after assign_patches must build pairlists");
   MI355NB_HIP(hipSetDevice(c->device));
-  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  return launchImpl(c, doEnergy, doVirial, doSlow, doPairlist, false);
+}
+
+int mi355nb_compute_launch_graph(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow) {
+  CHECK_ARG(c, "null compute");
+  CHECK_STATE(c->haveLayout && c->haveLists, "a pairlist launch must precede launch_graph");
+  CHECK_STATE(c->stream != nullptr, "launch_graph needs a non-null stream (the null stream cannot be captured)");
+  MI355NB_HIP(hipSetDevice(c->device));
+  const int key = (doEnergy ? 4 : 0) | (doVirial ? 2 : 0) | (doSlow ? 1 : 0);
+  if (!c->graphs[key]) {
+    MI355NB_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = launchImpl(c, doEnergy, doVirial, doSlow, 0, true);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    MI355NB_HIP(e);
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MI355NB_HIP(ei);
+    c->graphs[key] = x;
+  }
+  MI355NB_HIP(hipGraphLaunch(c->graphs[key], c->stream));
+  return MI355NB_OK;
+}
+
+static int copyForces(mi355nb_compute* c, double* forceOut, double* forceSlowOut) {
   const int n = c->L.atomStorageSize;
   if (forceOut) {
     c->hForce.resize(n);
@@ -481,6 +679,17 @@ int mi355nb_compute_finish(mi355nb_compute* c, double* forceOut, double* forceSl
       forceSlowOut[3 * a] = f.x; forceSlowOut[3 * a + 1] = f.y; forceSlowOut[3 * a + 2] = f.z;
     }
   }
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_finish(mi355nb_compute* c, double* forceOut, double* forceSlowOut, mi355nb_virial_energy* veOut,
+                           mi355nb_tile_list_stat* statOut) {
+  CHECK_ARG(c, "null compute");
+  CHECK_STATE(c->haveLayout, "nothing launched");
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  int rc = copyForces(c, forceOut, forceSlowOut);
+  if (rc) return rc;
   if (veOut) {
     static_assert(sizeof(VirialEnergy) == sizeof(mi355nb_virial_energy), "VirialEnergy layout");
     MI355NB_HIP(hipMemcpy(veOut, c->ve.ptr, sizeof(VirialEnergy), hipMemcpyDeviceToHost));
@@ -492,8 +701,23 @@ int mi355nb_compute_finish(mi355nb_compute* c, double* forceOut, double* forceSl
   return MI355NB_OK;
 }
 
+int mi355nb_compute_finish_batch(mi355nb_compute* c, double* forceOut, double* forceSlowOut,
+                                 mi355nb_virial_energy* vePerSystem, mi355nb_tile_list_stat* statOut) {
+  CHECK_ARG(c, "null compute");
+  CHECK_STATE(c->haveLayout, "nothing launched");
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  int rc = copyForces(c, forceOut, forceSlowOut);
+  if (rc) return rc;
+  if (vePerSystem)
+    MI355NB_HIP(hipMemcpy(vePerSystem, c->ve.ptr, sizeof(VirialEnergy) * c->L.numSystems, hipMemcpyDeviceToHost));
+  if (statOut) MI355NB_HIP(hipMemcpy(statOut, c->stat.ptr, sizeof(TileListStat), hipMemcpyDeviceToHost));
+  return MI355NB_OK;
+}
+
 int mi355nb_compute_set_launch_config(mi355nb_compute* c, int workgroupsPerCU) {
-  CHECK_ARG(c && workgroupsPerCU >= 1 && workgroupsPerCU <= 64, "workgroupsPerCU must be in 1..64");
+  CHECK_ARG(c && workgroupsPerCU >= 0 && workgroupsPerCU <= 64, "workgroupsPerCU must be in 0..64");
+  c->dropGraphs();
   c->wgPerCU = workgroupsPerCU;
   return MI355NB_OK;
 }
@@ -545,6 +769,7 @@ int mi355nb_compute_get_info(const mi355nb_compute* c, mi355nb_layout_info* info
     info->box[d] = c->L.box[d];
   }
   info->jtileCapacity = c->L.jtileCapacity;
+  info->numSystems = c->L.numSystems;
   return MI355NB_OK;
 }
 

@@ -162,6 +162,83 @@ static inline double wrap(double x, double L) {
 
 bool PatchLayout::build(int n, const double* pos, const double boxIn[3], double patchDim,
                         std::string& err) {
+  const int start[2] = {0, n};
+  return buildBatch(1, start, pos, boxIn, patchDim, err);
+}
+
+bool PatchLayout::buildBatch(int nsys, const int* atomStart, const double* pos, const double* boxes,
+                             double patchDim, std::string& err) {
+  if (nsys < 1) { err = "no systems"; return false; }
+  for (int s = 0; s < nsys; ++s)
+    if (atomStart[s + 1] <= atomStart[s]) { err = "every system needs at least one atom"; return false; }
+  if (atomStart[0] != 0) { err = "system atom ranges must start at 0"; return false; }
+  numSystems = nsys;
+  sysAtomStart.assign(atomStart, atomStart + nsys + 1);
+  sysSlotStart.assign(nsys + 1, 0);
+  sysListStart.assign(nsys + 1, 0);
+  sysBox.assign(3 * (size_t)nsys, 0.0);
+  natoms = atomStart[nsys];
+  centre.clear(); patches.clear(); computes.clear(); tileListSkeleton.clear(); patchSystem.clear();
+  patchOfAtom.assign(natoms, 0);
+  slotOfAtom.assign(natoms, -1);
+  atomIndex.clear();
+  jtileCapacity = 0;
+  int slotOff = 0;
+  for (int s = 0; s < nsys; ++s) {
+    const int a0 = atomStart[s];
+    PatchLayout sub;
+    if (!sub.buildOne(atomStart[s + 1] - a0, pos + 3 * (size_t)a0, boxes + 3 * s, patchDim, err)) {
+      if (nsys > 1) err = "system " + std::to_string(s) + ": " + err;
+      return false;
+    }
+    if (s == 0)
+      for (int d = 0; d < 3; ++d) { dims[d] = sub.dims[d]; box[d] = sub.box[d]; patchSize[d] = sub.patchSize[d]; }
+    for (int d = 0; d < 3; ++d) sysBox[3 * s + d] = sub.box[d];
+    const int patchOff = (int)patches.size();
+    const int computeOff = (int)computes.size();
+    centre.insert(centre.end(), sub.centre.begin(), sub.centre.end());
+    for (PatchRecord pr : sub.patches) {
+      pr.atomStart += slotOff;
+      pr.patchID += patchOff;
+      patches.push_back(pr);
+      patchSystem.push_back(s);
+    }
+    for (ComputeRecord cr : sub.computes) {
+      cr.patchInd[0] += patchOff;
+      cr.patchInd[1] += patchOff;
+      computes.push_back(cr);
+    }
+    sysListStart[s] = (int)tileListSkeleton.size();
+    for (TileList tl : sub.tileListSkeleton) {
+      tl.iatomStart += slotOff;
+      tl.patchInd[0] += patchOff;
+      tl.patchInd[1] += patchOff;
+      tl.icompute += computeOff;
+      tl.sys = s;
+      tl.listId = (int)tileListSkeleton.size();
+      tileListSkeleton.push_back(tl);
+    }
+    jtileCapacity += sub.jtileCapacity;
+    for (int i = 0; i < sub.natoms; ++i) {
+      patchOfAtom[a0 + i] = sub.patchOfAtom[i] + patchOff;
+      slotOfAtom[a0 + i] = sub.slotOfAtom[i] + slotOff;
+    }
+    for (int a : sub.atomIndex) atomIndex.push_back(a < 0 ? -1 : a0 + a);
+    sysSlotStart[s] = slotOff;
+    slotOff += sub.atomStorageSize;
+  }
+  sysSlotStart[nsys] = slotOff;
+  sysListStart[nsys] = (int)tileListSkeleton.size();
+  atomStorageSize = slotOff;
+  xyzq.assign(atomStorageSize, F4{0, 0, 0, 0});
+  vdwTypes.assign(atomStorageSize, 0);
+  exclStartMaxdiff.assign(atomStorageSize, I2{0, 0});
+  effPos.assign(3 * (size_t)natoms, 0.0);
+  return true;
+}
+
+bool PatchLayout::buildOne(int n, const double* pos, const double boxIn[3], double patchDim,
+                           std::string& err) {
   natoms = n;
   for (int d = 0; d < 3; ++d) {
     box[d] = quantize(boxIn[d], POS_QUANTUM);
@@ -272,6 +349,8 @@ bool PatchLayout::build(int n, const double* pos, const double boxIn[3], double 
       tl.patchInd[1] = c.patchInd[1];
       tl.icompute = ci;
       tl.self = c.self;
+      tl.sys = 0;
+      tl.listId = (int)tileListSkeleton.size();
       tileListSkeleton.push_back(tl);
       jtileCapacity += c.self ? (nb - t) : nb;
     }
@@ -288,6 +367,7 @@ void PatchLayout::packPositions(const double* pos) {
   const int np = (int)patches.size();
   for (int p = 0; p < np; ++p) {
     const PatchRecord& pr = patches[p];
+    const double* box = &sysBox[3 * (size_t)patchSystem[p]];
     int padded = (pr.numAtoms + TILE - 1) / TILE * TILE;
     for (int k = 0; k < padded; ++k) {
       int s = pr.atomStart + k;

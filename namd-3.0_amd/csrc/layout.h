@@ -54,11 +54,24 @@ inline bool isExcluded(const I2& sm, const uint32_t* bits, int iatom, int jatom)
   return (bits[idx >> 5] >> (idx & 31)) & 1u;
 }
 
+// One or more independent periodic systems laid out side by side: system s
+// owns atoms [sysAtomStart[s], sysAtomStart[s+1]), its own patch grid, slots
+// [sysSlotStart[s], sysSlotStart[s+1]) and candidate tile lists
+// [sysListStart[s], sysListStart[s+1]).  No compute pairs patches of two
+// different systems, so one pair-kernel launch serves the whole batch.
 struct PatchLayout {
-  // grid
+  // grid (of system 0; per-system edges in sysBox)
   int dims[3] = {0, 0, 0};
   double box[3] = {0, 0, 0};        // quantized edge lengths
   double patchSize[3] = {0, 0, 0};
+
+  // systems
+  int numSystems = 0;
+  std::vector<int> sysAtomStart;    // numSystems + 1
+  std::vector<int> sysSlotStart;    // numSystems + 1
+  std::vector<int> sysListStart;    // numSystems + 1
+  std::vector<double> sysBox;       // 3 per system, quantized
+  std::vector<int> patchSystem;     // per patch
   std::vector<double> centre;       // 3 per patch, multiples of CENTRE_QUANTUM
 
   // records
@@ -82,6 +95,10 @@ struct PatchLayout {
   // computes (self + 13 half-shell neighbours) and the tile-list skeleton.
   bool build(int n, const double* pos, const double boxIn[3], double patchDim, std::string& err);
 
+  // The same for numSystems systems; boxes holds 3 edge lengths per system.
+  bool buildBatch(int numSystems, const int* atomStart, const double* pos, const double* boxes,
+                  double patchDim, std::string& err);
+
   // Fill slot arrays for the current assignment (the xyzq staging of
   // HomePatch::positionsReady, src/HomePatch.C:1965-1995).  chargeScaled is q*sqrt(COULOMB*scaling/eps).
   void pack(const double* pos, const double* chargeScaled, const int* vdwType,
@@ -90,6 +107,9 @@ struct PatchLayout {
   // Repack positions only (same patch assignment); atoms may drift outside
   // their patch until the next build().
   void packPositions(const double* pos);
+
+ private:
+  bool buildOne(int n, const double* pos, const double boxIn[3], double patchDim, std::string& err);
 };
 
 // ORB partition of one patch into TILE-sized spatially compact groups

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include <vector>
+
 #include "nbtypes.h"
 
 namespace mi355nb {
@@ -29,7 +31,9 @@ struct ForceArgsHost {
   float* fsoa;
   int stride;
   TileListVirialEnergy* virialEnergy;
-  int wgPerCU;            // persistent workgroups per CU (4 waves each)
+  int wgPerCU;            // persistent workgroups per CU (4 waves each); 0: one wave per list
+  const AlchParams* alch; // per-system factors; null: no alchemical partitions
+  const signed char* part;// per-slot partition (alchemical only)
 };
 
 hipError_t launchBoundingBoxes(int numTiles, const F4* xyzq, BoundingBox* bbox, hipStream_t s);
@@ -49,10 +53,13 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& a, bool doEne
 hipError_t launchTransposeForces(int n, int stride, float* fsoa, F4* f, F4* fs, bool doSlow, hipStream_t s);
 hipError_t launchFill(size_t n, float* p, float v, hipStream_t s);
 hipError_t launchReduceTileListStat(int numLists, const int4* listStat, TileListStat* st, hipStream_t s);
-int reducePartialBlocks(int nSlots, int maxLists);
-hipError_t launchReduceVirialEnergy(int nSlots, int maxLists, const F4* xyzq, const F4* f, const F4* fs,
-                                    const int* numLists, const TileListVirialEnergy* tlve, bool doEnergy,
-                                    bool doVirial, bool doSlow, double* partials, VirialEnergy* out,
-                                    hipStream_t s);
+// Per-system reduction blocks: {system, kind (0 slots, 1 list records), begin, end}.
+constexpr int RED_VALUES = 24;   // doubles per partial block
+void buildReduceBlocks(const int* sysSlotStart, const int* sysListStart, int numSystems,
+                       std::vector<int4>& blocks, std::vector<int>& sysBlockStart);
+hipError_t launchReduceVirialEnergy(int numBlocks, int numSystems, const int4* blocks, const int* sysBlockStart,
+                                    const F4* xyzq, const F4* f, const F4* fs, const TileListVirialEnergy* tlve,
+                                    bool doEnergy, bool doVirial, bool doSlow, double* partials,
+                                    VirialEnergy* out, hipStream_t s);
 
 }  // namespace mi355nb

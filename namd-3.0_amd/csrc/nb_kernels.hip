@@ -164,6 +164,80 @@ __device__ __forceinline__ void pairForce(const float r2, const float qq, const 
   }
 }
 
+// Alchemical (FEP) pair math: calcForceEnergyFEP
+// (src/CudaComputeNonbondedKernel.cu:163-354) with potential switching, the
+// CPU soft-core of fep_vdw_forceandenergies (src/ComputeNonbondedFEP.C:156-165).
+// Pair classes from the partitions pi, pj (0 = environment, 1 = appearing,
+// 2 = disappearing):
+//    ref  (0,0)                  full interaction
+//    dec  (1,1)/(2,2), decoupled full interaction
+//    up   any 1 with 0, or (1,1) LJ lambda_vdw,up with soft-core shift, elec x lambda_elec,up
+//    down any 2 with 0, or (2,2) the same with the "down" factors
+//    (1,2)/(2,1)                 no interaction
+// One branch-free form covers all of them: the LJ energy is
+//    lv * S(r^2) * U(r^2 + sh),  U(x) = A/x^6 - B/x^3 (x = r^2),
+// with (lv, sh) = (1, 0) for ref/dec and (0, 0) for (1,2), and every
+// electrostatic term (fast, slow, energies) is scaled by le.  Energies at
+// lambda2 (the FEP "_s" sums) use the lambda2 factors and shifts.
+template <bool doEnergy, bool doSlow, typename Acc>
+__device__ __forceinline__ void pairForceAlch(const float r2, const float qq, const float A, const float B,
+                                              const NbConstants& c, const float beta3, const int pi,
+                                              const int pj, const AlchParams& ap, float& f, float& fSlow,
+                                              Acc& eVdw, Acc& eElec, Acc& eSlow, Acc& eVdw2, Acc& eElec2,
+                                              Acc& eSlow2) {
+  const bool ref = (pi | pj) == 0;
+  const bool cross = pi != 0 && pj != 0 && pi != pj;
+  const bool alch = !ref && !cross;
+  const bool dec = alch && pi == pj && ap.decouple != 0;
+  const bool up = alch && !dec && (pi == 1 || pj == 1);
+  const bool down = alch && !dec && (pi == 2 || pj == 2);
+  const float full = (ref || dec) ? 1.0f : 0.0f;
+  const float lv = up ? ap.vdwLambdaUp : (down ? ap.vdwLambdaDown : full);
+  const float le = up ? ap.elecLambdaUp : (down ? ap.elecLambdaDown : full);
+  const float sh = up ? ap.vdwShiftUp : (down ? ap.vdwShiftDown : 0.0f);
+
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float rinv3 = rinv * rinv * rinv;
+  const bool sw = r2 > c.lj_5;
+  const float delta_r = c.lj_4 - r2;
+  const float s = sw ? delta_r * delta_r * (c.lj_0 + c.lj_1 * r2) : 1.0f;
+  const float ds_r = sw ? delta_r * (c.lj_3 + c.lj_2 * r2) : 0.0f;
+  const float inv = __builtin_amdgcn_rcpf(r2 + sh);
+  const float i6 = inv * inv * inv;
+  const float ai6 = A * i6;
+  const float U = i6 * (ai6 - B);
+  const float dU = (ai6 + (ai6 - B)) * i6 * -6.0f * inv;
+  f = le * qq * (c.e_0 - rinv3) + lv * (U * ds_r + dU * s);
+  float slowE = 0.0f;
+  if (doSlow) {
+    const float u = r2 * c.ewald_2;
+    const float P = HP0 + u * (HP1 + u * (HP2 + u * (HP3 + u * (HP4 + u * (HP5 + u * HP6)))));
+    const float Q = 1.0f + u * (HQ1 + u * (HQ2 + u * (HQ3 + u * (HQ4 + u * (HQ5 + u * HQ6)))));
+    fSlow = le * qq * (beta3 * (P * __builtin_amdgcn_rcpf(Q)) - c.e_0);
+  }
+  if (doEnergy) {
+    const float lv2 = up ? ap.vdwLambda2Up : (down ? ap.vdwLambda2Down : full);
+    const float le2 = up ? ap.elecLambda2Up : (down ? ap.elecLambda2Down : full);
+    const float sh2 = up ? ap.vdwShift2Up : (down ? ap.vdwShift2Down : 0.0f);
+    const float r = r2 * rinv;
+    const float x = r * c.e_2;
+    const float omx = 1.0f - x;
+    const float fastE = qq * (omx * omx * (1.0f + 0.5f * x) * rinv);
+    if (doSlow) slowE = qq * erfcf(r * c.ewald_0) * rinv - fastE;
+    const float inv2 = __builtin_amdgcn_rcpf(r2 + sh2);
+    const float j6 = inv2 * inv2 * inv2;
+    const float U2 = j6 * (A * j6 - B);
+    eVdw.v += lv * (U * s);
+    eVdw2.v += lv2 * (U2 * s);
+    eElec.v += le * fastE;
+    eElec2.v += le2 * fastE;
+    if (doSlow) {
+      eSlow.v += le * slowE;
+      eSlow2.v += le2 * slowE;
+    }
+  }
+}
+
 // r^2 <= c2 exactly; dx, dy, dz are exact (multiples of 2^-17 A below 128 A),
 // so their FP64 squares and sum are exact.
 __device__ __forceinline__ bool within(float dx, float dy, float dz, float r2, const CutTest& t) {
@@ -224,15 +298,18 @@ struct ForceArgs {
   // outputs
   float* fsoa;                       // 6 * stride floats: fx fy fz fsx fsy fsz
   int stride;
-  TileListVirialEnergy* virialEnergy;
+  TileListVirialEnergy* virialEnergy;  // indexed by TileList::listId
+  // alchemical
+  const AlchParams* alch;            // per system
+  const signed char* part;           // per slot: partition 0, 1, 2
 };
 
-template <bool doEnergy, bool doVirial, bool doSlow, bool doPairlist>
+template <bool doEnergy, bool doVirial, bool doSlow, bool doPairlist, bool doAlch>
 __global__ void __launch_bounds__(64 * NB_WAVES)
 nonbondedForceKernel(const ForceArgs args) {
   extern __shared__ float2 s_lj[];   // vdwWidth^2 (A, B)
   __shared__ F4 s_xyzq[NB_WAVES][TILE];
-  __shared__ int s_type[NB_WAVES][TILE];
+  __shared__ int s_type[NB_WAVES][TILE];     // vdw type | partition << 16 (alchemical)
   __shared__ int s_index[NB_WAVES][TILE];
   __shared__ F4 s_jf[NB_WAVES][2][TILE];
   __shared__ F4 s_jfs[NB_WAVES][2][TILE];
@@ -261,10 +338,17 @@ nonbondedForceKernel(const ForceArgs args) {
     xi.y += tl.offy;
     xi.z += tl.offz;
     const int vdwRow = args.vdwTypes[iatom] * args.vdwWidth;
+    int iPart = 0;
+    AlchParams ap;
+    if (doAlch) {
+      iPart = args.part[iatom];
+      ap = args.alch[tl.sys];
+    }
 
     float ifx = 0.f, ify = 0.f, ifz = 0.f;
     float ifsx = 0.f, ifsy = 0.f, ifsz = 0.f;
     EnergyAcc<doEnergy> eVdw, eElec, eSlow;
+    EnergyAcc<doEnergy> eVdw2, eElec2, eSlow2;   // FEP sums at lambda2 (alchemical only)
     // FP64 copies of the list's i-force for the virial's shift term
     EnergyAcc<doVirial> vfx, vfy, vfz, vfsx, vfsy, vfsz;
 
@@ -295,6 +379,7 @@ nonbondedForceKernel(const ForceArgs args) {
       const F4 p = args.xyzq[ja0 + i];
       nx = p.x; ny = p.y; nz = p.z; nq = p.w;
       nType = args.vdwTypes[ja0 + i];
+      if (doAlch) nType |= (int)args.part[ja0 + i] << 16;
       if (doPairlist) nIdx = args.atomIndex[ja0 + i];
       if (!doPairlist) nMask = args.tileExcls[tl.jtileStart].excl[i];
     }
@@ -317,6 +402,7 @@ nonbondedForceKernel(const ForceArgs args) {
         const F4 p = args.xyzq[jn + i];
         nx = p.x; ny = p.y; nz = p.z; nq = p.w;
         nType = args.vdwTypes[jn + i];
+        if (doAlch) nType |= (int)args.part[jn + i] << 16;
         if (doPairlist) nIdx = args.atomIndex[jn + i];
         if (!doPairlist) nMask = args.tileExcls[jt + 1].excl[i];
       }
@@ -351,9 +437,16 @@ nonbondedForceKernel(const ForceArgs args) {
           active = ((mask >> j) & 1u) && within(dx, dy, dz, r2, args.cut);
         }
         if (active) {
-          const float2 ab = s_lj[vdwRow + st[j]];
           float f, fs = 0.f;
-          pairForce<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, eVdw, eElec, eSlow);
+          if (doAlch) {
+            const int tw = st[j];
+            const float2 ab = s_lj[vdwRow + (tw & 0xffff)];
+            pairForceAlch<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, iPart, tw >> 16, ap, f,
+                                            fs, eVdw, eElec, eSlow, eVdw2, eElec2, eSlow2);
+          } else {
+            const float2 ab = s_lj[vdwRow + st[j]];
+            pairForce<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, eVdw, eElec, eSlow);
+          }
           const float fx = dx * f, fy = dy * f, fz = dz * f;
           ifx += fx; ify += fy; ifz += fz;
           if (doVirial) { vfx.v += fx; vfy.v += fy; vfz.v += fz; }
@@ -431,7 +524,7 @@ nonbondedForceKernel(const ForceArgs args) {
       double ssx = 0.0, ssy = 0.0, ssz = 0.0;
       if (doSlow) { ssx = waveSumD(vfsx.v); ssy = waveSumD(vfsy.v); ssz = waveSumD(vfsz.v); }
       if (lane == 0) {
-        TileListVirialEnergy& ve = args.virialEnergy[list];
+        TileListVirialEnergy& ve = args.virialEnergy[tl.listId];
         ve.shx = tl.offx; ve.shy = tl.offy; ve.shz = tl.offz;
         ve.forcex = sx2; ve.forcey = sy2; ve.forcez = sz2;
         ve.forceSlowx = ssx; ve.forceSlowy = ssy; ve.forceSlowz = ssz;
@@ -439,9 +532,16 @@ nonbondedForceKernel(const ForceArgs args) {
     }
     if (doEnergy) {
       const double a = waveSumD(eVdw.v), b = waveSumD(eElec.v), e = doSlow ? waveSumD(eSlow.v) : 0.0;
+      double a2 = 0.0, b2 = 0.0, e2 = 0.0;
+      if (doAlch) {
+        a2 = waveSumD(eVdw2.v);
+        b2 = waveSumD(eElec2.v);
+        e2 = doSlow ? waveSumD(eSlow2.v) : 0.0;
+      }
       if (lane == 0) {
-        TileListVirialEnergy& ve = args.virialEnergy[list];
+        TileListVirialEnergy& ve = args.virialEnergy[tl.listId];
         ve.energyVdw = a; ve.energyElec = b; ve.energySlow = e;
+        ve.energyVdw_s = a2; ve.energyElec_s = b2; ve.energySlow_s = e2;
       }
     }
   }
@@ -560,7 +660,8 @@ reduceTileListStatKernel(int numLists, const int4* __restrict__ listStat, TileLi
 // Partial record per block: 9 fast + 9 slow + 3 energies = 21 doubles.
 // ---------------------------------------------------------------------------
 constexpr int RED_THREADS = 256;
-constexpr int RED_N = 21;
+constexpr int RED_N = 24;
+static_assert(RED_N == RED_VALUES, "partial block width");
 
 __device__ __forceinline__ double blockSumD(double v, double* sh) {
 #pragma unroll
@@ -575,75 +676,86 @@ __device__ __forceinline__ double blockSumD(double v, double* sh) {
   return s;
 }
 
+// Per-system reduction.  Each block of the partial pass covers one range of
+// one system: kind 0 = slots [begin, end) (virial x_pc (x) F), kind 1 = list
+// records [begin, end) (virial shift (x) sum F_i, energies).  The final pass
+// runs one block per system over that system's partial blocks, in a fixed
+// order, so the FP64 sums are deterministic.
 __global__ void __launch_bounds__(RED_THREADS)
-reducePartialsKernel(int nAtomBlocks, int nSlots, const F4* __restrict__ xyzq, const F4* __restrict__ f,
-                     const F4* __restrict__ fs, const int* __restrict__ numLists,
-                     const TileListVirialEnergy* __restrict__ tlve, int doEnergy, int doVirial, int doSlow,
-                     double* __restrict__ partials) {
+reducePartialsKernel(const int4* __restrict__ blocks, const F4* __restrict__ xyzq, const F4* __restrict__ f,
+                     const F4* __restrict__ fs, const TileListVirialEnergy* __restrict__ tlve, int doEnergy,
+                     int doVirial, int doSlow, double* __restrict__ partials) {
   __shared__ double sh[RED_THREADS / 64];
+  const int4 bk = blocks[blockIdx.x];
+  const int k = bk.z + threadIdx.x;
   double v[RED_N];
 #pragma unroll
-  for (int k = 0; k < RED_N; ++k) v[k] = 0.0;
-  if ((int)blockIdx.x < nAtomBlocks) {
-    const int s = blockIdx.x * RED_THREADS + threadIdx.x;
-    if (doVirial && s < nSlots) {
-      const F4 p = xyzq[s];
-      const F4 a = f[s];
+  for (int u = 0; u < RED_N; ++u) v[u] = 0.0;
+  if (bk.y == 0) {
+    if (doVirial && k < bk.w) {
+      const F4 p = xyzq[k];
+      const F4 a = f[k];
       const double x[3] = {p.x, p.y, p.z};
       const double fa[3] = {a.x, a.y, a.z};
       for (int u = 0; u < 3; ++u)
         for (int w = 0; w < 3; ++w) v[u * 3 + w] = fa[u] * x[w];
       if (doSlow) {
-        const F4 b = fs[s];
+        const F4 b = fs[k];
         const double fb[3] = {b.x, b.y, b.z};
         for (int u = 0; u < 3; ++u)
           for (int w = 0; w < 3; ++w) v[9 + u * 3 + w] = fb[u] * x[w];
       }
     }
-  } else {
-    const int l = (blockIdx.x - nAtomBlocks) * RED_THREADS + threadIdx.x;
-    if (l < *numLists) {
-      const TileListVirialEnergy e = tlve[l];
-      if (doVirial) {
-        const double sh3[3] = {e.shx, e.shy, e.shz};
-        const double fa[3] = {e.forcex, e.forcey, e.forcez};
-        const double fb[3] = {e.forceSlowx, e.forceSlowy, e.forceSlowz};
-        for (int u = 0; u < 3; ++u)
-          for (int w = 0; w < 3; ++w) {
-            v[u * 3 + w] = fa[u] * sh3[w];
-            if (doSlow) v[9 + u * 3 + w] = fb[u] * sh3[w];
-          }
-      }
-      if (doEnergy) {
-        v[18] = e.energyVdw;
-        v[19] = e.energyElec;
-        v[20] = doSlow ? e.energySlow : 0.0;
-      }
+  } else if (k < bk.w) {
+    const TileListVirialEnergy e = tlve[k];
+    if (doVirial) {
+      const double sh3[3] = {e.shx, e.shy, e.shz};
+      const double fa[3] = {e.forcex, e.forcey, e.forcez};
+      const double fb[3] = {e.forceSlowx, e.forceSlowy, e.forceSlowz};
+      for (int u = 0; u < 3; ++u)
+        for (int w = 0; w < 3; ++w) {
+          v[u * 3 + w] = fa[u] * sh3[w];
+          if (doSlow) v[9 + u * 3 + w] = fb[u] * sh3[w];
+        }
+    }
+    if (doEnergy) {
+      v[18] = e.energyVdw;
+      v[19] = e.energyElec;
+      v[20] = doSlow ? e.energySlow : 0.0;
+      v[21] = e.energyVdw_s;
+      v[22] = e.energyElec_s;
+      v[23] = doSlow ? e.energySlow_s : 0.0;
     }
   }
-  for (int k = 0; k < RED_N; ++k) {
-    double s = blockSumD(v[k], sh);
-    if (threadIdx.x == 0) partials[(size_t)blockIdx.x * RED_N + k] = s;
+  for (int u = 0; u < RED_N; ++u) {
+    double t = blockSumD(v[u], sh);
+    if (threadIdx.x == 0) partials[(size_t)blockIdx.x * RED_N + u] = t;
   }
 }
 
 __global__ void __launch_bounds__(RED_THREADS)
-reduceFinalKernel(int nBlocks, const double* __restrict__ partials, VirialEnergy* __restrict__ out) {
+reduceFinalKernel(const int* __restrict__ sysBlockStart, const double* __restrict__ partials,
+                  VirialEnergy* __restrict__ outAll) {
   __shared__ double sh[RED_THREADS / 64];
+  const int sys = blockIdx.x;
+  const int b0 = sysBlockStart[sys], b1 = sysBlockStart[sys + 1];
+  VirialEnergy* out = outAll + sys;
   for (int k = 0; k < RED_N; ++k) {
     double v = 0.0;
-    for (int b = threadIdx.x; b < nBlocks; b += RED_THREADS) v += partials[(size_t)b * RED_N + k];
-    double s = blockSumD(v, sh);
+    for (int b = b0 + threadIdx.x; b < b1; b += RED_THREADS) v += partials[(size_t)b * RED_N + k];
+    double t = blockSumD(v, sh);
     if (threadIdx.x == 0) {
-      if (k < 9) out->virial[k] = s;
-      else if (k < 18) out->virialSlow[k - 9] = s;
-      else if (k == 18) out->energyVdw = s;
-      else if (k == 19) out->energyElec = s;
-      else out->energySlow = s;
+      if (k < 9) out->virial[k] = t;
+      else if (k < 18) out->virialSlow[k - 9] = t;
+      else if (k == 18) out->energyVdw = t;
+      else if (k == 19) out->energyElec = t;
+      else if (k == 20) out->energySlow = t;
+      else if (k == 21) out->energyVdw_s = t;
+      else if (k == 22) out->energyElec_s = t;
+      else out->energySlow_s = t;
     }
   }
   if (threadIdx.x == 0) {
-    out->energyVdw_s = out->energyElec_s = out->energySlow_s = 0.0;
     out->energyVdw_ti_1 = out->energyVdw_ti_2 = 0.0;
     out->energyElec_ti_1 = out->energyElec_ti_2 = 0.0;
     out->energySlow_ti_1 = out->energySlow_ti_2 = 0.0;
@@ -727,7 +839,11 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
   a.fsoa = h.fsoa;
   a.stride = h.stride;
   a.virialEnergy = h.virialEnergy;
-  // persistent grid: enough workgroups to fill every CU several times over
+  a.alch = h.alch;
+  a.part = h.part;
+  // persistent grid of wgPerCU workgroups per CU, or (wgPerCU == 0) one wave
+  // per list.  Pairlist steps walk the candidate lists in layout order, whose
+  // depths vary widely, so they always take one wave per list.
   static int numCU = 0;
   if (numCU == 0) {
     int dev = 0;
@@ -736,18 +852,24 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
     numCU = prop.multiProcessorCount;
   }
   const int wgNeeded = (maxLists + NB_WAVES - 1) / NB_WAVES;
-  dim3 g(std::min(wgNeeded, numCU * h.wgPerCU)), b(64 * NB_WAVES);
+  const int wgCap = (doPairlist || h.wgPerCU <= 0) ? wgNeeded : numCU * h.wgPerCU;
+  dim3 g(std::min(wgNeeded, wgCap)), b(64 * NB_WAVES);
   const size_t shmem = sizeof(float2) * (size_t)h.vdwWidth * h.vdwWidth;
-  const int key = (doEnergy ? 8 : 0) | (doVirial ? 4 : 0) | (doSlow ? 2 : 0) | (doPairlist ? 1 : 0);
+  const bool doAlch = h.alch != nullptr;
+  const int key = (doAlch ? 16 : 0) | (doEnergy ? 8 : 0) | (doVirial ? 4 : 0) | (doSlow ? 2 : 0) | (doPairlist ? 1 : 0);
   switch (key) {
-#define NB_CASE(E, V, S, P)                                                                  \
-  case (E * 8 + V * 4 + S * 2 + P):                                                          \
-    hipLaunchKernelGGL((nonbondedForceKernel<E, V, S, P>), g, b, shmem, s, a);               \
+#define NB_CASE(A, E, V, S, P)                                                               \
+  case (A * 16 + E * 8 + V * 4 + S * 2 + P):                                                 \
+    hipLaunchKernelGGL((nonbondedForceKernel<E, V, S, P, A>), g, b, shmem, s, a);            \
     break;
-    NB_CASE(0, 0, 0, 0) NB_CASE(0, 0, 0, 1) NB_CASE(0, 0, 1, 0) NB_CASE(0, 0, 1, 1)
-    NB_CASE(0, 1, 0, 0) NB_CASE(0, 1, 0, 1) NB_CASE(0, 1, 1, 0) NB_CASE(0, 1, 1, 1)
-    NB_CASE(1, 0, 0, 0) NB_CASE(1, 0, 0, 1) NB_CASE(1, 0, 1, 0) NB_CASE(1, 0, 1, 1)
-    NB_CASE(1, 1, 0, 0) NB_CASE(1, 1, 0, 1) NB_CASE(1, 1, 1, 0) NB_CASE(1, 1, 1, 1)
+    NB_CASE(0, 0, 0, 0, 0) NB_CASE(0, 0, 0, 0, 1) NB_CASE(0, 0, 0, 1, 0) NB_CASE(0, 0, 0, 1, 1)
+    NB_CASE(0, 0, 1, 0, 0) NB_CASE(0, 0, 1, 0, 1) NB_CASE(0, 0, 1, 1, 0) NB_CASE(0, 0, 1, 1, 1)
+    NB_CASE(0, 1, 0, 0, 0) NB_CASE(0, 1, 0, 0, 1) NB_CASE(0, 1, 0, 1, 0) NB_CASE(0, 1, 0, 1, 1)
+    NB_CASE(0, 1, 1, 0, 0) NB_CASE(0, 1, 1, 0, 1) NB_CASE(0, 1, 1, 1, 0) NB_CASE(0, 1, 1, 1, 1)
+    NB_CASE(1, 0, 0, 0, 0) NB_CASE(1, 0, 0, 0, 1) NB_CASE(1, 0, 0, 1, 0) NB_CASE(1, 0, 0, 1, 1)
+    NB_CASE(1, 0, 1, 0, 0) NB_CASE(1, 0, 1, 0, 1) NB_CASE(1, 0, 1, 1, 0) NB_CASE(1, 0, 1, 1, 1)
+    NB_CASE(1, 1, 0, 0, 0) NB_CASE(1, 1, 0, 0, 1) NB_CASE(1, 1, 0, 1, 0) NB_CASE(1, 1, 0, 1, 1)
+    NB_CASE(1, 1, 1, 0, 0) NB_CASE(1, 1, 1, 0, 1) NB_CASE(1, 1, 1, 1, 0) NB_CASE(1, 1, 1, 1, 1)
 #undef NB_CASE
   }
   return hipGetLastError();
@@ -771,21 +893,30 @@ hipError_t launchReduceTileListStat(int numLists, const int4* listStat, TileList
   return hipGetLastError();
 }
 
-int reducePartialBlocks(int nSlots, int maxLists) {
-  return (nSlots + RED_THREADS - 1) / RED_THREADS + (maxLists + RED_THREADS - 1) / RED_THREADS;
+void buildReduceBlocks(const int* sysSlotStart, const int* sysListStart, int numSystems,
+                       std::vector<int4>& blocks, std::vector<int>& sysBlockStart) {
+  blocks.clear();
+  sysBlockStart.assign(numSystems + 1, 0);
+  for (int sys = 0; sys < numSystems; ++sys) {
+    sysBlockStart[sys] = (int)blocks.size();
+    for (int k = sysSlotStart[sys]; k < sysSlotStart[sys + 1]; k += RED_THREADS)
+      blocks.push_back(make_int4(sys, 0, k, std::min(k + RED_THREADS, sysSlotStart[sys + 1])));
+    for (int k = sysListStart[sys]; k < sysListStart[sys + 1]; k += RED_THREADS)
+      blocks.push_back(make_int4(sys, 1, k, std::min(k + RED_THREADS, sysListStart[sys + 1])));
+  }
+  sysBlockStart[numSystems] = (int)blocks.size();
 }
 
-hipError_t launchReduceVirialEnergy(int nSlots, int maxLists, const F4* xyzq, const F4* f, const F4* fs,
-                                    const int* numLists, const TileListVirialEnergy* tlve, bool doEnergy,
-                                    bool doVirial, bool doSlow, double* partials, VirialEnergy* out,
-                                    hipStream_t s) {
-  const int nA = (nSlots + RED_THREADS - 1) / RED_THREADS;
-  const int nL = (maxLists + RED_THREADS - 1) / RED_THREADS;
-  hipLaunchKernelGGL(reducePartialsKernel, dim3(nA + nL), dim3(RED_THREADS), 0, s, nA, nSlots, xyzq, f, fs,
-                     numLists, tlve, doEnergy ? 1 : 0, doVirial ? 1 : 0, doSlow ? 1 : 0, partials);
+hipError_t launchReduceVirialEnergy(int numBlocks, int numSystems, const int4* blocks, const int* sysBlockStart,
+                                    const F4* xyzq, const F4* f, const F4* fs, const TileListVirialEnergy* tlve,
+                                    bool doEnergy, bool doVirial, bool doSlow, double* partials,
+                                    VirialEnergy* out, hipStream_t s) {
+  if (numBlocks <= 0 || numSystems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reducePartialsKernel, dim3(numBlocks), dim3(RED_THREADS), 0, s, blocks, xyzq, f, fs, tlve,
+                     doEnergy ? 1 : 0, doVirial ? 1 : 0, doSlow ? 1 : 0, partials);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(reduceFinalKernel, dim3(1), dim3(RED_THREADS), 0, s, nA + nL, partials, out);
+  hipLaunchKernelGGL(reduceFinalKernel, dim3(numSystems), dim3(RED_THREADS), 0, s, sysBlockStart, partials, out);
   return hipGetLastError();
 }
 

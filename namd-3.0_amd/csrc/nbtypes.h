@@ -72,6 +72,10 @@ struct TileList {
   int patchInd[2];
   int icompute;
   int self;          // first j-tile is the self tile (upper triangle only)
+  int sys;           // system of a batch (0 for a single system)
+  int listId;        // index in the candidate (compute, i-tile) order; fixed
+                     // by the layout and kept through repacks, so per-list
+                     // records are grouped by system
 };
 
 // TileExcl (src/CudaTileListKernel.h:6-8).  Bit = 1 means the pair is in the
@@ -109,6 +113,20 @@ struct TileListVirialEnergy {
   double forcex, forcey, forcez;
   double forceSlowx, forceSlowy, forceSlowz;
   double energyVdw, energyElec, energySlow;
+  double energyVdw_s, energyElec_s, energySlow_s;   // FEP: at lambda2
+};
+
+// Per-system alchemical factors, the lambda-dependent part of AlchData
+// (src/CudaComputeNonbondedKernel.h:24-50) as filled in
+// CudaComputeNonbonded::launchWork (src/CudaComputeNonbonded.C:1431-1455).
+// Potential switching, no WCA: the soft-core shift of an "up" pair is
+// vdwShiftUp = alchVdwShiftCoeff * (1 - vdwLambdaUp), and so on.
+struct AlchParams {
+  float elecLambdaUp, elecLambdaDown, vdwLambdaUp, vdwLambdaDown;
+  float elecLambda2Up, elecLambda2Down, vdwLambda2Up, vdwLambda2Down;
+  float vdwShiftUp, vdwShiftDown, vdwShift2Up, vdwShift2Down;
+  int decouple;
+  int pad[3];
 };
 
 // VirialEnergy (src/CudaTileListKernel.h:96-117), same field order.

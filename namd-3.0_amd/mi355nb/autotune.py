@@ -1,12 +1,13 @@
 """Launch-shape autotuner for the pair kernel.
 
 Times the force-only pair kernel (HIP events on the compute's stream) for
-each candidate number of persistent workgroups per CU and keeps the fastest.
+each candidate number of persistent workgroups per CU (0 = one wave per tile
+list, no persistence) and keeps the fastest.
 The compute object must have built its pairlists already.
 """
 from __future__ import annotations
 
-DEFAULT_CANDIDATES = (2, 4, 6, 8, 12, 16, 24, 32)
+DEFAULT_CANDIDATES = (0, 4, 8, 16, 32)
 
 
 def tune_pair_kernel(compute, candidates=DEFAULT_CANDIDATES, reps=8, do_slow=True):

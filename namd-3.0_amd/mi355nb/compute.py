@@ -42,6 +42,40 @@ class ComputeNonbonded:
         c.assign_patches(s.box, s.pos)
         return c
 
+    @classmethod
+    def from_systems(cls, systems, device=0, stream=None):
+        """One compute object for a batch of independent systems that share
+        cutoffs and the LJ parameter table (each keeps its own box)."""
+        s0 = systems[0]
+        p = s0.params
+        for s in systems[1:]:
+            q = s.params
+            if (q.cutoff, q.switchdist, q.pairlistdist, q.ewald_tol, q.margin) != \
+                    (p.cutoff, p.switchdist, p.pairlistdist, p.ewald_tol, p.margin):
+                raise ValueError("batched systems must share cutoff, switchdist, pairlistdist, ewald_tol and margin")
+            if not (np.array_equal(s.lj_sigma, s0.lj_sigma) and np.array_equal(s.lj_eps, s0.lj_eps)):
+                raise ValueError("batched systems must share the LJ parameter table")
+        beta = ewald_coefficient(p.cutoff, p.ewald_tol)
+        c = cls(p.cutoff, p.switchdist, p.pairlistdist, beta, slow_scale=1.0, margin=p.margin, device=device,
+                stream=stream)
+        A, B = L.lj_mix(s0.lj_sigma, s0.lj_eps)
+        starts = np.zeros(len(systems) + 1, np.int32)
+        for k, s in enumerate(systems):
+            starts[k + 1] = starts[k] + s.natoms
+        ptr = [np.zeros(1, np.int64)]
+        idx = []
+        off_e = 0
+        for k, s in enumerate(systems):
+            ptr.append(np.asarray(s.excl_ptr[1:], np.int64) + off_e)
+            idx.append(np.asarray(s.excl_idx, np.int64) + starts[k])
+            off_e += int(s.excl_ptr[-1])
+        c.set_molecule(np.concatenate([s.vdw_type for s in systems]), np.concatenate([s.charge for s in systems]),
+                       A, B, np.concatenate(ptr), np.concatenate(idx) if idx else np.zeros(0, np.int32))
+        c.sys_atom_start = starts
+        c.assign_patches_batch(starts, np.array([s.box for s in systems], np.float64),
+                               np.concatenate([s.pos for s in systems]))
+        return c
+
     def close(self):
         if self._h:
             L.lib().mi355nb_compute_destroy(self._h)
@@ -73,6 +107,23 @@ class ComputeNonbonded:
                                                        np.ascontiguousarray(pos, np.float64).reshape(-1)),
                 "assign_patches")
 
+    def assign_patches_batch(self, sys_atom_start, boxes, pos):
+        starts = np.ascontiguousarray(sys_atom_start, np.int32)
+        self.sys_atom_start = starts
+        L.check(L.lib().mi355nb_compute_assign_patches_batch(
+            self._h, starts.size - 1, starts, np.ascontiguousarray(boxes, np.float64).reshape(-1),
+            np.ascontiguousarray(pos, np.float64).reshape(-1)), "assign_patches_batch")
+
+    def set_alch(self, partition, alch):
+        """partition: int8 per atom (0, 1, 2) or None to switch off; alch: one
+        lib.AlchData per system."""
+        if partition is None:
+            L.check(L.lib().mi355nb_compute_set_alch(self._h, None, 0, None), "set_alch")
+            return
+        part = np.ascontiguousarray(partition, np.int8)
+        arr = (L.AlchData * len(alch))(*alch)
+        L.check(L.lib().mi355nb_compute_set_alch(self._h, part.ctypes.data, len(alch), arr), "set_alch")
+
     def update_positions(self, pos):
         L.check(L.lib().mi355nb_compute_update_positions(self._h, np.ascontiguousarray(pos, np.float64).reshape(-1)),
                 "update_positions")
@@ -81,6 +132,37 @@ class ComputeNonbonded:
     def launch_work(self, do_energy=True, do_virial=True, do_slow=True, do_pairlist=False):
         L.check(L.lib().mi355nb_compute_launch(self._h, int(do_energy), int(do_virial), int(do_slow),
                                                int(do_pairlist)), "launch")
+
+    def launch_graph(self, do_energy=False, do_virial=False, do_slow=True):
+        """Non-pairlist launch replayed from a HIP graph (captured on first use)."""
+        L.check(L.lib().mi355nb_compute_launch_graph(self._h, int(do_energy), int(do_virial), int(do_slow)),
+                "launch_graph")
+
+    @staticmethod
+    def _ve_dict(ve):
+        return dict(
+            e_vdw=ve.energyVdw, e_elec=ve.energyElec, e_slow=ve.energySlow,
+            e_vdw_s=ve.energyVdw_s, e_elec_s=ve.energyElec_s, e_slow_s=ve.energySlow_s,
+            virial=np.array(ve.virial[:]).reshape(3, 3), virial_slow=np.array(ve.virialSlow[:]).reshape(3, 3))
+
+    @staticmethod
+    def _stat_dict(st):
+        return dict(numTileLists=st.numTileLists, numJtiles=st.numJtiles, numExcluded=st.numExcluded,
+                    maxDepth=st.maxDepth, numPairsCutoff=st.numPairsCutoff, numPairsList=st.numPairsList)
+
+    def finish_batch(self, forces=True):
+        """(per-system list of energy/virial dicts, stat, f, fs)."""
+        n = self.natoms
+        nsys = len(self.sys_atom_start) - 1
+        f = np.zeros(3 * n) if forces else None
+        fs = np.zeros(3 * n) if forces else None
+        ves = (L.VirialEnergy * nsys)()
+        st = L.TileListStat()
+        L.check(L.lib().mi355nb_compute_finish_batch(self._h, f.ctypes.data if forces else None,
+                                                     fs.ctypes.data if forces else None, ves, C.byref(st)),
+                "finish_batch")
+        return ([self._ve_dict(v) for v in ves], self._stat_dict(st), f.reshape(n, 3) if forces else None,
+                fs.reshape(n, 3) if forces else None)
 
     def finish(self, forces=True):
         n = self.natoms
@@ -91,11 +173,8 @@ class ComputeNonbonded:
         L.check(L.lib().mi355nb_compute_finish(self._h, f.ctypes.data if forces else None,
                                                fs.ctypes.data if forces else None, C.byref(ve), C.byref(st)),
                 "finish")
-        out = dict(
-            e_vdw=ve.energyVdw, e_elec=ve.energyElec, e_slow=ve.energySlow,
-            virial=np.array(ve.virial[:]).reshape(3, 3), virial_slow=np.array(ve.virialSlow[:]).reshape(3, 3),
-            stat=dict(numTileLists=st.numTileLists, numJtiles=st.numJtiles, numExcluded=st.numExcluded,
-                      maxDepth=st.maxDepth, numPairsCutoff=st.numPairsCutoff, numPairsList=st.numPairsList))
+        out = self._ve_dict(ve)
+        out["stat"] = self._stat_dict(st)
         if forces:
             out["f"] = f.reshape(n, 3)
             out["fs"] = fs.reshape(n, 3)
@@ -126,7 +205,8 @@ class ComputeNonbonded:
         L.check(L.lib().mi355nb_compute_get_info(self._h, C.byref(i)), "get_info")
         return dict(natoms=i.natoms, atomStorageSize=i.atomStorageSize, numPatches=i.numPatches,
                     numComputes=i.numComputes, numCandidateLists=i.numCandidateLists,
-                    patchDims=tuple(i.patchDims), jtileCapacity=i.jtileCapacity, box=tuple(i.box))
+                    patchDims=tuple(i.patchDims), jtileCapacity=i.jtileCapacity, box=tuple(i.box),
+                    numSystems=i.numSystems)
 
     def device_buffers(self):
         d = L.DeviceBuffers()

@@ -45,7 +45,15 @@ class NbParams(C.Structure):
 class LayoutInfo(C.Structure):
     _fields_ = [("natoms", C.c_int32), ("atomStorageSize", C.c_int32), ("numPatches", C.c_int32),
                 ("numComputes", C.c_int32), ("numCandidateLists", C.c_int32), ("patchDims", C.c_int32 * 3),
-                ("jtileCapacity", C.c_int64), ("box", C.c_double * 3)]
+                ("jtileCapacity", C.c_int64), ("box", C.c_double * 3), ("numSystems", C.c_int32)]
+
+
+class AlchData(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "scaling", "switchdist2", "cutoff2", "switchfactor", "alchVdwShiftCoeff", "lambdaUp", "lambdaDown",
+        "elecLambdaUp", "elecLambdaDown", "vdwLambdaUp", "vdwLambdaDown", "lambda2Up", "lambda2Down",
+        "elecLambda2Up", "elecLambda2Down", "vdwLambda2Up", "vdwLambda2Down", "vdwShiftUp", "vdwShift2Up",
+        "vdwShiftDown", "vdwShift2Down")] + [("alchDecouple", C.c_int32)]
 
 
 class DeviceBuffers(C.Structure):
@@ -61,7 +69,8 @@ EXPORTED = [
     "mi355nb_compute_get_info", "mi355nb_compute_get_device_buffers",
     "mi355nb_compute_get_nonbonded_coef", "mi355nb_compute_get_effective_positions",
     "mi355nb_compute_get_tile_lists", "mi355nb_spread_charge", "mi355nb_gather_force", "mi355nb_self_energy",
-    "mi355nb_pme_frac_coords",
+    "mi355nb_pme_frac_coords", "mi355nb_compute_assign_patches_batch", "mi355nb_alch_data_fill",
+    "mi355nb_compute_set_alch", "mi355nb_compute_finish_batch", "mi355nb_compute_launch_graph",
 ]
 
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -107,6 +116,12 @@ def lib():
     L.mi355nb_gather_force.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp]
     L.mi355nb_self_energy.argtypes = [vp, C.c_int, C.c_double, vp, vp]
     L.mi355nb_pme_frac_coords.argtypes = [vp, vp, C.c_int, dp, dp, vp, vp]
+    L.mi355nb_compute_assign_patches_batch.argtypes = [vp, C.c_int, i32p, dp, dp]
+    L.mi355nb_alch_data_fill.argtypes = [C.POINTER(NbParams), C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.c_int, C.POINTER(AlchData)]
+    L.mi355nb_compute_set_alch.argtypes = [vp, vp, C.c_int, vp]
+    L.mi355nb_compute_finish_batch.argtypes = [vp, vp, vp, vp, C.POINTER(TileListStat)]
+    L.mi355nb_compute_launch_graph.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     _LIB = L
     return L
 
@@ -152,6 +167,17 @@ def build_exclusions(natoms, excl_ptr, excl_idx):
     check(lib().mi355nb_build_exclusions(natoms, excl_ptr, excl_idx, sm.ctypes.data, bits.ctypes.data,
                                          C.byref(nwords), C.byref(nlists)), "buildExclusions")
     return sm.reshape(natoms, 2), bits[: nwords.value], nlists.value
+
+
+def alch_data(params, lam, lam2, elec_lambda_start=0.5, vdw_lambda_end=1.0, vdw_shift_coeff=5.0,
+              decouple=False) -> AlchData:
+    """AlchData for one lambda window (NAMD defaults: alchElecLambdaStart 0.5,
+    alchVdwLambdaEnd 1.0, alchVdwShiftCoeff 5.0)."""
+    d = AlchData()
+    p = params if isinstance(params, NbParams) else NbParams(*params)
+    check(lib().mi355nb_alch_data_fill(C.byref(p), lam, lam2, elec_lambda_start, vdw_lambda_end, vdw_shift_coeff,
+                                       int(decouple), C.byref(d)), "alch_data_fill")
+    return d
 
 
 def tile_excl_to_namd(masks):

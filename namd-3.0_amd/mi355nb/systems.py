@@ -24,6 +24,7 @@ is deterministic for a seed.
 from __future__ import annotations
 
 import math
+from typing import NamedTuple
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -213,14 +214,18 @@ def _relax_contacts(rng, wat, base, bead_pos, boxv, jitter, dmin, iters=60):
 
 def make_system(name: str, n_waters: int, n_chains: int, chain_len: int, box: float,
                 n_bead_types: int, seed: int, params: NbParams | None = None,
-                lattice_jitter: float = 0.3, lattice_shape=None, min_contact: float = 1.2) -> System:
-    """Water + chain box.  Atom order: chains first (bead 0..L-1 per chain), then waters O,H1,H2."""
+                lattice_jitter: float = 0.3, lattice_shape=None, min_contact: float = 1.2,
+                type_seed: int | None = None) -> System:
+    """Water + chain box.  Atom order: chains first (bead 0..L-1 per chain), then waters O,H1,H2.
+    type_seed: draw the bead LJ parameters from their own seed (systems of a
+    batch share one parameter table)."""
     rng = np.random.default_rng(seed)
     boxv = np.array([box, box, box], dtype=np.float64)
 
     # ---- LJ types: 0..n_bead_types-1 beads, then water O, water H
-    sig = rng.uniform(2.5, 4.0, n_bead_types)
-    eps = rng.uniform(0.02, 0.2, n_bead_types)
+    trng = rng if type_seed is None else np.random.default_rng(type_seed)
+    sig = trng.uniform(2.5, 4.0, n_bead_types)
+    eps = trng.uniform(0.02, 0.2, n_bead_types)
     lj_sigma = np.concatenate([sig, [TIP3P_O_RMIN2 * RMIN_TO_SIGMA, TIP3P_H_RMIN2 * RMIN_TO_SIGMA]])
     lj_eps = np.concatenate([eps, [TIP3P_O_EPS, TIP3P_H_EPS]])
     tO, tH = n_bead_types, n_bead_types + 1
@@ -229,7 +234,17 @@ def make_system(name: str, n_waters: int, n_chains: int, chain_len: int, box: fl
     beads = []
     grid = _BeadGrid(boxv, 3.7)
     for c in range(n_chains):
+        # a chain starts at least 2.2 A from every earlier bead (redrawn only on a clash)
         start = rng.random(3) * boxv
+        for _ in range(200):
+            near = grid.near(start, skip=set())
+            if near.shape[0] == 0:
+                break
+            d = near - start
+            d -= boxv * np.round(d / boxv)
+            if np.sqrt((d * d).sum(-1)).min() >= 2.2:
+                break
+            start = rng.random(3) * boxv
         beads.append(_chain_walk(rng, start, chain_len, boxv, grid, c))
     bead_pos = np.concatenate(beads, 0) if beads else np.zeros((0, 3))
     n_beads = bead_pos.shape[0]
@@ -331,6 +346,38 @@ def fit_23k(seed: int = 5) -> System:
     """23,000 atoms: 7,000 TIP3P + 20 chains x 100 beads (16 LJ types), 61.3 A cube."""
     p = NbParams(cutoff=12.0, switchdist=10.0, pairlistdist=13.5, pme_grid=(64, 64, 64))
     return make_system("fit23k", 7000, 20, 100, 61.3, 16, seed, p)
+
+
+class BatchMember(NamedTuple):
+    system: System
+    partition: np.ndarray   # int8 per atom: 1 for the alchemical solute, 0 otherwise
+    lam: float              # alchemical lambda of this system
+
+
+def batch_systems(n_systems: int = 64, seed: int = 7, n_min: int = 3000, n_max: int = 40000,
+                  density: float = 0.102, solute: int = 50, chain_len: int = 50, bead_fraction: float = 0.19,
+                  n_bead_types: int = 32) -> list:
+    """Independent systems of uneven size for one batched launch: atom counts
+    round(exp(U(ln n_min, ln n_max))), each a water + chain box at the given
+    atom density (cutoff 12, switch 10, pairlist 13.5) sharing one LJ table;
+    the first `solute` beads (the first chain) form alchemical partition 1
+    with a lambda drawn from U[0, 1]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_systems):
+        n_target = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        n_chains = max(1, int(round(bead_fraction * n_target / chain_len)))
+        n_waters = max(1, (n_target - n_chains * chain_len) // 3)
+        natoms = n_chains * chain_len + 3 * n_waters
+        box = (natoms / density) ** (1.0 / 3.0)
+        p = NbParams(cutoff=12.0, switchdist=10.0, pairlistdist=13.5,
+                     pme_grid=tuple([int(2 * math.ceil(box / 2.0))] * 3))
+        s = make_system(f"batch{k}", n_waters, n_chains, chain_len, box, n_bead_types, seed * 1000 + k, p,
+                        type_seed=seed)
+        part = np.zeros(s.natoms, np.int8)
+        part[: min(solute, s.n_beads)] = 1
+        out.append(BatchMember(s, part, float(rng.uniform(0.0, 1.0))))
+    return out
 
 
 def small_box(n_waters: int = 216, n_chains: int = 2, chain_len: int = 20, box: float = 20.0,

@@ -11,6 +11,14 @@
  *                          CudaComputeNonbondedInteractions.h:37-61) and the
  *                          C1-split Ewald real space (CudaComputeNonbondedInteractions.h:107-159;
  *                          fast + slow = erfc(beta r)/r).
+ *  oracle_nb_fp64_alch     the same with alchemical partitions (FEP):
+ *                          soft-core LJ of fep_vdw_forceandenergies
+ *                          (ComputeNonbondedFEP.C:156-165, potential switching,
+ *                          vdw_switch / vdw_forceandenergy of
+ *                          ComputeNonbondedAlch.h:16-48) and lambda-scaled
+ *                          electrostatics (ComputeNonbondedBase2.h:367-372,
+ *                          :497, :645-686), with the pair classes of
+ *                          calcForceEnergyFEP (CudaComputeNonbondedKernel.cu:203-213).
  *  oracle_pairs_fp32       the canonical pair-membership rule the kernels use:
  *                          r2 = fma(dz,dz,fma(dy,dy,dx*dx)) in FP32 on exact
  *                          displacements, r2 < (float)rc^2; counts and
@@ -84,6 +92,75 @@ static void pair_fp64(const PairConst* c, double r2, double qq, double A, double
   *es += qq * ((ec - 1.0) * rinv + 1.5 * c->rcinv - 0.5 * r2 * c->rc3inv);
 }
 
+/* Alchemical factors of one lambda window (AlchData's lambda part). */
+typedef struct {
+  double elecUp, elecDown, vdwUp, vdwDown;
+  double elec2Up, elec2Down, vdw2Up, vdw2Down;
+  double shUp, shDown, sh2Up, sh2Down;
+  int decouple;
+} AlchF;
+
+/* vdw_switch (ComputeNonbondedAlch.h:35-48): S and switchmul2 = -2 dS/d(r^2) */
+static void vdw_switch64(const PairConst* c, double r2, double* S, double* S2) {
+  if (r2 <= c->rs2) {
+    *S = 1.0;
+    *S2 = 0.0;
+  } else {
+    const double cd = c->rc2 - r2, sd = r2 - c->rs2;
+    *S = c->c1 * cd * cd * (cd + 3.0 * sd);
+    *S2 = 12.0 * c->c1 * cd * sd;
+  }
+}
+
+/* vdw_forceandenergy (ComputeNonbondedAlch.h:16-22) at x = r^2 + shift:
+ * U = A/x^6 - B/x^3, F = -2 dU/dx */
+static void vdw_fe64(double A, double B, double x, double* U, double* F) {
+  const double i2 = 1.0 / x, i6 = i2 * i2 * i2;
+  *U = i6 * (A * i6 - B);
+  *F = 6.0 * i2 * (2.0 * (*U) + B * i6);
+}
+
+/* e[0..5] = vdw, elec, slow, vdw at lambda2, elec at lambda2, slow at lambda2 */
+static void pair_fp64_alch(const PairConst* c, double r2, double qq, double A, double B, int pi, int pj,
+                           const AlchF* al, double* f, double* fs, double* e) {
+  const int ref = (pi == 0 && pj == 0);
+  const int cross = (pi != 0 && pj != 0 && pi != pj);
+  const int alch = !ref && !cross;
+  const int dec = alch && pi == pj && al->decouple;
+  const int up = alch && !dec && (pi == 1 || pj == 1);
+  const int down = alch && !dec && (pi == 2 || pj == 2);
+  if (cross) {
+    *f = *fs = 0.0;
+    return;
+  }
+  const double lv = up ? al->vdwUp : down ? al->vdwDown : 1.0;
+  const double le = up ? al->elecUp : down ? al->elecDown : 1.0;
+  const double lv2 = up ? al->vdw2Up : down ? al->vdw2Down : 1.0;
+  const double le2 = up ? al->elec2Up : down ? al->elec2Down : 1.0;
+  const double sh = up ? al->shUp : down ? al->shDown : 0.0;
+  const double sh2 = up ? al->sh2Up : down ? al->sh2Down : 0.0;
+  double S, S2, U, F, U2, F2;
+  vdw_switch64(c, r2, &S, &S2);
+  vdw_fe64(A, B, r2 + sh, &U, &F);
+  vdw_fe64(A, B, r2 + sh2, &U2, &F2);
+  /* alch_vdw_force = lv (S F + S2 U) is -2 dE/d(r^2); here f = 2 dE/d(r^2) */
+  const double fv = -lv * (S * F + S2 * U);
+  e[0] += lv * S * U;
+  e[3] += lv2 * S * U2;
+  const double r = sqrt(r2), rinv = 1.0 / r, rinv2 = rinv * rinv;
+  const double fastE = qq * (rinv - 1.5 * c->rcinv + 0.5 * r2 * c->rc3inv);
+  *f = fv + le * qq * (-rinv * rinv2 + c->rc3inv);
+  e[1] += le * fastE;
+  e[4] += le2 * fastE;
+  const double a = c->beta * r;
+  const double ec = erfc(a);
+  const double g = ec + TWO_OVER_SQRT_PI * a * exp(-a * a);
+  *fs = le * qq * ((1.0 - g) * rinv * rinv2 - c->rc3inv);
+  const double slowE = qq * ((ec - 1.0) * rinv + 1.5 * c->rcinv - 0.5 * r2 * c->rc3inv);
+  e[2] += le * slowE;
+  e[5] += le2 * slowE;
+}
+
 /* cell list over an orthorhombic box; calls back for every i<j pair with r2 < cut2 */
 typedef struct {
   int n, nc[3];
@@ -151,10 +228,11 @@ static int cell_neighbours(const Cells* C, int gx, int gy, int gz, int* out) {
  * pairs of F_ij,a * (x_i - x_j)_b, i.e. NAMD's sum_atoms F_a x_b);
  * npairs = number of pairs with r < rc.  nthreads <= 0: all available.
  */
-int oracle_nb_fp64(int n, const double* pos, const double* q, const int32_t* type, int ntypes, const double* A,
-                   const double* B, const int64_t* eptr, const int32_t* eidx, const double L[3], double cutoff,
-                   double switchdist, double beta, double* f, double* fs, double* energies, double* virial,
-                   double* virialSlow, int64_t* npairs, int nthreads) {
+static int nb_fp64_core(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+                        const double* A, const double* B, const int64_t* eptr, const int32_t* eidx,
+                        const double L[3], double cutoff, double switchdist, double beta, const int8_t* part,
+                        const AlchF* al, double* f, double* fs, double* energies, int nenergies, double* virial,
+                        double* virialSlow, int64_t* npairs, int nthreads) {
   PairConst c;
   c.rc = cutoff;
   c.rc2 = cutoff * cutoff;
@@ -171,7 +249,7 @@ int oracle_nb_fp64(int n, const double* pos, const double* q, const int32_t* typ
   nt = nthreads > 0 ? nthreads : omp_get_max_threads();
 #endif
   double* fb = (double*)calloc((size_t)nt * 6 * n, sizeof(double));
-  double* acc = (double*)calloc((size_t)nt * 24, sizeof(double));
+  double* acc = (double*)calloc((size_t)nt * 24, sizeof(double));  /* 9 + 9 virial, up to 6 energies */
   int64_t* np = (int64_t*)calloc((size_t)nt, sizeof(int64_t));
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
@@ -197,7 +275,10 @@ int oracle_nb_fp64(int n, const double* pos, const double* q, const int32_t* typ
           if (is_excluded(eptr, eidx, i, j)) continue;
           double ff, ffs;
           const int ab = type[i] * ntypes + type[j];
-          pair_fp64(&c, r2, q[i] * q[j], A[ab], B[ab], &ff, &ffs, &S[18], &S[19], &S[20]);
+          if (part)
+            pair_fp64_alch(&c, r2, q[i] * q[j], A[ab], B[ab], part[i], part[j], al, &ff, &ffs, &S[18]);
+          else
+            pair_fp64(&c, r2, q[i] * q[j], A[ab], B[ab], &ff, &ffs, &S[18], &S[19], &S[20]);
           for (int u = 0; u < 3; ++u) {
             F[3 * i + u] += ff * d[u];
             F[3 * j + u] -= ff * d[u];
@@ -229,13 +310,38 @@ int oracle_nb_fp64(int n, const double* pos, const double* q, const int32_t* typ
   }
   if (virial) memcpy(virial, tot, 9 * sizeof(double));
   if (virialSlow) memcpy(virialSlow, tot + 9, 9 * sizeof(double));
-  if (energies) memcpy(energies, tot + 18, 3 * sizeof(double));
+  if (energies) memcpy(energies, tot + 18, nenergies * sizeof(double));
   if (npairs) *npairs = ntot;
   free(fb);
   free(acc);
   free(np);
   cells_free(&C);
   return 0;
+}
+
+int oracle_nb_fp64(int n, const double* pos, const double* q, const int32_t* type, int ntypes, const double* A,
+                   const double* B, const int64_t* eptr, const int32_t* eidx, const double L[3], double cutoff,
+                   double switchdist, double beta, double* f, double* fs, double* energies, double* virial,
+                   double* virialSlow, int64_t* npairs, int nthreads) {
+  return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, NULL, NULL, f, fs,
+                      energies, 3, virial, virialSlow, npairs, nthreads);
+}
+
+/* lam[12] = elecUp, elecDown, vdwUp, vdwDown, elec2Up, elec2Down, vdw2Up,
+ * vdw2Down, shiftUp, shiftDown, shift2Up, shift2Down; energies[6] = vdw, elec,
+ * slow and the same at lambda2. */
+int oracle_nb_fp64_alch(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+                        const double* A, const double* B, const int64_t* eptr, const int32_t* eidx,
+                        const double L[3], double cutoff, double switchdist, double beta, const int8_t* part,
+                        const double* lam, int decouple, double* f, double* fs, double* energies, double* virial,
+                        double* virialSlow, int64_t* npairs, int nthreads) {
+  AlchF al;
+  al.elecUp = lam[0]; al.elecDown = lam[1]; al.vdwUp = lam[2]; al.vdwDown = lam[3];
+  al.elec2Up = lam[4]; al.elec2Down = lam[5]; al.vdw2Up = lam[6]; al.vdw2Down = lam[7];
+  al.shUp = lam[8]; al.shDown = lam[9]; al.sh2Up = lam[10]; al.sh2Down = lam[11];
+  al.decouple = decouple;
+  return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, part, &al, f, fs,
+                      energies, 6, virial, virialSlow, npairs, nthreads);
 }
 
 /* ---- canonical FP32 pair sets ------------------------------------------- */
@@ -735,7 +841,7 @@ int oracle_namd_cpu(int n, const double* pos, const double* q, const int32_t* ty
   nt = nthreads > 0 ? nthreads : omp_get_max_threads();
 #endif
   double* fb = (double*)calloc((size_t)nt * 6 * n, sizeof(double));
-  double* acc = (double*)calloc((size_t)nt * 24, sizeof(double));
+  double* acc = (double*)calloc((size_t)nt * 24, sizeof(double));  /* 9 + 9 virial, up to 6 energies */
   int64_t* np = (int64_t*)calloc((size_t)nt, sizeof(int64_t));
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt)

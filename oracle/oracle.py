@@ -46,6 +46,10 @@ def lib():
         L.oracle_namd_tables.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int),
                                          C.c_void_p, C.c_void_p]
         L.oracle_namd_cpu.argtypes = L.oracle_nb_fp64.argtypes
+        i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+        L.oracle_nb_fp64_alch.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, i64p, i32p, dp, C.c_double,
+                                          C.c_double, C.c_double, i8p, dp, C.c_int, dp, dp, dp, dp, dp, i64p,
+                                          C.c_int]
     return _LIB
 
 
@@ -69,6 +73,35 @@ def nb_fp64(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff, swit
                          beta, f, fs, en, vir, virs, npairs, int(nthreads))
     return dict(f=f.reshape(n, 3), fs=fs.reshape(n, 3), e_vdw=en[0], e_elec=en[1], e_slow=en[2],
                 virial=vir.reshape(3, 3), virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]))
+
+
+ALCH_KEYS = ("elecLambdaUp", "elecLambdaDown", "vdwLambdaUp", "vdwLambdaDown", "elecLambda2Up", "elecLambda2Down",
+             "vdwLambda2Up", "vdwLambda2Down", "vdwShiftUp", "vdwShiftDown", "vdwShift2Up", "vdwShift2Down")
+
+
+def nb_fp64_alch(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff, switchdist, beta, partition,
+                 alch, nthreads=0):
+    """FP64 alchemical (FEP) oracle.  alch: mapping with ALCH_KEYS and
+    'alchDecouple' (an AlchData works).  Returns nb_fp64's dict plus the
+    lambda2 energies e_vdw_s, e_elec_s, e_slow_s."""
+    n = len(q_scaled)
+    ntypes = int(round(np.sqrt(np.asarray(A).size)))
+    get = (lambda k: alch[k]) if isinstance(alch, dict) else (lambda k: getattr(alch, k))
+    lam = np.array([float(get(k)) for k in ALCH_KEYS], np.float64)
+    f = np.zeros(3 * n)
+    fs = np.zeros(3 * n)
+    en = np.zeros(6)
+    vir = np.zeros(9)
+    virs = np.zeros(9)
+    npairs = np.zeros(1, np.int64)
+    lib().oracle_nb_fp64_alch(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32),
+                              ntypes, _c(A, np.float64).reshape(-1), _c(B, np.float64).reshape(-1),
+                              _c(excl_ptr, np.int64), _c(excl_idx, np.int32), _c(box, np.float64), cutoff,
+                              switchdist, beta, _c(partition, np.int8), lam, int(bool(get("alchDecouple"))), f, fs,
+                              en, vir, virs, npairs, int(nthreads))
+    return dict(f=f.reshape(n, 3), fs=fs.reshape(n, 3), e_vdw=en[0], e_elec=en[1], e_slow=en[2], e_vdw_s=en[3],
+                e_elec_s=en[4], e_slow_s=en[5], virial=vir.reshape(3, 3), virial_slow=virs.reshape(3, 3),
+                npairs=int(npairs[0]))
 
 
 def pairs_fp32(pos, excl_ptr, excl_idx, box, cutoff2f, plcutoff2f, want_pairs=False, maxpairs=None):

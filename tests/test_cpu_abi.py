@@ -27,7 +27,31 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in _declared() if s not in exported]
     assert not missing, missing
     assert set(L.EXPORTED) <= exported
-    assert L.lib().mi355nb_abi_version() == 1
+    hdr = open(os.path.join(ROOT, "include", "mi355nb.h")).read()
+    assert L.lib().mi355nb_abi_version() == int(re.search(r"#define MI355NB_ABI_VERSION (\d+)", hdr).group(1))
+
+
+def test_alch_data_fill_follows_lambda_schedule():
+    """AlchData of one window: elec lambda starts at alchElecLambdaStart, vdw
+    lambda saturates at alchVdwLambdaEnd, soft-core shift = coeff (1 - vdw
+    lambda), 'down' factors at 1 - lambda (src/SimParameters.C:8182-8210,
+    src/CudaComputeNonbonded.C:1424-1455)."""
+    p = L.NbParams(12.0, 10.0, 13.5, 0.26, 1.0, 0.0)
+    for lam, lam2 in [(0.0, 0.1), (0.3, 0.35), (0.5, 0.55), (0.8, 0.9), (1.0, 0.95)]:
+        d = L.alch_data(p, lam, lam2, elec_lambda_start=0.5, vdw_lambda_end=0.6, vdw_shift_coeff=5.0)
+        el = lambda x: 0.0 if x <= 0.5 else (x - 0.5) / 0.5
+        vl = lambda x: 1.0 if x >= 0.6 else x / 0.6
+        exp = dict(lambdaUp=lam, lambdaDown=1 - lam, elecLambdaUp=el(lam), elecLambdaDown=el(1 - lam),
+                   vdwLambdaUp=vl(lam), vdwLambdaDown=vl(1 - lam), lambda2Up=lam2, elecLambda2Up=el(lam2),
+                   vdwLambda2Up=vl(lam2), vdwLambda2Down=vl(1 - lam2),
+                   vdwShiftUp=5.0 * (1 - vl(lam)), vdwShiftDown=5.0 * (1 - vl(1 - lam)),
+                   vdwShift2Up=5.0 * (1 - vl(lam2)), vdwShift2Down=5.0 * (1 - vl(1 - lam2)),
+                   cutoff2=144.0, switchdist2=100.0, switchfactor=1.0 / 44.0 ** 3)
+        for k, v in exp.items():
+            assert abs(getattr(d, k) - v) <= 1e-6 * max(1.0, abs(v)), (lam, k, getattr(d, k), v)
+        assert d.alchDecouple == 0
+    with pytest.raises(L.Mi355nbError):
+        L.alch_data(p, 1.5, 0.0)
 
 
 def test_nonbonded_coef_matches_formulas():

@@ -1,0 +1,145 @@
+"""Batched systems (one launch for many independent boxes), alchemical FEP
+pair math and HIP-graph replays, through the C ABI, against the FP64
+oracles run system by system."""
+import math
+
+import numpy as np
+import pytest
+
+from mi355nb import lib as L
+from mi355nb import systems
+from mi355nb.compute import ComputeNonbonded
+from oracle import oracle as orc
+
+from nbcheck import rel_rms
+
+COULOMB = 332.0636
+FORCE_TOL = 1e-5     # relative RMS vs the FP64 oracle
+ENERGY_TOL = 1e-5    # relative, per system
+VIRIAL_TOL = 2e-5    # relative (Frobenius), per system
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def members():
+    return systems.batch_systems(n_systems=6, seed=3, n_min=3000, n_max=9000)
+
+
+def _alch_for(m, params, lam2_shift=0.05):
+    lam2 = min(1.0, m.lam + lam2_shift)
+    return L.alch_data(params, m.lam, lam2, elec_lambda_start=0.5, vdw_lambda_end=1.0, vdw_shift_coeff=5.0)
+
+
+def _check_system(k, m, r, f, fs, pos, npairs_acc, alch=None):
+    s = m.system
+    p = s.params
+    A, B = L.lj_mix(s.lj_sigma, s.lj_eps)
+    beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+    q = s.charge * math.sqrt(COULOMB)
+    if alch is None:
+        o = orc.nb_fp64(pos, q, s.vdw_type, A, B, s.excl_ptr, s.excl_idx, s.box, p.cutoff, p.switchdist, beta)
+        keys = ("e_vdw", "e_elec", "e_slow")
+    else:
+        o = orc.nb_fp64_alch(pos, q, s.vdw_type, A, B, s.excl_ptr, s.excl_idx, s.box, p.cutoff, p.switchdist, beta,
+                             m.partition, alch)
+        keys = ("e_vdw", "e_elec", "e_slow", "e_vdw_s", "e_elec_s", "e_slow_s")
+    npairs_acc.append(o["npairs"])
+    # relative force errors only mean something without atom overlaps
+    assert np.abs(o["f"]).max() < 1e5, "test system has an atom overlap"
+    ef, es = rel_rms(f, o["f"]), rel_rms(fs, o["fs"])
+    print(f"[batch sys {k} n={s.natoms} lam={m.lam:.3f}] force fast {ef:.2e} slow {es:.2e}", flush=True)
+    assert ef < FORCE_TOL and es < FORCE_TOL
+    elec_scale = abs(o["e_elec"] + o["e_slow"])
+    for key in keys:
+        scale = abs(o[key]) if "vdw" in key else elec_scale
+        err = abs(r[key] - o[key]) / max(scale, 1e-30)
+        print(f"    {key} gpu {r[key]:.10e} oracle {o[key]:.10e} rel {err:.2e}")
+        assert err < ENERGY_TOL, key
+    for key in ("virial", "virial_slow"):
+        err = np.linalg.norm(r[key] - o[key]) / np.linalg.norm(o[key])
+        print(f"    {key} rel {err:.2e}")
+        assert err < VIRIAL_TOL, key
+
+
+@pytest.mark.parametrize("alch", [False, True])
+def test_batch_matches_oracle_per_system(torch_cuda, members, alch):
+    torch = torch_cuda
+    stream = torch.cuda.current_stream().cuda_stream
+    c = ComputeNonbonded.from_systems([m.system for m in members], stream=stream)
+    info = c.info()
+    assert info["numSystems"] == len(members)
+    if alch:
+        part = np.concatenate([m.partition for m in members])
+        c.set_alch(part, [_alch_for(m, c.params) for m in members])
+    print(f"[batch alch={alch}] {len(members)} systems, {c.natoms} atoms, launching", flush=True)
+    c.launch_work(do_energy=True, do_virial=True, do_slow=True, do_pairlist=True)
+    per, stat, f, fs = c.finish_batch()
+    pos = c.effective_positions()
+    starts = c.sys_atom_start
+    npairs = []
+    for k, m in enumerate(members):
+        a, b = starts[k], starts[k + 1]
+        _check_system(k, m, per[k], f[a:b], fs[a:b], pos[a:b], npairs,
+                      alch=_alch_for(m, c.params) if alch else None)
+    # the exact inclusive cutoff rule: the pair count over the batch matches
+    assert stat["numPairsCutoff"] == sum(npairs)
+    # a force-only step on the refined lists reproduces the forces
+    c.launch_work(do_energy=True, do_virial=True, do_slow=True, do_pairlist=False)
+    per2, _, f2, fs2 = c.finish_batch()
+    assert rel_rms(f2, f) < 1e-6 and rel_rms(fs2, fs) < 1e-6
+    for k in range(len(members)):
+        assert abs(per2[k]["e_vdw"] - per[k]["e_vdw"]) <= 1e-9 * abs(per[k]["e_vdw"])
+    c.close()
+
+
+def test_batch_of_one_equals_single_system(torch_cuda, members):
+    torch = torch_cuda
+    stream = torch.cuda.current_stream().cuda_stream
+    s = members[0].system
+    a = ComputeNonbonded.from_system(s, stream=stream)
+    b = ComputeNonbonded.from_systems([s], stream=stream)
+    ra = a.compute()
+    b.launch_work(True, True, True, True)
+    per, stat, f, fs = b.finish_batch()
+    assert stat["numPairsCutoff"] == ra["stat"]["numPairsCutoff"]
+    assert stat["numPairsList"] == ra["stat"]["numPairsList"]
+    assert rel_rms(f, ra["f"]) < 1e-6 and rel_rms(fs, ra["fs"]) < 1e-6
+    for k in ("e_vdw", "e_elec", "e_slow"):
+        assert abs(per[0][k] - ra[k]) <= 1e-9 * abs(ra[k])
+    a.close()
+    b.close()
+
+
+def test_graph_replays_match_eager(torch_cuda, members):
+    """Force-only and energy steps replayed from captured HIP graphs, with
+    eager work and synchronisation between replays, agree with eager launches
+    every time."""
+    torch = torch_cuda
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        c = ComputeNonbonded.from_systems([m.system for m in members], stream=side.cuda_stream)
+        part = np.concatenate([m.partition for m in members])
+        c.set_alch(part, [_alch_for(m, c.params) for m in members])
+        c.launch_work(True, True, True, True)
+        per0, _, f0, fs0 = c.finish_batch()
+        c.launch_work(False, False, True, False)
+        _, _, fe, fse = c.finish_batch()
+        scratch = torch.zeros(1 << 20, device="cuda")
+        for rep in range(4):
+            print(f"[graph] replay {rep}", flush=True)
+            c.launch_graph(do_energy=False, do_virial=False, do_slow=True)
+            _, _, fg, fsg = c.finish_batch()
+            assert rel_rms(fg, fe) < 1e-6 and rel_rms(fsg, fse) < 1e-6
+            scratch.add_(1.0)   # eager work between replays
+            c.launch_graph(do_energy=True, do_virial=True, do_slow=True)
+            perg, _, fg2, _ = c.finish_batch()
+            assert rel_rms(fg2, f0) < 1e-6
+            for k in range(len(members)):
+                for key in ("e_vdw", "e_elec", "e_slow", "e_vdw_s"):
+                    assert abs(perg[k][key] - per0[k][key]) <= 1e-9 * max(abs(per0[k][key]), 1e-30), (rep, k, key)
+                assert np.linalg.norm(perg[k]["virial"] - per0[k]["virial"]) <= \
+                    1e-6 * np.linalg.norm(per0[k]["virial"])
+        torch.cuda.synchronize()
+        assert float(scratch[0]) == 4.0
+        c.close()
