@@ -130,9 +130,19 @@ double mi355nb_ewald_coefficient(double cutoff, double tolerance);
 int mi355nb_get_nonbonded_coef(double cutoff, double switchdist, double ewaldcof, double slowScale,
                                mi355nb_nb_constants* out);
 
-/* CudaComputeNonbonded::getDoTable; replaces src/CudaComputeNonbonded.C:2351-2366.
- * Always 0: this library evaluates LJ + Ewald by direct math on every step. */
+/* CudaComputeNonbonded::getDoTable; replaces src/CudaComputeNonbonded.C:2351-2366:
+ * the reference's choice (useCUDANonbondedForceTable || doSlow).  The
+ * library's default is direct math; a caller that wants the reference's
+ * choice passes this value to mi355nb_compute_set_force_table. */
 int mi355nb_get_do_table(int useCUDANonbondedForceTable, int doSlow, int doVirial);
+
+/* CudaNonbondedTables::buildForceAndEnergyTables (nonbonded set); replaces
+ * src/CudaNonbondedTables.C:158-203, :263-300.  tableSize float4 entries
+ * indexed by 1/r (entry i at r = tableSize/(i + 0.5)), columns (fast, -vdwB,
+ * vdwA, slow correction); force4 = 2 d/d(r^2) of energy4.  Entries are FP64
+ * closed forms at r (the reference samples its r^2 splines).  Host only. */
+int mi355nb_build_force_energy_tables(int tableSize, double cutoff, double switchdist, double ewaldcof,
+                                      float* force4, float* energy4);
 
 /* LJTable::compute_vdw_params mixing; replaces src/LJTable.C:118-135.
  * A and B are ntypes*ntypes row-major. */
@@ -219,6 +229,24 @@ int mi355nb_compute_finish(mi355nb_compute* c, double* force, double* forceSlow,
 int mi355nb_compute_finish_batch(mi355nb_compute* c, double* force, double* forceSlow,
                                  mi355nb_virial_energy* vePerSystem, mi355nb_tile_list_stat* stat);
 
+/* Replaces charges (e, per atom) and/or the LJ A/B table (ntypes^2) of the
+ * current molecule without rebuilding the layout or the pairlists (a
+ * parameter-fitting step).  NULL leaves that part unchanged.  No counterpart
+ * in the reference, whose parameters are fixed for a run. */
+int mi355nb_compute_update_parameters(mi355nb_compute* c, const double* charge, const double* lj_A,
+                                      const double* lj_B);
+
+/* Force-matching adjoint: evaluates F = fast + slow nonbonded force on the
+ * current tile lists, the loss L = sum_atoms |F - T|^2 for the target T
+ * (natoms*3 doubles, global atom order), and its gradient: dL/dA and dL/dB
+ * (ntypes^2 doubles each, ordered bins [t_i][t_j] over pairs (i, j); the
+ * gradient of a symmetric table entry is bin[a][b] + bin[b][a] for a != b)
+ * and dL/dq (natoms doubles, per unit charge e).  FP32 pair terms, FP64 sums.
+ * Needs a pairlist launch first, at most 24 LJ types and no alchemical
+ * partitions.  Synchronous.  No counterpart in the reference. */
+int mi355nb_compute_force_matching_grad(mi355nb_compute* c, const double* targetForce, double* loss,
+                                        double* dLdA, double* dLdB, double* dLdq);
+
 /* A non-pairlist launch replayed from a HIP graph: the first call for each
  * (doEnergy, doVirial, doSlow) captures the launch sequence on the compute's
  * stream (which must not be NULL); later calls replay it.  Graphs are dropped
@@ -233,6 +261,12 @@ int mi355nb_compute_launch_graph(mi355nb_compute* c, int doEnergy, int doVirial,
  * get_kernel_time synchronises and returns the summed milliseconds and
  * launch counts since the last reset, split by pairlist / non-pairlist. */
 int mi355nb_compute_set_timing(mi355nb_compute* c, int enable);
+
+/* Table mode (doTable of CudaComputeNonbondedKernel::nonbondedForce,
+ * src/CudaComputeNonbondedKernel.cu:99-161): later launches take LJ and
+ * electrostatics from 4096-entry 1/r tables with linear interpolation
+ * instead of direct math.  Not combinable with alchemical partitions. */
+int mi355nb_compute_set_force_table(mi355nb_compute* c, int enable);
 
 /* Launch shape of the pair kernel on non-pairlist steps: persistent
  * workgroups (4 waves each) per CU, 1..64, or 0 (default) for one wave per

@@ -101,6 +101,13 @@ struct mi355nb_compute {
   DevBuf<signed char> part;
   DevBuf<int4> redBlocks;
   DevBuf<int> redSysStart;
+  // NAMD table mode (optional)
+  bool tableMode = false;
+  int tableSize = 4096;   // FORCE_ENERGY_TABLE_SIZE of the reference
+  DevBuf<F4> forceTable, energyTable;
+  // force-matching adjoint scratch
+  DevBuf<double> fmTarget, fmLossPartials, fmLoss, fmGq, fmBinPartials, fmBins;
+  DevBuf<F4> fmG;
 
   // host staging
   std::vector<F4> hForce, hForceSlow;
@@ -130,6 +137,9 @@ struct mi355nb_compute {
   void releaseAll() {
     dropGraphs();
     alch.release(); part.release(); redBlocks.release(); redSysStart.release();
+    forceTable.release(); energyTable.release();
+    fmTarget.release(); fmLossPartials.release(); fmLoss.release(); fmGq.release(); fmBinPartials.release();
+    fmBins.release(); fmG.release();
     xyzq.release(); force.release(); forceSlow.release();
     vdwTypes.release(); atomIndex.release(); excl.release(); bits.release(); vdwCoef.release();
     patches.release(); bbox.release(); listsCand.release(); listsAct.release();
@@ -175,7 +185,18 @@ int mi355nb_get_nonbonded_coef(double cutoff, double switchdist, double ewaldcof
   return MI355NB_OK;
 }
 
-int mi355nb_get_do_table(int, int, int) { return 0; }
+int mi355nb_get_do_table(int useCUDANonbondedForceTable, int doSlow, int doVirial) {
+  return (useCUDANonbondedForceTable || doSlow || (doSlow && doVirial)) ? 1 : 0;
+}
+
+int mi355nb_build_force_energy_tables(int tableSize, double cutoff, double switchdist, double ewaldcof,
+                                      float* force4, float* energy4) {
+  CHECK_ARG(tableSize >= 2 && force4 && energy4 && cutoff > 0 && switchdist > 0 && switchdist <= cutoff &&
+                ewaldcof >= 0,
+            "bad table arguments");
+  buildForceEnergyTables(tableSize, cutoff, switchdist, ewaldcof, force4, energy4);
+  return MI355NB_OK;
+}
 
 int mi355nb_lj_mix(int ntypes, const double* sigma, const double* epsilon, int geometric, double* A,
                    double* B) {
@@ -492,6 +513,7 @@ int mi355nb_compute_set_alch(mi355nb_compute* c, const int8_t* partition, int nu
     return MI355NB_OK;
   }
   CHECK_ARG(perSystem && numSystems >= 1, "bad alchemical arguments");
+  CHECK_STATE(!c->tableMode, "table mode and alchemical partitions cannot be combined");
   if (c->haveLayout) CHECK_ARG(numSystems == c->L.numSystems, "one alchemical record per system");
   for (int i = 0; i < c->natoms; ++i) CHECK_ARG(partition[i] >= 0 && partition[i] <= 2, "partition must be 0, 1 or 2");
   const float c2 = (float)(c->p.cutoff * c->p.cutoff), s2 = (float)(c->p.switchdist * c->p.switchdist);
@@ -560,6 +582,9 @@ static int launchImpl(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow
   a.wgPerCU = c->wgPerCU;
   a.alch = c->alchOn ? c->alch.ptr : nullptr;
   a.part = c->alchOn ? c->part.ptr : nullptr;
+  a.forceTable = c->tableMode ? c->forceTable.ptr : nullptr;
+  a.energyTable = c->tableMode ? c->energyTable.ptr : nullptr;
+  a.tableSize = c->tableSize;
 
   if (doPairlist) {
     const float cull = (float)(c->p.pairlistdist + 0.01);
@@ -661,6 +686,113 @@ int mi355nb_compute_launch_graph(mi355nb_compute* c, int doEnergy, int doVirial,
   return MI355NB_OK;
 }
 
+int mi355nb_compute_update_parameters(mi355nb_compute* c, const double* charge, const double* lj_A,
+                                      const double* lj_B) {
+  CHECK_ARG(c, "null compute");
+  CHECK_STATE(c->haveMolecule, "set_molecule must precede update_parameters");
+  CHECK_ARG((lj_A == nullptr) == (lj_B == nullptr), "give both LJ tables or neither");
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  if (charge) {
+    for (int i = 0; i < c->natoms; ++i) CHECK_ARG(std::isfinite(charge[i]), "non-finite charge");
+    const double qs = std::sqrt(COULOMB);
+    for (int i = 0; i < c->natoms; ++i) c->qScaled[i] = charge[i] * qs;
+    if (c->haveLayout) {
+      for (int k = 0; k < c->L.atomStorageSize; ++k) {
+        const int a = c->L.atomIndex[k];
+        c->L.xyzq[k].w = a >= 0 ? (float)c->qScaled[a] : 0.0f;
+      }
+      int rc = uploadSlots(c);
+      if (rc) return rc;
+    }
+  }
+  if (lj_A) {
+    const int nt = c->ntypes;
+    for (int k = 0; k < nt * nt; ++k) {
+      CHECK_ARG(std::isfinite(lj_A[k]) && std::isfinite(lj_B[k]), "non-finite LJ coefficient");
+      c->vdwCoefHost[2 * k] = (float)lj_A[k];
+      c->vdwCoefHost[2 * k + 1] = (float)lj_B[k];
+    }
+    MI355NB_HIP(hipMemcpyAsync(c->vdwCoef.ptr, c->vdwCoefHost.data(), c->vdwCoefHost.size() * sizeof(float),
+                               hipMemcpyHostToDevice, c->stream));
+  }
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_force_matching_grad(mi355nb_compute* c, const double* targetForce, double* loss, double* dLdA,
+                                        double* dLdB, double* dLdq) {
+  CHECK_ARG(c && targetForce, "null argument");
+  CHECK_STATE(c->haveLayout && c->haveLists, "a pairlist launch must precede force_matching_grad");
+  CHECK_STATE(!c->alchOn, "the force-matching adjoint does not support alchemical partitions");
+  if (c->ntypes > 24) {
+    setLastError("force-matching adjoint: more than 24 LJ types (its LDS bins hold 24)");
+    return MI355NB_ERR_CAPACITY;
+  }
+  MI355NB_HIP(hipSetDevice(c->device));
+  const int n = c->L.atomStorageSize;
+  const int nt = c->ntypes;
+  const int nbins = 2 * nt * nt;
+  const int gb = adjointGridBlocks();
+  CHECK_STATE(gb > 0, "no device properties");
+  MI355NB_HIP(c->fmTarget.reserve(4 * (size_t)n));
+  MI355NB_HIP(c->fmLossPartials.reserve(fmResidualBlocks(n) + 1));
+  MI355NB_HIP(c->fmLoss.reserve(1));
+  MI355NB_HIP(c->fmGq.reserve(n));
+  MI355NB_HIP(c->fmBinPartials.reserve((size_t)gb * nbins));
+  MI355NB_HIP(c->fmBins.reserve(nbins));
+  MI355NB_HIP(c->fmG.reserve(n));
+  // forward: fast + slow forces on the current lists
+  int rc = launchImpl(c, 0, 0, 1, 0, false);
+  if (rc) return rc;
+  std::vector<double> tgt(4 * (size_t)n, 0.0);
+  for (int a = 0; a < c->natoms; ++a) {
+    const int k = c->L.slotOfAtom[a];
+    for (int d = 0; d < 3; ++d) tgt[4 * (size_t)k + d] = targetForce[3 * (size_t)a + d];
+    tgt[4 * (size_t)k + 3] = 1.0;
+  }
+  MI355NB_HIP(hipMemcpyAsync(c->fmTarget.ptr, tgt.data(), tgt.size() * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+  MI355NB_HIP(launchFill(2 * (size_t)n, reinterpret_cast<float*>(c->fmGq.ptr), 0.0f, c->stream));
+  AdjointArgsHost h;
+  h.nslots = n;
+  h.force = c->force.ptr;
+  h.forceSlow = c->forceSlow.ptr;
+  h.target = c->fmTarget.ptr;
+  h.g = c->fmG.ptr;
+  h.lossPartials = c->fmLossPartials.ptr;
+  h.loss = c->fmLoss.ptr;
+  h.numLists = &c->stat.ptr->numTileLists;
+  h.tileLists = c->listsAct.ptr;
+  h.tileJatomStart = c->jAct.ptr;
+  h.tileExcls = c->exAct.ptr;
+  h.xyzq = c->xyzq.ptr;
+  h.vdwTypes = c->vdwTypes.ptr;
+  h.ntypes = nt;
+  h.c = c->c;
+  h.cut = makeCutTest(c->p.cutoff);
+  h.gq = c->fmGq.ptr;
+  h.gridBlocks = gb;
+  h.binPartials = c->fmBinPartials.ptr;
+  h.bins = c->fmBins.ptr;
+  MI355NB_HIP(launchForceMatchingAdjoint(h, c->stream));
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  double l = 0.0;
+  MI355NB_HIP(hipMemcpy(&l, c->fmLoss.ptr, sizeof(double), hipMemcpyDeviceToHost));
+  if (loss) *loss = l;
+  std::vector<double> bins(nbins);
+  MI355NB_HIP(hipMemcpy(bins.data(), c->fmBins.ptr, nbins * sizeof(double), hipMemcpyDeviceToHost));
+  if (dLdA) std::memcpy(dLdA, bins.data(), (size_t)nt * nt * sizeof(double));
+  if (dLdB) std::memcpy(dLdB, bins.data() + (size_t)nt * nt, (size_t)nt * nt * sizeof(double));
+  if (dLdq) {
+    std::vector<double> gq(n);
+    MI355NB_HIP(hipMemcpy(gq.data(), c->fmGq.ptr, n * sizeof(double), hipMemcpyDeviceToHost));
+    const double qs = std::sqrt(COULOMB);
+    for (int a = 0; a < c->natoms; ++a) dLdq[a] = gq[c->L.slotOfAtom[a]] * qs;
+  }
+  return MI355NB_OK;
+}
+
 static int copyForces(mi355nb_compute* c, double* forceOut, double* forceSlowOut) {
   const int n = c->L.atomStorageSize;
   if (forceOut) {
@@ -712,6 +844,24 @@ int mi355nb_compute_finish_batch(mi355nb_compute* c, double* forceOut, double* f
   if (vePerSystem)
     MI355NB_HIP(hipMemcpy(vePerSystem, c->ve.ptr, sizeof(VirialEnergy) * c->L.numSystems, hipMemcpyDeviceToHost));
   if (statOut) MI355NB_HIP(hipMemcpy(statOut, c->stat.ptr, sizeof(TileListStat), hipMemcpyDeviceToHost));
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_set_force_table(mi355nb_compute* c, int enable) {
+  CHECK_ARG(c, "null compute");
+  if (enable) CHECK_STATE(!c->alchOn, "table mode and alchemical partitions cannot be combined");
+  MI355NB_HIP(hipSetDevice(c->device));
+  c->dropGraphs();
+  if (enable && !c->forceTable.ptr) {
+    const int N = c->tableSize;
+    std::vector<float> ft(4 * (size_t)N), et(4 * (size_t)N);
+    buildForceEnergyTables(N, c->p.cutoff, c->p.switchdist, c->p.ewaldcof, ft.data(), et.data());
+    MI355NB_HIP(c->forceTable.reserve(N, 1.0f));
+    MI355NB_HIP(c->energyTable.reserve(N, 1.0f));
+    MI355NB_HIP(hipMemcpy(c->forceTable.ptr, ft.data(), ft.size() * sizeof(float), hipMemcpyHostToDevice));
+    MI355NB_HIP(hipMemcpy(c->energyTable.ptr, et.data(), et.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  c->tableMode = enable != 0;
   return MI355NB_OK;
 }
 

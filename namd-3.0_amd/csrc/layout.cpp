@@ -56,6 +56,53 @@ NbConstants getNonbondedCoef(double cutoffD, double switchD, double ewaldcof, do
   return c;
 }
 
+void buildForceEnergyTables(int N, double cutoff, double switchdist, double ewaldcof, float* force4,
+                            float* energy4) {
+  const double rc2 = cutoff * cutoff, rs2 = switchdist * switchdist;
+  const double c1 = (switchdist < cutoff) ? 1.0 / ((rc2 - rs2) * (rc2 - rs2) * (rc2 - rs2)) : 0.0;
+  const double c3 = 3.0 * (rc2 - rs2);
+  const double twoOverSqrtPi = 1.1283791670955126;
+  for (int k = 0; k < 4 * N; ++k) force4[k] = energy4[k] = 0.0f;
+  for (int i = 1; i < N; ++i) {
+    const double r = (double)N / ((double)i + 0.5);
+    if (r > cutoff) continue;
+    const double r2 = r * r, rinv = 1.0 / r, rinv2 = rinv * rinv;
+    // C1 split: slow_E = (3 - r^2/rc^2) / (2 rc), fast_E = 1/r - slow_E
+    const double slowE = 0.5 / cutoff * (3.0 - r2 / rc2);
+    const double slowG = -r / (rc2 * cutoff);             // d slowE / dr
+    const double fastE = rinv - slowE;
+    const double fastG = -rinv2 - slowG;                   // d fastE / dr
+    // scor = erfc(beta r)/r - fast = slow_E + (erfc(beta r) - 1)/r
+    const double a = ewaldcof * r;
+    const double ec = std::erfc(a);
+    const double corrG = twoOverSqrtPi * a * std::exp(-a * a) + ec;
+    const double scorE = slowE + (ec - 1.0) * rinv;
+    const double scorG = slowG - (corrG - 1.0) * rinv2;    // d scorE / dr
+    // energy switch S(r^2) and dS/d(r^2)
+    double S = 1.0, dS = 0.0;
+    if (r2 > rs2) {
+      const double c2 = rc2 - r2, c4 = c2 * (c3 - 2.0 * c2);
+      S = c2 * c4 * c1;
+      dS = 2.0 * c1 * (c2 * c2 - c4);
+    }
+    const double r6 = rinv2 * rinv2 * rinv2, r12 = r6 * r6;
+    const double vdwaE = S * r12, vdwbE = S * r6;
+    const double vdwaD = (dS - 6.0 * S * rinv2) * r12;    // d/d(r^2)
+    const double vdwbD = (dS - 3.0 * S * rinv2) * r6;
+    // 2 d/d(r^2) = (1/r) d/dr
+    force4[4 * i + 0] = (float)(fastG * rinv);
+    force4[4 * i + 1] = (float)(-2.0 * vdwbD);
+    force4[4 * i + 2] = (float)(2.0 * vdwaD);
+    force4[4 * i + 3] = (float)(scorG * rinv);
+    energy4[4 * i + 0] = (float)fastE;
+    energy4[4 * i + 1] = (float)(-vdwbE);
+    energy4[4 * i + 2] = (float)vdwaE;
+    energy4[4 * i + 3] = (float)scorE;
+  }
+  if (N > 1)
+    for (int k = 0; k < 4; ++k) energy4[k] = energy4[4 + k];
+}
+
 void ljMix(int ntypes, const double* sigma, const double* eps, int geometric, double* A, double* B) {
   for (int i = 0; i < ntypes; ++i) {
     for (int j = 0; j < ntypes; ++j) {

@@ -31,6 +31,16 @@ double ewaldCoefficient(double cutoff, double tol);
 NbConstants getNonbondedCoef(double cutoff, double switchdist, double ewaldcof,
                              double slowScale);
 
+// NAMD's GPU force/energy tables (CudaNonbondedTables::buildForceAndEnergyTables,
+// src/CudaNonbondedTables.C:158-203, :263-300): N float4 entries indexed by
+// 1/r, entry i at r = N/(i + 0.5) for i >= 1, zero beyond the cutoff;
+// columns (x, y, z, w) = (fast, -vdwB, vdwA, slow correction); force columns
+// are 2 d/d(r^2) of the energy columns.  Here every entry is the FP64
+// closed form at r (NAMD samples its r^2 cubic splines there).  Entry 0 has
+// zero force and the energy of entry 1.  force4 / energy4: 4*N floats.
+void buildForceEnergyTables(int N, double cutoff, double switchdist, double ewaldcof, float* force4,
+                            float* energy4);
+
 // LJTable::compute_vdw_params mixing (src/LJTable.C:118-135):
 // sigma_ij = (s_i+s_j)/2 (or sqrt(s_i s_j) if geometric), eps_ij = sqrt(e_i e_j),
 // B = 4 eps sigma^6, A = B sigma^6.

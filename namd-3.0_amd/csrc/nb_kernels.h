@@ -34,6 +34,9 @@ struct ForceArgsHost {
   int wgPerCU;            // persistent workgroups per CU (4 waves each); 0: one wave per list
   const AlchParams* alch; // per-system factors; null: no alchemical partitions
   const signed char* part;// per-slot partition (alchemical only)
+  const F4* forceTable;   // table mode: N float4 force entries; null: direct math
+  const F4* energyTable;
+  int tableSize;
 };
 
 hipError_t launchBoundingBoxes(int numTiles, const F4* xyzq, BoundingBox* bbox, hipStream_t s);
@@ -61,5 +64,33 @@ hipError_t launchReduceVirialEnergy(int numBlocks, int numSystems, const int4* b
                                     const F4* xyzq, const F4* f, const F4* fs, const TileListVirialEnergy* tlve,
                                     bool doEnergy, bool doVirial, bool doSlow, double* partials,
                                     VirialEnergy* out, hipStream_t s);
+
+// Force-matching adjoint (adjoint.hip)
+struct AdjointArgsHost {
+  int nslots;
+  const F4* force;          // per slot, after the transpose
+  const F4* forceSlow;
+  const double* target;     // per slot: Tx, Ty, Tz, valid (0 for padding)
+  F4* g;                    // per slot scratch: dL/dF
+  double* lossPartials;     // fmResidualBlocks(nslots)
+  double* loss;             // 1
+  const int* numLists;
+  const TileList* tileLists;
+  const int* tileJatomStart;
+  const TileExcl* tileExcls;
+  const F4* xyzq;
+  const int* vdwTypes;
+  int ntypes;
+  NbConstants c;
+  CutTest cut;
+  double* gq;               // per slot, zeroed by the caller
+  int gridBlocks;           // adjointGridBlocks()
+  double* binPartials;      // gridBlocks * 2 * ntypes^2
+  double* bins;             // 2 * ntypes^2: dL/dA then dL/dB, ordered (t_i, t_j)
+};
+int adjointGridBlocks();
+size_t adjointSharedBytes(int ntypes);
+int fmResidualBlocks(int n);
+hipError_t launchForceMatchingAdjoint(const AdjointArgsHost& h, hipStream_t s);
 
 }  // namespace mi355nb

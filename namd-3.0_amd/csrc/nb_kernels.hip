@@ -17,6 +17,7 @@
 
 #include <algorithm>
 
+#include "nb_device.h"
 #include "nb_kernels.h"
 #include "slowforce.h"
 
@@ -238,21 +239,36 @@ __device__ __forceinline__ void pairForceAlch(const float r2, const float qq, co
   }
 }
 
-// r^2 <= c2 exactly; dx, dy, dz are exact (multiples of 2^-17 A below 128 A),
-// so their FP64 squares and sum are exact.
-__device__ __forceinline__ bool within(float dx, float dy, float dz, float r2, const CutTest& t) {
-  if (r2 < t.lo) return true;
-  if (r2 > t.hi) return false;
-  const double x = dx, y = dy, z = dz;
-  return x * x + y * y + z * z <= t.c2;
-}
-
-// Orders LDS accesses between the lanes of one wave: a compiler barrier plus
-// wavefront-scope fences (a wave's LDS operations complete in order).
-__device__ __forceinline__ void waveSync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// NAMD's table mode (calcForceEnergy with the 1/r force/energy tables,
+// src/CudaComputeNonbondedKernel.cu:99-161; the HIP build's manual linear
+// filtering, src/CudaComputeNonbondedKernelHip.cu:170-220): the float4 entry
+// at x = N/r - 0.5 is interpolated linearly between entries floor(x) and
+// floor(x) + 1 (both clamped to the table, so pairs closer than N/(N - 0.5) A
+// see the last entry), then
+//    f = A T.z + B T.y + qq T.x,  fSlow = qq T.w
+// and on energy steps the energy table gives A E.z + B E.y, qq E.x, qq E.w.
+template <bool doEnergy, bool doSlow, typename Acc>
+__device__ __forceinline__ void pairForceTable(const float r2, const float qq, const float A, const float B,
+                                               const F4* __restrict__ ft, const F4* __restrict__ et, const int N,
+                                               float& f, float& fSlow, Acc& eVdw, Acc& eElec, Acc& eSlow) {
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float x = rinv * (float)N - 0.5f;
+  const float fl = floorf(x);
+  const float a = x - fl;
+  const int i0 = min(max((int)fl, 0), N - 1);
+  const int i1 = min(i0 + 1, N - 1);
+  const F4 t0 = ft[i0], t1 = ft[i1];
+  const float tx = a * (t1.x - t0.x) + t0.x;
+  const float ty = a * (t1.y - t0.y) + t0.y;
+  const float tz = a * (t1.z - t0.z) + t0.z;
+  f = A * tz + B * ty + qq * tx;
+  if (doSlow) fSlow = qq * (a * (t1.w - t0.w) + t0.w);
+  if (doEnergy) {
+    const F4 e0 = et[i0], e1 = et[i1];
+    eVdw.v += A * (a * (e1.z - e0.z) + e0.z) + B * (a * (e1.y - e0.y) + e0.y);
+    eElec.v += qq * (a * (e1.x - e0.x) + e0.x);
+    if (doSlow) eSlow.v += qq * (a * (e1.w - e0.w) + e0.w);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -302,11 +318,21 @@ struct ForceArgs {
   // alchemical
   const AlchParams* alch;            // per system
   const signed char* part;           // per slot: partition 0, 1, 2
+  // table mode
+  const F4* forceTable;
+  const F4* energyTable;
+  int tableSize;
 };
 
-template <bool doEnergy, bool doVirial, bool doSlow, bool doPairlist, bool doAlch>
+// pair-math variants of the kernel
+constexpr int PAIR_DIRECT = 0;
+constexpr int PAIR_ALCH = 1;
+constexpr int PAIR_TABLE = 2;
+
+template <bool doEnergy, bool doVirial, bool doSlow, bool doPairlist, int mode>
 __global__ void __launch_bounds__(64 * NB_WAVES)
 nonbondedForceKernel(const ForceArgs args) {
+  constexpr bool doAlch = mode == PAIR_ALCH;
   extern __shared__ float2 s_lj[];   // vdwWidth^2 (A, B)
   __shared__ F4 s_xyzq[NB_WAVES][TILE];
   __shared__ int s_type[NB_WAVES][TILE];     // vdw type | partition << 16 (alchemical)
@@ -438,11 +464,15 @@ nonbondedForceKernel(const ForceArgs args) {
         }
         if (active) {
           float f, fs = 0.f;
-          if (doAlch) {
+          if constexpr (mode == PAIR_ALCH) {
             const int tw = st[j];
             const float2 ab = s_lj[vdwRow + (tw & 0xffff)];
             pairForceAlch<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, iPart, tw >> 16, ap, f,
                                             fs, eVdw, eElec, eSlow, eVdw2, eElec2, eSlow2);
+          } else if constexpr (mode == PAIR_TABLE) {
+            const float2 ab = s_lj[vdwRow + st[j]];
+            pairForceTable<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.forceTable, args.energyTable,
+                                             args.tableSize, f, fs, eVdw, eElec, eSlow);
           } else {
             const float2 ab = s_lj[vdwRow + st[j]];
             pairForce<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, eVdw, eElec, eSlow);
@@ -841,6 +871,9 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
   a.virialEnergy = h.virialEnergy;
   a.alch = h.alch;
   a.part = h.part;
+  a.forceTable = h.forceTable;
+  a.energyTable = h.energyTable;
+  a.tableSize = h.tableSize;
   // persistent grid of wgPerCU workgroups per CU, or (wgPerCU == 0) one wave
   // per list.  Pairlist steps walk the candidate lists in layout order, whose
   // depths vary widely, so they always take one wave per list.
@@ -855,22 +888,23 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
   const int wgCap = (doPairlist || h.wgPerCU <= 0) ? wgNeeded : numCU * h.wgPerCU;
   dim3 g(std::min(wgNeeded, wgCap)), b(64 * NB_WAVES);
   const size_t shmem = sizeof(float2) * (size_t)h.vdwWidth * h.vdwWidth;
-  const bool doAlch = h.alch != nullptr;
-  const int key = (doAlch ? 16 : 0) | (doEnergy ? 8 : 0) | (doVirial ? 4 : 0) | (doSlow ? 2 : 0) | (doPairlist ? 1 : 0);
+  const int mode = h.alch ? PAIR_ALCH : (h.forceTable ? PAIR_TABLE : PAIR_DIRECT);
+  const int key = mode * 16 + (doEnergy ? 8 : 0) + (doVirial ? 4 : 0) + (doSlow ? 2 : 0) + (doPairlist ? 1 : 0);
   switch (key) {
-#define NB_CASE(A, E, V, S, P)                                                               \
-  case (A * 16 + E * 8 + V * 4 + S * 2 + P):                                                 \
-    hipLaunchKernelGGL((nonbondedForceKernel<E, V, S, P, A>), g, b, shmem, s, a);            \
+#define NB_CASE(M, E, V, S, P)                                                               \
+  case (M * 16 + E * 8 + V * 4 + S * 2 + P):                                                 \
+    hipLaunchKernelGGL((nonbondedForceKernel<E, V, S, P, M>), g, b, shmem, s, a);            \
     break;
-    NB_CASE(0, 0, 0, 0, 0) NB_CASE(0, 0, 0, 0, 1) NB_CASE(0, 0, 0, 1, 0) NB_CASE(0, 0, 0, 1, 1)
-    NB_CASE(0, 0, 1, 0, 0) NB_CASE(0, 0, 1, 0, 1) NB_CASE(0, 0, 1, 1, 0) NB_CASE(0, 0, 1, 1, 1)
-    NB_CASE(0, 1, 0, 0, 0) NB_CASE(0, 1, 0, 0, 1) NB_CASE(0, 1, 0, 1, 0) NB_CASE(0, 1, 0, 1, 1)
-    NB_CASE(0, 1, 1, 0, 0) NB_CASE(0, 1, 1, 0, 1) NB_CASE(0, 1, 1, 1, 0) NB_CASE(0, 1, 1, 1, 1)
-    NB_CASE(1, 0, 0, 0, 0) NB_CASE(1, 0, 0, 0, 1) NB_CASE(1, 0, 0, 1, 0) NB_CASE(1, 0, 0, 1, 1)
-    NB_CASE(1, 0, 1, 0, 0) NB_CASE(1, 0, 1, 0, 1) NB_CASE(1, 0, 1, 1, 0) NB_CASE(1, 0, 1, 1, 1)
-    NB_CASE(1, 1, 0, 0, 0) NB_CASE(1, 1, 0, 0, 1) NB_CASE(1, 1, 0, 1, 0) NB_CASE(1, 1, 0, 1, 1)
-    NB_CASE(1, 1, 1, 0, 0) NB_CASE(1, 1, 1, 0, 1) NB_CASE(1, 1, 1, 1, 0) NB_CASE(1, 1, 1, 1, 1)
+#define NB_MODE(M)                                                                                           \
+  NB_CASE(M, 0, 0, 0, 0) NB_CASE(M, 0, 0, 0, 1) NB_CASE(M, 0, 0, 1, 0) NB_CASE(M, 0, 0, 1, 1)               \
+  NB_CASE(M, 0, 1, 0, 0) NB_CASE(M, 0, 1, 0, 1) NB_CASE(M, 0, 1, 1, 0) NB_CASE(M, 0, 1, 1, 1)               \
+  NB_CASE(M, 1, 0, 0, 0) NB_CASE(M, 1, 0, 0, 1) NB_CASE(M, 1, 0, 1, 0) NB_CASE(M, 1, 0, 1, 1)               \
+  NB_CASE(M, 1, 1, 0, 0) NB_CASE(M, 1, 1, 0, 1) NB_CASE(M, 1, 1, 1, 0) NB_CASE(M, 1, 1, 1, 1)
+    NB_MODE(0) NB_MODE(1) NB_MODE(2)
+#undef NB_MODE
 #undef NB_CASE
+    default:
+      return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -94,6 +94,7 @@ class ComputeNonbonded:
         A = np.ascontiguousarray(A, np.float64).reshape(-1)
         B = np.ascontiguousarray(B, np.float64).reshape(-1)
         ntypes = int(round(math.sqrt(A.size)))
+        self.ntypes = ntypes
         excl_ptr = np.ascontiguousarray(excl_ptr, np.int64)
         excl_idx = np.ascontiguousarray(excl_idx, np.int32)
         if excl_idx.size == 0:
@@ -123,6 +124,31 @@ class ComputeNonbonded:
         part = np.ascontiguousarray(partition, np.int8)
         arr = (L.AlchData * len(alch))(*alch)
         L.check(L.lib().mi355nb_compute_set_alch(self._h, part.ctypes.data, len(alch), arr), "set_alch")
+
+    def update_parameters(self, charge=None, A=None, B=None):
+        """New charges (e) and/or LJ tables without rebuilding layout or pairlists."""
+        q = None if charge is None else np.ascontiguousarray(charge, np.float64)
+        a = None if A is None else np.ascontiguousarray(A, np.float64).reshape(-1)
+        b = None if B is None else np.ascontiguousarray(B, np.float64).reshape(-1)
+        L.check(L.lib().mi355nb_compute_update_parameters(
+            self._h, None if q is None else q.ctypes.data, None if a is None else a.ctypes.data,
+            None if b is None else b.ctypes.data), "update_parameters")
+
+    def force_matching_grad(self, target):
+        """Loss sum |F - T|^2 (F = fast + slow) and its gradient:
+        (loss, dL/dA [nt, nt] ordered bins, dL/dB, dL/dq per atom)."""
+        n = self.natoms
+        nt = self.ntypes
+        t = np.ascontiguousarray(target, np.float64).reshape(-1)
+        if t.size != 3 * n:
+            raise ValueError("target must hold natoms x 3 forces")
+        loss = C.c_double(0)
+        gA = np.zeros(nt * nt)
+        gB = np.zeros(nt * nt)
+        gq = np.zeros(n)
+        L.check(L.lib().mi355nb_compute_force_matching_grad(self._h, t, C.byref(loss), gA.ctypes.data,
+                                                            gB.ctypes.data, gq.ctypes.data), "force_matching_grad")
+        return loss.value, gA.reshape(nt, nt), gB.reshape(nt, nt), gq
 
     def update_positions(self, pos):
         L.check(L.lib().mi355nb_compute_update_positions(self._h, np.ascontiguousarray(pos, np.float64).reshape(-1)),
@@ -187,6 +213,10 @@ class ComputeNonbonded:
     # --- kernel timing (HIP events on the compute's stream) ----------------
     def set_timing(self, enable=True):
         L.check(L.lib().mi355nb_compute_set_timing(self._h, int(enable)), "set_timing")
+
+    def set_force_table(self, enable=True):
+        """NAMD's 1/r table mode (True) or direct math (False, the default)."""
+        L.check(L.lib().mi355nb_compute_set_force_table(self._h, int(enable)), "set_force_table")
 
     def set_launch_config(self, workgroups_per_cu):
         L.check(L.lib().mi355nb_compute_set_launch_config(self._h, int(workgroups_per_cu)), "set_launch_config")

@@ -71,6 +71,8 @@ EXPORTED = [
     "mi355nb_compute_get_tile_lists", "mi355nb_spread_charge", "mi355nb_gather_force", "mi355nb_self_energy",
     "mi355nb_pme_frac_coords", "mi355nb_compute_assign_patches_batch", "mi355nb_alch_data_fill",
     "mi355nb_compute_set_alch", "mi355nb_compute_finish_batch", "mi355nb_compute_launch_graph",
+    "mi355nb_compute_update_parameters", "mi355nb_compute_force_matching_grad",
+    "mi355nb_build_force_energy_tables", "mi355nb_compute_set_force_table",
 ]
 
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -122,6 +124,11 @@ def lib():
     L.mi355nb_compute_set_alch.argtypes = [vp, vp, C.c_int, vp]
     L.mi355nb_compute_finish_batch.argtypes = [vp, vp, vp, vp, C.POINTER(TileListStat)]
     L.mi355nb_compute_launch_graph.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.mi355nb_compute_update_parameters.argtypes = [vp, vp, vp, vp]
+    fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.mi355nb_build_force_energy_tables.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, fp, fp]
+    L.mi355nb_compute_set_force_table.argtypes = [vp, C.c_int]
+    L.mi355nb_compute_force_matching_grad.argtypes = [vp, dp, C.POINTER(C.c_double), vp, vp, vp]
     _LIB = L
     return L
 
@@ -178,6 +185,15 @@ def alch_data(params, lam, lam2, elec_lambda_start=0.5, vdw_lambda_end=1.0, vdw_
     check(lib().mi355nb_alch_data_fill(C.byref(p), lam, lam2, elec_lambda_start, vdw_lambda_end, vdw_shift_coeff,
                                        int(decouple), C.byref(d)), "alch_data_fill")
     return d
+
+
+def force_energy_tables(n, cutoff, switchdist, ewaldcof):
+    """The library's 1/r force and energy tables: two (n, 4) float32 arrays."""
+    f = np.zeros(4 * n, np.float32)
+    e = np.zeros(4 * n, np.float32)
+    check(lib().mi355nb_build_force_energy_tables(int(n), cutoff, switchdist, ewaldcof, f, e),
+          "build_force_energy_tables")
+    return f.reshape(n, 4), e.reshape(n, 4)
 
 
 def tile_excl_to_namd(masks):

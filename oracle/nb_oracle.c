@@ -19,6 +19,20 @@
  *                          electrostatics (ComputeNonbondedBase2.h:367-372,
  *                          :497, :645-686), with the pair classes of
  *                          calcForceEnergyFEP (CudaComputeNonbondedKernel.cu:203-213).
+ *  oracle_nb_table         table-mode pair evaluation: the pair loop of
+ *                          oracle_nb_fp64 with forces and energies from given
+ *                          1/r float4 tables, linearly interpolated in FP64
+ *                          as calcForceEnergy does (CudaComputeNonbondedKernel.cu:99-161,
+ *                          CudaComputeNonbondedKernelHip.cu:170-220).
+ *  oracle_namd_gpu_tables  NAMD's 1/r-indexed float4 force/energy tables
+ *                          (CudaNonbondedTables.C:158-203, :263-300) sampled
+ *                          from the spline tables above at r = N/(i + 0.5).
+ *  oracle_nb_adjoint_fp64  FP64 gradient of a force-matching loss
+ *                          L = sum_i |F_i - T_i|^2 (F = fast + slow force) with
+ *                          respect to the LJ table entries A_ab, B_ab (ordered
+ *                          type pairs) and the scaled charges, given the
+ *                          adjoint vectors g_i = dL/dF_i; the pair terms are
+ *                          the derivatives of the oracle_nb_fp64 pair force.
  *  oracle_pairs_fp32       the canonical pair-membership rule the kernels use:
  *                          r2 = fma(dz,dz,fma(dy,dy,dx*dx)) in FP32 on exact
  *                          displacements, r2 < (float)rc^2; counts and
@@ -161,6 +175,34 @@ static void pair_fp64_alch(const PairConst* c, double r2, double qq, double A, d
   e[5] += le2 * slowE;
 }
 
+/* table-mode pair: columns (fast, -vdwB, vdwA, scor); e[0..2] = vdw, elec, slow */
+typedef struct {
+  const float* ft;
+  const float* et;
+  int N;
+} Tables;
+
+static void pair_table(const Tables* T, double r2, double qq, double A, double B, double* f, double* fs,
+                       double* e) {
+  const double x = (double)T->N / sqrt(r2) - 0.5;
+  const double fl = floor(x);
+  const double a = x - fl;
+  int i0 = (int)fl;
+  if (i0 < 0) i0 = 0;
+  if (i0 > T->N - 1) i0 = T->N - 1;
+  const int i1 = i0 + 1 < T->N ? i0 + 1 : T->N - 1;
+  double t[4], u[4];
+  for (int k = 0; k < 4; ++k) {
+    t[k] = T->ft[4 * i0 + k] + a * ((double)T->ft[4 * i1 + k] - T->ft[4 * i0 + k]);
+    u[k] = T->et[4 * i0 + k] + a * ((double)T->et[4 * i1 + k] - T->et[4 * i0 + k]);
+  }
+  *f = A * t[2] + B * t[1] + qq * t[0];
+  *fs = qq * t[3];
+  e[0] += A * u[2] + B * u[1];
+  e[1] += qq * u[0];
+  e[2] += qq * u[3];
+}
+
 /* cell list over an orthorhombic box; calls back for every i<j pair with r2 < cut2 */
 typedef struct {
   int n, nc[3];
@@ -231,8 +273,8 @@ static int cell_neighbours(const Cells* C, int gx, int gy, int gz, int* out) {
 static int nb_fp64_core(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
                         const double* A, const double* B, const int64_t* eptr, const int32_t* eidx,
                         const double L[3], double cutoff, double switchdist, double beta, const int8_t* part,
-                        const AlchF* al, double* f, double* fs, double* energies, int nenergies, double* virial,
-                        double* virialSlow, int64_t* npairs, int nthreads) {
+                        const AlchF* al, const Tables* tabs, double* f, double* fs, double* energies,
+                        int nenergies, double* virial, double* virialSlow, int64_t* npairs, int nthreads) {
   PairConst c;
   c.rc = cutoff;
   c.rc2 = cutoff * cutoff;
@@ -277,6 +319,8 @@ static int nb_fp64_core(int n, const double* pos, const double* q, const int32_t
           const int ab = type[i] * ntypes + type[j];
           if (part)
             pair_fp64_alch(&c, r2, q[i] * q[j], A[ab], B[ab], part[i], part[j], al, &ff, &ffs, &S[18]);
+          else if (tabs)
+            pair_table(tabs, r2, q[i] * q[j], A[ab], B[ab], &ff, &ffs, &S[18]);
           else
             pair_fp64(&c, r2, q[i] * q[j], A[ab], B[ab], &ff, &ffs, &S[18], &S[19], &S[20]);
           for (int u = 0; u < 3; ++u) {
@@ -323,7 +367,20 @@ int oracle_nb_fp64(int n, const double* pos, const double* q, const int32_t* typ
                    const double* B, const int64_t* eptr, const int32_t* eidx, const double L[3], double cutoff,
                    double switchdist, double beta, double* f, double* fs, double* energies, double* virial,
                    double* virialSlow, int64_t* npairs, int nthreads) {
-  return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, NULL, NULL, f, fs,
+  return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, NULL, NULL, NULL, f,
+                      fs, energies, 3, virial, virialSlow, npairs, nthreads);
+}
+
+/* force4/energy4: N float4 table entries (columns fast, -vdwB, vdwA, scor) */
+int oracle_nb_table(int n, const double* pos, const double* q, const int32_t* type, int ntypes, const double* A,
+                    const double* B, const int64_t* eptr, const int32_t* eidx, const double L[3], double cutoff,
+                    double switchdist, int N, const float* force4, const float* energy4, double* f, double* fs,
+                    double* energies, double* virial, double* virialSlow, int64_t* npairs, int nthreads) {
+  Tables T;
+  T.ft = force4;
+  T.et = energy4;
+  T.N = N;
+  return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, 0.0, NULL, NULL, &T, f, fs,
                       energies, 3, virial, virialSlow, npairs, nthreads);
 }
 
@@ -340,8 +397,101 @@ int oracle_nb_fp64_alch(int n, const double* pos, const double* q, const int32_t
   al.elec2Up = lam[4]; al.elec2Down = lam[5]; al.vdw2Up = lam[6]; al.vdw2Down = lam[7];
   al.shUp = lam[8]; al.shDown = lam[9]; al.sh2Up = lam[10]; al.sh2Down = lam[11];
   al.decouple = decouple;
-  return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, part, &al, f, fs,
-                      energies, 6, virial, virialSlow, npairs, nthreads);
+  return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, part, &al, NULL, f,
+                      fs, energies, 6, virial, virialSlow, npairs, nthreads);
+}
+
+/* ---- force-matching adjoint ------------------------------------------------
+ * Pair force f (F_i += f d, F_j -= f d, d = x_j - x_i) as in pair_fp64:
+ *   f = A cA(r) + B cB(r) + qq fe(r),
+ *   cA = r^-12 dS_r - 12 S r^-14,  cB = 6 S r^-8 - r^-6 dS_r,
+ *   fe = -G(beta r)/r^3,  G = erfc(a) + 2a/sqrt(pi) exp(-a^2)   (fast + slow)
+ * With s = (g_i - g_j) . d = dL/df for this pair:
+ *   dL/dA_{t_i t_j} += s cA, dL/dB_{t_i t_j} += s cB (ordered bins),
+ *   dL/dq_i += s q_j fe, dL/dq_j += s q_i fe (scaled charges).
+ */
+int oracle_nb_adjoint_fp64(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+                           const int64_t* eptr, const int32_t* eidx, const double L[3], double cutoff,
+                           double switchdist, double beta, const double* g, double* gA, double* gB, double* gq,
+                           int nthreads) {
+  PairConst c;
+  c.rc = cutoff;
+  c.rc2 = cutoff * cutoff;
+  c.rs2 = switchdist * switchdist;
+  c.c1 = (switchdist < cutoff) ? 1.0 / ((c.rc2 - c.rs2) * (c.rc2 - c.rs2) * (c.rc2 - c.rs2)) : 0.0;
+  c.rcinv = 1.0 / cutoff;
+  c.rc3inv = c.rcinv * c.rcinv * c.rcinv;
+  c.beta = beta;
+  Cells C;
+  cells_build(&C, n, pos, L, cutoff);
+  const int ncell = C.nc[0] * C.nc[1] * C.nc[2];
+  int nt = 1;
+#ifdef _OPENMP
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+  const size_t nb2 = (size_t)ntypes * ntypes;
+  double* bins = (double*)calloc((size_t)nt * 2 * nb2, sizeof(double));
+  double* gqt = (double*)calloc((size_t)nt * n, sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+#endif
+  for (int cell = 0; cell < ncell; ++cell) {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double* bA = bins + (size_t)tid * 2 * nb2;
+    double* bB = bA + nb2;
+    double* Q = gqt + (size_t)tid * n;
+    int nb[27];
+    const int gz = cell % C.nc[2], gy = (cell / C.nc[2]) % C.nc[1], gx = cell / (C.nc[1] * C.nc[2]);
+    const int m = cell_neighbours(&C, gx, gy, gz, nb);
+    for (int i = C.head[cell]; i >= 0; i = C.next[i]) {
+      for (int k = 0; k < m; ++k) {
+        for (int j = C.head[nb[k]]; j >= 0; j = C.next[j]) {
+          if (j <= i) continue;
+          double d[3];
+          for (int u = 0; u < 3; ++u) d[u] = min_image(pos[3 * j + u] - pos[3 * i + u], L[u]);
+          const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          if (!(r2 <= c.rc2)) continue;
+          if (is_excluded(eptr, eidx, i, j)) continue;
+          const double sp = (g[3 * i] - g[3 * j]) * d[0] + (g[3 * i + 1] - g[3 * j + 1]) * d[1] +
+                            (g[3 * i + 2] - g[3 * j + 2]) * d[2];
+          const double r = sqrt(r2), ri2 = 1.0 / r2, ri6 = ri2 * ri2 * ri2;
+          double S = 1.0, dS_r = 0.0;
+          if (r2 > c.rs2) {
+            const double dd = c.rc2 - r2;
+            S = dd * dd * (c.rc2 + 2.0 * r2 - 3.0 * c.rs2) * c.c1;
+            dS_r = 2.0 * c.c1 * (-2.0 * dd * (c.rc2 + 2.0 * r2 - 3.0 * c.rs2) + 2.0 * dd * dd);
+          }
+          const double cA = ri6 * ri6 * (dS_r - 12.0 * S * ri2);
+          const double cB = ri6 * (6.0 * S * ri2 - dS_r);
+          const double a = c.beta * r;
+          const double G = erfc(a) + TWO_OVER_SQRT_PI * a * exp(-a * a);
+          const double fe = -G * ri2 / r;
+          const size_t bin = (size_t)type[i] * ntypes + type[j];
+          bA[bin] += sp * cA;
+          bB[bin] += sp * cB;
+          Q[i] += sp * q[j] * fe;
+          Q[j] += sp * q[i] * fe;
+        }
+      }
+    }
+  }
+  memset(gA, 0, nb2 * sizeof(double));
+  memset(gB, 0, nb2 * sizeof(double));
+  memset(gq, 0, (size_t)n * sizeof(double));
+  for (int t = 0; t < nt; ++t) {
+    for (size_t k = 0; k < nb2; ++k) {
+      gA[k] += bins[(size_t)t * 2 * nb2 + k];
+      gB[k] += bins[(size_t)t * 2 * nb2 + nb2 + k];
+    }
+    for (int k = 0; k < n; ++k) gq[k] += gqt[(size_t)t * n + k];
+  }
+  free(bins);
+  free(gqt);
+  cells_free(&C);
+  return 0;
 }
 
 /* ---- canonical FP32 pair sets ------------------------------------------- */
@@ -806,6 +956,41 @@ int oracle_namd_tables(double cutoff, double switchOn, double ewaldcof, double l
   if (r2_table) memcpy(r2_table, T.r2_table, sizeof(double) * T.n);
   if (tables)
     for (int k = 0; k < 5; ++k) memcpy(tables + (size_t)k * 4 * T.n, T.t[k], sizeof(double) * 4 * T.n);
+  namd_tables_free(&T);
+  return 0;
+}
+
+/* buildForceAndEnergyTable for the nonbonded set (CudaNonbondedTables.C:158-203
+ * and :273-300): entry i (1 <= i < N) samples the splines at
+ * r2 = r^2 + r2_delta with r = N / (i + 0.5); zero beyond the cutoff; force
+ * column = prefac * 2 * d(spline)/d(r^2), energy column = prefac * spline;
+ * columns (x, y, z, w) = (fast, -vdwb, vdwa, scor); entry 0: zero force,
+ * energy of entry 1.  force4, energy4: 4*N floats. */
+int oracle_namd_gpu_tables(int N, double cutoff, double switchOn, double ewaldcof, float* force4, float* energy4) {
+  NamdTables T;
+  namd_tables_build(&T, cutoff, switchOn, ewaldcof, 0.0);
+  const int r2_delta_expc = 64 * (T.r2_delta_exp - 1023);
+  const double* src[4] = {T.t[0], T.t[4], T.t[3], T.t[1]};
+  const double prefac[4] = {1.0, -1.0, 1.0, 1.0};
+  memset(force4, 0, sizeof(float) * 4 * (size_t)N);
+  memset(energy4, 0, sizeof(float) * 4 * (size_t)N);
+  for (int i = 1; i < N; ++i) {
+    const double r = (double)N / ((double)i + 0.5);
+    if (r > cutoff) continue;
+    const double r2 = r * r + T.r2_delta;
+    uint64_t bits;
+    memcpy(&bits, &r2, sizeof bits);
+    const int ti = (int)(bits >> 46) + r2_delta_expc;
+    const double diffa = r2 - T.r2_table[ti];
+    for (int k = 0; k < 4; ++k) {
+      const double* t = src[k] + 4 * (size_t)ti;
+      const double grad = (3.0 * t[3] * diffa + 2.0 * t[2]) * diffa + t[1];
+      const double ener = t[0] + diffa * ((t[3] * diffa + t[2]) * diffa + t[1]);
+      force4[4 * (size_t)i + k] = (float)(prefac[k] * 2.0 * grad);
+      energy4[4 * (size_t)i + k] = (float)(prefac[k] * ener);
+    }
+  }
+  for (int k = 0; k < 4; ++k) energy4[k] = energy4[4 + k];
   namd_tables_free(&T);
   return 0;
 }

@@ -46,6 +46,11 @@ def lib():
         L.oracle_namd_tables.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int),
                                          C.c_void_p, C.c_void_p]
         L.oracle_namd_cpu.argtypes = L.oracle_nb_fp64.argtypes
+        L.oracle_nb_table.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, i64p, i32p, dp, C.c_double,
+                                      C.c_double, C.c_int, fp, fp, dp, dp, dp, dp, dp, i64p, C.c_int]
+        L.oracle_namd_gpu_tables.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, fp, fp]
+        L.oracle_nb_adjoint_fp64.argtypes = [C.c_int, dp, dp, i32p, C.c_int, i64p, i32p, dp, C.c_double,
+                                             C.c_double, C.c_double, dp, dp, dp, dp, C.c_int]
         i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
         L.oracle_nb_fp64_alch.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, i64p, i32p, dp, C.c_double,
                                           C.c_double, C.c_double, i8p, dp, C.c_int, dp, dp, dp, dp, dp, i64p,
@@ -104,6 +109,21 @@ def nb_fp64_alch(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff,
                 npairs=int(npairs[0]))
 
 
+def nb_adjoint_fp64(pos, q_scaled, vdw_type, ntypes, excl_ptr, excl_idx, box, cutoff, switchdist, beta, g,
+                    nthreads=0):
+    """FP64 force-matching adjoint: given g = dL/dF (n x 3, total force),
+    returns (dL/dA, dL/dB as ntypes x ntypes ordered bins, dL/dq_scaled)."""
+    n = len(q_scaled)
+    gA = np.zeros(ntypes * ntypes)
+    gB = np.zeros(ntypes * ntypes)
+    gq = np.zeros(n)
+    lib().oracle_nb_adjoint_fp64(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64),
+                                 _c(vdw_type, np.int32), int(ntypes), _c(excl_ptr, np.int64), _c(excl_idx, np.int32),
+                                 _c(box, np.float64), cutoff, switchdist, beta, _c(g, np.float64).reshape(-1), gA, gB,
+                                 gq, int(nthreads))
+    return gA.reshape(ntypes, ntypes), gB.reshape(ntypes, ntypes), gq
+
+
 def pairs_fp32(pos, excl_ptr, excl_idx, box, cutoff2f, plcutoff2f, want_pairs=False, maxpairs=None):
     """Canonical FP32 pair sets (see nb_oracle.c). Returns dict of counts, checksums, optional pairs."""
     n = pos.shape[0]
@@ -158,6 +178,35 @@ def namd_tables(cutoff, switchdist, ewaldcof, limitdist=0.0):
     t = np.zeros(5 * 4 * n.value)
     lib().oracle_namd_tables(cutoff, switchdist, ewaldcof, limitdist, C.byref(n), r2.ctypes.data, t.ctypes.data)
     return r2, t.reshape(5, n.value, 4)
+
+
+def nb_table(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff, switchdist, force4, energy4,
+             nthreads=0):
+    """Table-mode pair evaluation with the given (N, 4) float32 tables; outputs as nb_fp64."""
+    n = len(q_scaled)
+    ntypes = int(round(np.sqrt(np.asarray(A).size)))
+    ft = _c(force4, np.float32).reshape(-1)
+    et = _c(energy4, np.float32).reshape(-1)
+    f = np.zeros(3 * n)
+    fs = np.zeros(3 * n)
+    en = np.zeros(3)
+    vir = np.zeros(9)
+    virs = np.zeros(9)
+    npairs = np.zeros(1, np.int64)
+    lib().oracle_nb_table(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32),
+                          ntypes, _c(A, np.float64).reshape(-1), _c(B, np.float64).reshape(-1),
+                          _c(excl_ptr, np.int64), _c(excl_idx, np.int32), _c(box, np.float64), cutoff, switchdist,
+                          ft.size // 4, ft, et, f, fs, en, vir, virs, npairs, int(nthreads))
+    return dict(f=f.reshape(n, 3), fs=fs.reshape(n, 3), e_vdw=en[0], e_elec=en[1], e_slow=en[2],
+                virial=vir.reshape(3, 3), virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]))
+
+
+def namd_gpu_tables(n, cutoff, switchdist, ewaldcof):
+    """NAMD's float4 1/r force and energy tables (n entries each) from the spline restatement."""
+    f = np.zeros(4 * n, np.float32)
+    e = np.zeros(4 * n, np.float32)
+    lib().oracle_namd_gpu_tables(int(n), cutoff, switchdist, ewaldcof, f, e)
+    return f.reshape(n, 4), e.reshape(n, 4)
 
 
 def namd_cpu(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff, switchdist, beta, nthreads=0):

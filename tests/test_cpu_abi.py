@@ -67,7 +67,7 @@ def test_nonbonded_coef_matches_formulas():
     assert abs(c.e_2 - 1 / rc) < 1e-7
     assert abs(c.ewald_1 - 2 * beta / math.sqrt(math.pi)) < 1e-6
     assert c.slowScale == 1.0
-    assert L.lib().mi355nb_get_do_table(1, 1, 1) == 0
+    assert L.lib().mi355nb_get_do_table(1, 1, 1) == 1
 
 
 def test_lj_mix_closed_form():

@@ -1,0 +1,169 @@
+"""Force-matching adjoint through the C ABI: loss and dL/d(sigma, eps, q) on
+frames of the 23k-atom box (7,000 waters + 20 chains x 100 beads, 16 bead
+types) against the FP64 reference gradient (oracle_nb_adjoint_fp64 + the
+mixing chain rule; that reference is pinned by finite differences in
+test_cpu_adjoint_oracle.py), plus the torch.autograd wrapper."""
+import math
+
+import numpy as np
+import pytest
+
+from mi355nb import fitting
+from mi355nb import systems
+from oracle import oracle as orc
+
+from nbcheck import rel_rms
+
+COULOMB = 332.0636
+# End to end (FP32 forward forces + FP32 adjoint vs all-FP64): the residual
+# F - T enters squared, so the forward force error is amplified by |F| / |F - T|.
+GRAD_TOL = 1e-3   # relative L2 of each gradient block vs the FP64 reference
+LOSS_TOL = 2e-4
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def problem(torch_cuda):
+    torch = torch_cuda
+    base = systems.fit_23k()
+    frames = fitting.make_frames(base, n_frames=4, seed=0)
+    fm = fitting.ForceMatching(frames, stream=torch.cuda.current_stream().cuda_stream)
+    sig_t, eps_t, q_t = fitting.perturbed_parameters(base, seed=1)
+    fm.set_target(fm.forces(sig_t, eps_t, q_t))
+    yield base, frames, fm
+    fm.close()
+
+
+def _reference(base, frames, fm, sig, eps, q):
+    """FP64 loss and gradient on the positions the kernels see."""
+    p = base.params
+    beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+    A, B = fitting.lj_tables(sig, eps)
+    nt = sig.size
+    qs = q * math.sqrt(COULOMB)
+    pos_all = fm.compute.effective_positions()
+    T = fm.target.reshape(len(frames), base.natoms, 3)
+    loss = 0.0
+    gA = np.zeros((nt, nt))
+    gB = np.zeros((nt, nt))
+    gq = np.zeros(base.natoms)
+    for k, fr in enumerate(frames):
+        pos = pos_all[k * base.natoms:(k + 1) * base.natoms]
+        o = orc.nb_fp64(pos, qs, fr.vdw_type, A, B, fr.excl_ptr, fr.excl_idx, fr.box, p.cutoff, p.switchdist, beta)
+        d = o["f"] + o["fs"] - T[k]
+        loss += float((d * d).sum())
+        a, b, c = orc.nb_adjoint_fp64(pos, qs, fr.vdw_type, nt, fr.excl_ptr, fr.excl_idx, fr.box, p.cutoff,
+                                      p.switchdist, beta, 2.0 * d)
+        gA += a
+        gB += b
+        gq += c
+    ds, de = fitting.lj_param_grad(sig, eps, gA, gB)
+    return loss, ds, de, gq * math.sqrt(COULOMB)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
+
+
+def test_adjoint_kernel_matches_fp64_adjoint_on_same_residual(problem):
+    """The adjoint kernel alone: the FP64 adjoint is given the residual of the
+    GPU's own forces, g = 2 (F_gpu - T), so the forward force error cancels."""
+    base, frames, fm = problem
+    p = base.params
+    sig, eps, q = base.lj_sigma, base.lj_eps, base.charge
+    Fg = fm.forces(sig, eps, q)
+    loss, ds, de, dq = fm.loss_and_grad(sig, eps, q)
+    beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+    A, B = fitting.lj_tables(sig, eps)
+    nt = sig.size
+    qs = q * math.sqrt(COULOMB)
+    pos_all = fm.compute.effective_positions()
+    T = fm.target.reshape(len(frames), base.natoms, 3)
+    gA = np.zeros((nt, nt))
+    gB = np.zeros((nt, nt))
+    gq = np.zeros(base.natoms)
+    lsum = 0.0
+    for k, fr in enumerate(frames):
+        pos = pos_all[k * base.natoms:(k + 1) * base.natoms]
+        o = orc.nb_fp64(pos, qs, fr.vdw_type, A, B, fr.excl_ptr, fr.excl_idx, fr.box, p.cutoff, p.switchdist, beta)
+        ffe = rel_rms(Fg[k], o["f"] + o["fs"])
+        print(f"[adjoint] frame {k}: GPU total force vs FP64 rel RMS {ffe:.2e}", flush=True)
+        d = Fg[k] - T[k]
+        lsum += float((d * d).sum())
+        a, b, c = orc.nb_adjoint_fp64(pos, qs, fr.vdw_type, nt, fr.excl_ptr, fr.excl_idx, fr.box, p.cutoff,
+                                      p.switchdist, beta, 2.0 * d)
+        gA += a
+        gB += b
+        gq += c
+    rs, re = fitting.lj_param_grad(sig, eps, gA, gB)
+    rq = gq * math.sqrt(COULOMB)
+    print(f"[adjoint same-residual] loss rel {abs(loss - lsum) / lsum:.2e}  dsigma {_rel(ds, rs):.2e}  "
+          f"deps {_rel(de, re):.2e}  dq {_rel(dq, rq):.2e}", flush=True)
+    # two forward passes (float atomics in a different order) agree to ~1e-7
+    assert abs(loss - lsum) <= 1e-6 * lsum
+    assert _rel(ds, rs) < 1e-5 and _rel(de, re) < 1e-5 and _rel(dq, rq) < 1e-5
+
+
+def test_gradient_matches_fp64_reference(problem):
+    base, frames, fm = problem
+    sig, eps, q = base.lj_sigma, base.lj_eps, base.charge
+    print(f"[adjoint] {len(frames)} frames x {base.natoms} atoms", flush=True)
+    loss, ds, de, dq = fm.loss_and_grad(sig, eps, q)
+    rl, rs, re, rq = _reference(base, frames, fm, sig, eps, q)
+    print(f"[adjoint] loss gpu {loss:.10e} fp64 {rl:.10e} rel {abs(loss - rl) / rl:.2e}")
+    print(f"[adjoint] dL/dsigma rel {_rel(ds, rs):.2e}  dL/deps rel {_rel(de, re):.2e}  dL/dq rel {_rel(dq, rq):.2e}")
+    assert abs(loss - rl) <= LOSS_TOL * rl
+    assert _rel(ds, rs) < GRAD_TOL and _rel(de, re) < GRAD_TOL and _rel(dq, rq) < GRAD_TOL
+
+
+def test_gradient_consistent_with_gpu_loss_differences(problem):
+    """Central differences (h = 1e-4 relative) of the GPU loss itself for a
+    few parameters; the FP32 forces limit this check to about 1e-2."""
+    base, frames, fm = problem
+    sig, eps, q = base.lj_sigma.copy(), base.lj_eps.copy(), base.charge.copy()
+    _, ds, de, dq = fm.loss_and_grad(sig, eps, q)
+    checks = [("sigma", 0), ("sigma", 16), ("eps", 3), ("eps", 16)]
+    for name, k in checks:
+        arr = sig if name == "sigma" else eps
+        h = 1e-4 * arr[k]
+        ap, am = arr.copy(), arr.copy()
+        ap[k] += h
+        am[k] -= h
+        if name == "sigma":
+            lp = fm.loss_and_grad(ap, eps, q)[0]
+            lm = fm.loss_and_grad(am, eps, q)[0]
+            an = ds[k]
+        else:
+            lp = fm.loss_and_grad(sig, ap, q)[0]
+            lm = fm.loss_and_grad(sig, am, q)[0]
+            an = de[k]
+        fd = (lp - lm) / (2 * h)
+        print(f"[adjoint fd] d/d{name}[{k}] fd {fd:.6e} adjoint {an:.6e}")
+        assert abs(fd - an) <= 2e-2 * max(abs(fd), abs(an)), (name, k)
+
+
+def test_autograd_function_and_descent(problem, torch_cuda):
+    torch = torch_cuda
+    base, frames, fm = problem
+    FML = fitting.force_matching_loss()
+    sig = torch.tensor(base.lj_sigma, dtype=torch.float64, requires_grad=True)
+    eps = torch.tensor(base.lj_eps, dtype=torch.float64, requires_grad=True)
+    q = torch.tensor(base.charge, dtype=torch.float64, requires_grad=True)
+    loss = FML.apply(sig, eps, q, fm)
+    loss.backward()
+    l2, ds, de, dq = fm.loss_and_grad(base.lj_sigma, base.lj_eps, base.charge)
+    # separate GPU evaluations: float atomics make the last bits differ
+    assert abs(float(loss) - l2) <= 1e-6 * l2
+    assert _rel(sig.grad.numpy(), ds) < 1e-5 and _rel(eps.grad.numpy(), de) < 1e-5
+    # a few normalised gradient steps on sigma and eps lower the loss
+    losses = [float(loss)]
+    opt = torch.optim.Adam([sig, eps], lr=2e-3)
+    for _ in range(5):
+        opt.zero_grad()
+        l = FML.apply(sig, eps, q, fm)
+        l.backward()
+        opt.step()
+        losses.append(float(FML.apply(sig, eps, q, fm)))
+    print(f"[fit] losses {['%.4e' % v for v in losses]}")
+    assert losses[-1] < losses[0]
