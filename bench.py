@@ -5,7 +5,8 @@ One step = the hot path of one MD force evaluation on the 96,000-atom
 solvated box (26,000 TIP3P + 60 chains x 300 beads, 98 A cube, cutoff 12 A,
 switchdist 10 A, pairlistdist 13.5 A, PME 108^3 order 4):
 
-    PME fractional coordinates -> grid clear -> charge spread -> force gather
+    PME fractional coordinates (from the pair path's patch-ordered slots, as
+    NAMD feeds PME per patch) -> grid clear -> charge spread -> force gather
     -> nonbonded pair kernel (fast + slow forces) -> force transpose
 
 Pairlists are rebuilt (tile-list build + refinement in the pair kernel) every
@@ -87,11 +88,10 @@ def main():
     K = p.pme_grid
     order = p.pme_order
     n = s.natoms
-    pos_d = torch.from_numpy(np.ascontiguousarray(s.pos)).to(dev)
-    q_d = torch.from_numpy(np.ascontiguousarray(s.charge)).to(dev)
-    atoms = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    nslots = info["atomStorageSize"]
+    atoms = torch.empty((nslots, 4), dtype=torch.float32, device=dev)
     grid = torch.zeros(K[0] * K[1] * K[2], dtype=torch.float32, device=dev)
-    fpme = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    fpme = torch.empty((nslots, 4), dtype=torch.float32, device=dev)
 
     if args.calibrate_copy:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
@@ -105,7 +105,7 @@ def main():
 
     def step(k, timed=False):
         pl = (k % args.pairlist_every) == 0
-        mc.frac_coords(pos_d, q_d, recip, out=atoms, stream=sp)
+        c.pme_atoms(recip, out=atoms)
         grid.zero_()
         if timed:
             e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
@@ -228,7 +228,9 @@ def run_batch(args):
     from mi355nb import lib as L
     from mi355nb import systems
 
-    stream = torch.cuda.current_stream()
+    # graph capture needs a stream of its own (the null stream cannot be captured)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     sp = stream.cuda_stream
     log("building 64 batched systems")
     members = systems.batch_systems(64)

@@ -262,6 +262,16 @@ int mi355nb_compute_launch_graph(mi355nb_compute* c, int doEnergy, int doVirial,
  * launch counts since the last reset, split by pairlist / non-pairlist. */
 int mi355nb_compute_set_timing(mi355nb_compute* c, int enable);
 
+/* PME input in the pair path's slot order: fractional coordinates (the fold
+ * of mi355nb_pme_frac_coords) of patch centre + patch-centred position, and
+ * the scaled charge, for atomStorageSize slots into atoms (device, float4 per
+ * slot); padding slots get zero charge.  NAMD feeds PME per patch
+ * (ComputePmeCUDA, src/ComputePmeCUDA.C:173-330); patch order keeps each
+ * spread workgroup's atoms together.  Enqueued on the compute's stream;
+ * single system only. */
+int mi355nb_compute_pme_atoms(const mi355nb_compute* c, const double recip[9], const double origin[3],
+                              float* atoms);
+
 /* Table mode (doTable of CudaComputeNonbondedKernel::nonbondedForce,
  * src/CudaComputeNonbondedKernel.cu:99-161): later launches take LJ and
  * electrostatics from 4096-entry 1/r tables with linear interpolation
@@ -280,6 +290,10 @@ int mi355nb_compute_get_kernel_time(mi355nb_compute* c, double* msPairlist, int3
 int mi355nb_compute_get_info(const mi355nb_compute* c, mi355nb_layout_info* info);
 int mi355nb_compute_get_device_buffers(const mi355nb_compute* c, mi355nb_device_buffers* out);
 int mi355nb_compute_get_nonbonded_coef(const mi355nb_compute* c, mi355nb_nb_constants* out);
+
+/* Box edges the kernels use, 3 per system (numSystems*3 doubles): the given
+ * edges rounded to multiples of 2^-17 A. */
+int mi355nb_compute_get_system_boxes(const mi355nb_compute* c, double* boxes);
 
 /* Positions the kernels actually see, in global atom order (natoms*3
  * doubles): patch centre + patch-centred position rounded to 2^-17 A. */

@@ -101,6 +101,9 @@ struct mi355nb_compute {
   DevBuf<signed char> part;
   DevBuf<int4> redBlocks;
   DevBuf<int> redSysStart;
+  // PME atoms in slot order
+  DevBuf<int> slotPatch;
+  DevBuf<double> patchCentre;
   // NAMD table mode (optional)
   bool tableMode = false;
   int tableSize = 4096;   // FORCE_ENERGY_TABLE_SIZE of the reference
@@ -137,7 +140,7 @@ struct mi355nb_compute {
   void releaseAll() {
     dropGraphs();
     alch.release(); part.release(); redBlocks.release(); redSysStart.release();
-    forceTable.release(); energyTable.release();
+    forceTable.release(); energyTable.release(); slotPatch.release(); patchCentre.release();
     fmTarget.release(); fmLossPartials.release(); fmLoss.release(); fmGq.release(); fmBinPartials.release();
     fmBins.release(); fmG.release();
     xyzq.release(); force.release(); forceSlow.release();
@@ -415,6 +418,19 @@ static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomSt
                              c->stream));
   MI355NB_HIP(hipMemcpyAsync(c->redSysStart.ptr, c->redSysStartHost.data(), (nsys + 1) * sizeof(int),
                              hipMemcpyHostToDevice, c->stream));
+  {
+    std::vector<int> sp(n, 0);
+    for (int p = 0; p < (int)c->L.patches.size(); ++p) {
+      const PatchRecord& pr = c->L.patches[p];
+      const int padded = (pr.numAtoms + TILE - 1) / TILE * TILE;
+      for (int k = 0; k < padded; ++k) sp[pr.atomStart + k] = p;
+    }
+    MI355NB_HIP(c->slotPatch.reserve(n));
+    MI355NB_HIP(c->patchCentre.reserve(c->L.centre.size()));
+    MI355NB_HIP(hipMemcpy(c->slotPatch.ptr, sp.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    MI355NB_HIP(hipMemcpy(c->patchCentre.ptr, c->L.centre.data(), c->L.centre.size() * sizeof(double),
+                          hipMemcpyHostToDevice));
+  }
   MI355NB_HIP(launchFill(c->fsoa.cap, c->fsoa.ptr, 0.0f, c->stream));
   int rc = uploadSlots(c);
   if (rc) return rc;
@@ -847,6 +863,17 @@ int mi355nb_compute_finish_batch(mi355nb_compute* c, double* forceOut, double* f
   return MI355NB_OK;
 }
 
+int mi355nb_compute_pme_atoms(const mi355nb_compute* c, const double recip[9], const double origin[3],
+                              float* atoms) {
+  CHECK_ARG(c && recip && origin && atoms, "null argument");
+  CHECK_STATE(c->haveLayout, "assign_patches must precede pme_atoms");
+  CHECK_STATE(c->L.numSystems == 1, "pme_atoms serves a single system");
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(launchFracCoordsFromSlots(c->L.atomStorageSize, c->xyzq.ptr, c->slotPatch.ptr, c->patchCentre.ptr,
+                                        c->patches.ptr, recip, origin, reinterpret_cast<F4*>(atoms), c->stream));
+  return MI355NB_OK;
+}
+
 int mi355nb_compute_set_force_table(mi355nb_compute* c, int enable) {
   CHECK_ARG(c, "null compute");
   if (enable) CHECK_STATE(!c->alchOn, "table mode and alchemical partitions cannot be combined");
@@ -936,6 +963,13 @@ int mi355nb_compute_get_device_buffers(const mi355nb_compute* c, mi355nb_device_
 int mi355nb_compute_get_nonbonded_coef(const mi355nb_compute* c, mi355nb_nb_constants* out) {
   CHECK_ARG(c && out, "null argument");
   std::memcpy(out, &c->c, sizeof(c->c));
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_get_system_boxes(const mi355nb_compute* c, double* boxes) {
+  CHECK_ARG(c && boxes, "null argument");
+  CHECK_STATE(c->haveLayout, "no layout yet");
+  std::memcpy(boxes, c->L.sysBox.data(), c->L.sysBox.size() * sizeof(double));
   return MI355NB_OK;
 }
 

@@ -301,7 +301,7 @@ bool PatchLayout::buildOne(int n, const double* pos, const double boxIn[3], doub
       for (int gz = 0; gz < dims[2]; ++gz) {
         int p = (gx * dims[1] + gy) * dims[2] + gz;
         int g[3] = {gx, gy, gz};
-        for (int d = 0; d < 3; ++d) centre[3 * p + d] = quantize((g[d] + 0.5) * patchSize[d], CENTRE_QUANTUM);
+        for (int d = 0; d < 3; ++d) centre[3 * p + d] = quantize((g[d] + 0.5) * patchSize[d], POS_QUANTUM);
       }
 
   // assign atoms
@@ -366,10 +366,18 @@ bool PatchLayout::buildOne(int n, const double* pos, const double boxIn[3], doub
               ComputeRecord c;
               c.patchInd[0] = pa;
               c.patchInd[1] = pb;
-              // shift added to patch A atoms: c_A - c_B - T (exact in FP32)
-              c.offx = (float)quantize(centre[3 * pa + 0] - centre[3 * pb + 0] - T[0], CENTRE_QUANTUM);
-              c.offy = (float)quantize(centre[3 * pa + 1] - centre[3 * pb + 1] - T[1], CENTRE_QUANTUM);
-              c.offz = (float)quantize(centre[3 * pa + 2] - centre[3 * pb + 2] - T[2], CENTRE_QUANTUM);
+              // shift added to patch A atoms: c_A - c_B - T, a multiple of
+              // 2^-17 A below 128 A in magnitude, so exact in double and FP32
+              const double sh[3] = {centre[3 * pa + 0] - centre[3 * pb + 0] - T[0],
+                                    centre[3 * pa + 1] - centre[3 * pb + 1] - T[1],
+                                    centre[3 * pa + 2] - centre[3 * pb + 2] - T[2]};
+              c.offx = (float)sh[0];
+              c.offy = (float)sh[1];
+              c.offz = (float)sh[2];
+              if ((double)c.offx != sh[0] || (double)c.offy != sh[1] || (double)c.offz != sh[2]) {
+                err = "image shift not exact in FP32";
+                return false;
+              }
               c.self = (code == 0) ? 1 : 0;
               computes.push_back(c);
             }

@@ -11,13 +11,13 @@
 
 namespace mi355nb {
 
-// Patch-centred coordinates are rounded to multiples of POS_QUANTUM and patch
-// centres / image shifts to multiples of CENTRE_QUANTUM.  With |x| < 128 A
-// every sum and difference of such values is exact in FP32, so the kernel's
+// Box edges, patch centres, image shifts and patch-centred coordinates are all
+// multiples of POS_QUANTUM.  A shift c_A - c_B - T (T a multiple of the box)
+// is then exact, and with every |value| < 128 A each sum and difference of
+// such values is exact in FP32 (24 significant bits), so the kernel's
 // displacement (x_j - (x_i + shift)) is the exact displacement and the pair
 // membership test depends only on the rounding of r^2 itself.
 constexpr double POS_QUANTUM = 1.0 / 131072.0;     // 2^-17 A
-constexpr double CENTRE_QUANTUM = 1.0 / 1024.0;    // 2^-10 A
 constexpr double COULOMB = 332.0636;               // src/common.h:53
 
 double quantize(double x, double q);
@@ -82,7 +82,7 @@ struct PatchLayout {
   std::vector<int> sysListStart;    // numSystems + 1
   std::vector<double> sysBox;       // 3 per system, quantized
   std::vector<int> patchSystem;     // per patch
-  std::vector<double> centre;       // 3 per patch, multiples of CENTRE_QUANTUM
+  std::vector<double> centre;       // 3 per patch, multiples of POS_QUANTUM
 
   // records
   std::vector<PatchRecord> patches;

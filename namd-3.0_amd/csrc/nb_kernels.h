@@ -65,6 +65,11 @@ hipError_t launchReduceVirialEnergy(int numBlocks, int numSystems, const int4* b
                                     bool doEnergy, bool doVirial, bool doSlow, double* partials,
                                     VirialEnergy* out, hipStream_t s);
 
+// PME fractional coordinates from the pair path's slots (pme.hip)
+hipError_t launchFracCoordsFromSlots(int nslots, const F4* xyzq, const int* slotPatch, const double* centre,
+                                     const PatchRecord* patches, const double recip[9], const double origin[3],
+                                     F4* out, hipStream_t s);
+
 // Force-matching adjoint (adjoint.hip)
 struct AdjointArgsHost {
   int nslots;

@@ -120,10 +120,24 @@ __global__ void buildJtilesKernel(int numLists, TileList* __restrict__ lists,
 //    E_slow = qq erfc(a)/r - E_fast,  a = beta r
 // with the device library's erfcf, and accumulate per lane in FP64.
 // ---------------------------------------------------------------------------
-template <bool doEnergy>
-struct EnergyAcc { float v = 0.f; };
+// Per-lane sums: FP32 partial p over one j-tile sweep (16 pair terms),
+// folded into the FP64 total v once per j-tile (flush).  The disabled form
+// keeps the same members so code that does not run still compiles.
+template <bool on>
+struct EnergyAcc {
+  float p = 0.f;
+  float v = 0.f;
+  __device__ __forceinline__ void flush() {}
+};
 template <>
-struct EnergyAcc<true> { double v = 0.0; };
+struct EnergyAcc<true> {
+  float p = 0.f;
+  double v = 0.0;
+  __device__ __forceinline__ void flush() {
+    v += (double)p;
+    p = 0.f;
+  }
+};
 
 template <bool doEnergy, bool doSlow, typename Acc>
 __device__ __forceinline__ void pairForce(const float r2, const float qq, const float A, const float B,
@@ -159,9 +173,9 @@ __device__ __forceinline__ void pairForce(const float r2, const float qq, const 
     const float x = r * c.e_2;
     const float omx = 1.0f - x;
     const float fastE = omx * omx * (1.0f + 0.5f * x) * rinv;
-    eVdw.v += w * s;
-    eElec.v += qq * fastE;
-    if (doSlow) eSlow.v += qq * (erfcf(r * c.ewald_0) * rinv - fastE);
+    eVdw.p += w * s;
+    eElec.p += qq * fastE;
+    if (doSlow) eSlow.p += qq * (erfcf(r * c.ewald_0) * rinv - fastE);
   }
 }
 
@@ -228,13 +242,13 @@ __device__ __forceinline__ void pairForceAlch(const float r2, const float qq, co
     const float inv2 = __builtin_amdgcn_rcpf(r2 + sh2);
     const float j6 = inv2 * inv2 * inv2;
     const float U2 = j6 * (A * j6 - B);
-    eVdw.v += lv * (U * s);
-    eVdw2.v += lv2 * (U2 * s);
-    eElec.v += le * fastE;
-    eElec2.v += le2 * fastE;
+    eVdw.p += lv * (U * s);
+    eVdw2.p += lv2 * (U2 * s);
+    eElec.p += le * fastE;
+    eElec2.p += le2 * fastE;
     if (doSlow) {
-      eSlow.v += le * slowE;
-      eSlow2.v += le2 * slowE;
+      eSlow.p += le * slowE;
+      eSlow2.p += le2 * slowE;
     }
   }
 }
@@ -265,9 +279,9 @@ __device__ __forceinline__ void pairForceTable(const float r2, const float qq, c
   if (doSlow) fSlow = qq * (a * (t1.w - t0.w) + t0.w);
   if (doEnergy) {
     const F4 e0 = et[i0], e1 = et[i1];
-    eVdw.v += A * (a * (e1.z - e0.z) + e0.z) + B * (a * (e1.y - e0.y) + e0.y);
-    eElec.v += qq * (a * (e1.x - e0.x) + e0.x);
-    if (doSlow) eSlow.v += qq * (a * (e1.w - e0.w) + e0.w);
+    eVdw.p += A * (a * (e1.z - e0.z) + e0.z) + B * (a * (e1.y - e0.y) + e0.y);
+    eElec.p += qq * (a * (e1.x - e0.x) + e0.x);
+    if (doSlow) eSlow.p += qq * (a * (e1.w - e0.w) + e0.w);
   }
 }
 
@@ -479,20 +493,24 @@ nonbondedForceKernel(const ForceArgs args) {
           }
           const float fx = dx * f, fy = dy * f, fz = dz * f;
           ifx += fx; ify += fy; ifz += fz;
-          if (doVirial) { vfx.v += fx; vfy.v += fy; vfz.v += fz; }
+          if (doVirial) { vfx.p += fx; vfy.p += fy; vfz.p += fz; }
           F4 jf = sjf[j];
           jf.x -= fx; jf.y -= fy; jf.z -= fz;
           sjf[j] = jf;
           if (doSlow) {
             const float fsx = dx * fs, fsy = dy * fs, fsz = dz * fs;
             ifsx += fsx; ifsy += fsy; ifsz += fsz;
-            if (doVirial) { vfsx.v += fsx; vfsy.v += fsy; vfsz.v += fsz; }
+            if (doVirial) { vfsx.p += fsx; vfsy.p += fsy; vfsz.p += fsz; }
             F4 jfs = sjfs[j];
             jfs.x -= fsx; jfs.y -= fsy; jfs.z -= fsz;
             sjfs[j] = jfs;
           }
         }
       }
+      eVdw.flush(); eElec.flush(); eSlow.flush();
+      eVdw2.flush(); eElec2.flush(); eSlow2.flush();
+      vfx.flush(); vfy.flush(); vfz.flush();
+      vfsx.flush(); vfsy.flush(); vfsz.flush();
       waveSync();
 
       // j-forces: sum the two half-wave copies, coalesced atomics to SOA

@@ -46,8 +46,8 @@ struct PatchRecord {
 // CudaComputeRecord (src/CudaTileListKernel.h:41-45).  offsetXYZ here is the
 // Cartesian shift in Angstrom that is added to patchInd[0] atoms to bring them
 // into the frame of patchInd[1]; NAMD stores lattice units and multiplies by
-// the lattice vectors inside the kernel.  The shift is exact in FP32 (patch
-// centres are multiples of 2^-10 A).
+// the lattice vectors inside the kernel.  The shift is exact in FP32 (box,
+// patch centres and shifts are multiples of 2^-17 A).
 struct ComputeRecord {
   int patchInd[2];
   float offx, offy, offz;

@@ -15,6 +15,7 @@
 
 #include "../../include/mi355nb.h"
 #include "devutil.h"
+#include "nb_kernels.h"
 #include "nbtypes.h"
 
 namespace mi355nb {
@@ -77,6 +78,102 @@ spreadChargeKernel(const F4* __restrict__ atoms, int natoms, int K1, int K2, int
       int x = s_i[0][k] + x0; if (x >= K1) x -= K1;
       int y = s_i[1][k] + y0; if (y >= K2) y -= K2;
       int z = s_i[2][k] + z0; if (z >= K3) z -= K3;
+      const float v = s_t[0][k * ORDER + x0] * s_t[1][k * ORDER + y0] * s_t[2][k * ORDER + z0];
+      atomicAdd(&grid[x + (size_t)xdim * (y + (size_t)K2 * z)], v);
+    }
+  }
+}
+
+// Tiled spread.  A workgroup takes SP_ATOMS consecutive atoms, one per thread.
+// When their stencils fit a box of at most SP_TILE grid points (in grid
+// coordinates unwrapped around the group's first atom), the group accumulates
+// into that box in LDS (float atomics in LDS) and then adds each touched point
+// to the grid once, with consecutive threads on consecutive x: about half the
+// global atomics of the direct scheme for spatially grouped input (patch
+// order), and no two atomics of one group on the same address.  Otherwise
+// (scattered input) it spreads directly, ORDER^3 threads per atom.
+constexpr int SP_ATOMS = 128;
+constexpr int SP_TILE = 8192;
+
+template <int ORDER>
+__global__ void __launch_bounds__(SP_ATOMS)
+spreadTiledKernel(const F4* __restrict__ atoms, int natoms, int K1, int K2, int K3, int xdim,
+                  float* __restrict__ grid) {
+  __shared__ float tile[SP_TILE];
+  __shared__ int s_u[3][SP_ATOMS];
+  __shared__ float s_t[3][SP_ATOMS * ORDER];
+  __shared__ int s_lo[3], s_hi[3], s_ref[3];
+  const int tid = threadIdx.x;
+  const int base = blockIdx.x * SP_ATOMS;
+  const int na = min(SP_ATOMS, natoms - base);
+  const int a = base + tid;
+  const int K[3] = {K1, K2, K3};
+  if (tid < 3) {
+    s_lo[tid] = 0x7fffffff;
+    s_hi[tid] = -0x7fffffff;
+  }
+  // reference cell: the group's first atom
+  if (tid == 0) {
+    const F4 at = atoms[base];
+    const float fr[3] = {(float)K1 * at.x, (float)K2 * at.y, (float)K3 * at.z};
+    for (int d = 0; d < 3; ++d) s_ref[d] = min(max((int)fr[d], 0), K[d] - 1);
+  }
+  __syncthreads();
+  if (tid < na) {
+    const F4 at = atoms[a];
+    const float fr[3] = {(float)K1 * at.x, (float)K2 * at.y, (float)K3 * at.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int fi = min(max((int)fr[d], 0), K[d] - 1);
+      float th[ORDER];
+      bsplineWeights<ORDER>(fr[d] - (float)fi, th, nullptr);
+      // unwrap around the reference cell (periodic images within K/2)
+      int u = fi - s_ref[d];
+      if (2 * u > K[d]) u -= K[d];
+      if (2 * u < -K[d]) u += K[d];
+      s_u[d][tid] = u;
+      atomicMin(&s_lo[d], u);
+      atomicMax(&s_hi[d], u);
+#pragma unroll
+      for (int k = 0; k < ORDER; ++k) s_t[d][tid * ORDER + k] = (d == 0 ? at.w : 1.0f) * th[k];
+    }
+  }
+  __syncthreads();
+  const int e0 = s_hi[0] - s_lo[0] + ORDER, e1 = s_hi[1] - s_lo[1] + ORDER, e2 = s_hi[2] - s_lo[2] + ORDER;
+  constexpr int P = ORDER * ORDER * ORDER;
+  if ((long long)e0 * e1 * e2 <= SP_TILE) {
+    const int total = e0 * e1 * e2;
+    for (int p = tid; p < total; p += SP_ATOMS) tile[p] = 0.0f;
+    __syncthreads();
+    if (tid < na) {
+      const int ux = s_u[0][tid] - s_lo[0], uy = s_u[1][tid] - s_lo[1], uz = s_u[2][tid] - s_lo[2];
+      for (int z0 = 0; z0 < ORDER; ++z0)
+        for (int y0 = 0; y0 < ORDER; ++y0) {
+          const float tyz = s_t[1][tid * ORDER + y0] * s_t[2][tid * ORDER + z0];
+          float* row = tile + ((uz + z0) * e1 + (uy + y0)) * e0 + ux;
+#pragma unroll
+          for (int x0 = 0; x0 < ORDER; ++x0) atomicAdd(row + x0, s_t[0][tid * ORDER + x0] * tyz);
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < total; p += SP_ATOMS) {
+      const float v = tile[p];
+      if (v == 0.0f) continue;
+      int x = s_ref[0] + s_lo[0] + p % e0;
+      int y = s_ref[1] + s_lo[1] + (p / e0) % e1;
+      int z = s_ref[2] + s_lo[2] + p / (e0 * e1);
+      x %= K1; if (x < 0) x += K1;
+      y %= K2; if (y < 0) y += K2;
+      z %= K3; if (z < 0) z += K3;
+      atomicAdd(&grid[x + (size_t)xdim * (y + (size_t)K2 * z)], v);
+    }
+  } else {
+    for (int idx = tid; idx < na * P; idx += SP_ATOMS) {
+      const int k = idx / P, p = idx % P;
+      const int x0 = p % ORDER, y0 = (p / ORDER) % ORDER, z0 = p / (ORDER * ORDER);
+      int x = (s_ref[0] + s_u[0][k] + x0) % K1; if (x < 0) x += K1;
+      int y = (s_ref[1] + s_u[1][k] + y0) % K2; if (y < 0) y += K2;
+      int z = (s_ref[2] + s_u[2][k] + z0) % K3; if (z < 0) z += K3;
       const float v = s_t[0][k * ORDER + x0] * s_t[1][k * ORDER + y0] * s_t[2][k * ORDER + z0];
       atomicAdd(&grid[x + (size_t)xdim * (y + (size_t)K2 * z)], v);
     }
@@ -210,6 +307,46 @@ __global__ void fracCoordsKernel(const double* __restrict__ pos, const double* _
   out[a] = o;
 }
 
+// Fractional coordinates straight from the pair path's slots: position =
+// patch centre + patch-centred coordinate (exact), the same fold as
+// fracCoordsKernel; padding slots (copies of a patch's last atom) get zero
+// charge so they add nothing to the grid.
+__global__ void fracCoordsFromSlotsKernel(int nslots, const F4* __restrict__ xyzq, const int* __restrict__ slotPatch,
+                                          const double* __restrict__ centre, const PatchRecord* __restrict__ patches,
+                                          RecipD rc, double ox, double oy, double oz, F4* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nslots) return;
+  const int p = slotPatch[k];
+  const F4 a = xyzq[k];
+  const double px = centre[3 * p] + (double)a.x - ox;
+  const double py = centre[3 * p + 1] + (double)a.y - oy;
+  const double pz = centre[3 * p + 2] + (double)a.z - oz;
+  double w[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double r = px * rc.m[3 * d] + py * rc.m[3 * d + 1] + pz * rc.m[3 * d + 2];
+    w[d] = r - (floor(r + 0.5) - 0.5);
+  }
+  F4 o;
+  o.x = (float)w[0]; if (o.x >= 1.0f) o.x -= 1.0f;
+  o.y = (float)w[1]; if (o.y >= 1.0f) o.y -= 1.0f;
+  o.z = (float)w[2]; if (o.z >= 1.0f) o.z -= 1.0f;
+  const PatchRecord pr = patches[p];
+  o.w = (k - pr.atomStart < pr.numAtoms) ? a.w : 0.0f;
+  out[k] = o;
+}
+
+hipError_t launchFracCoordsFromSlots(int nslots, const F4* xyzq, const int* slotPatch, const double* centre,
+                                     const PatchRecord* patches, const double recip[9], const double origin[3],
+                                     F4* out, hipStream_t s) {
+  if (nslots <= 0) return hipSuccess;
+  RecipD r;
+  for (int k = 0; k < 9; ++k) r.m[k] = recip[k];
+  hipLaunchKernelGGL(fracCoordsFromSlotsKernel, dim3((nslots + 255) / 256), dim3(256), 0, s, nslots, xyzq, slotPatch,
+                     centre, patches, r, origin[0], origin[1], origin[2], out);
+  return hipGetLastError();
+}
+
 }  // namespace mi355nb
 
 using namespace mi355nb;
@@ -240,11 +377,11 @@ int mi355nb_spread_charge(const float* atoms, int natoms, int nfftx, int nffty, 
   if (natoms == 0) return MI355NB_OK;
   PME_CHECK(atoms && grid, "null pointer");
   hipStream_t s = (hipStream_t)stream;
-  dim3 g((natoms + 63) / 64), b(64);
   const F4* at = reinterpret_cast<const F4*>(atoms);
-  if (order == 4) hipLaunchKernelGGL(spreadChargeKernel<4>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid);
-  else if (order == 6) hipLaunchKernelGGL(spreadChargeKernel<6>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid);
-  else hipLaunchKernelGGL(spreadChargeKernel<8>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid);
+  dim3 g((natoms + SP_ATOMS - 1) / SP_ATOMS), b(SP_ATOMS);
+  if (order == 4) hipLaunchKernelGGL(spreadTiledKernel<4>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid);
+  else if (order == 6) hipLaunchKernelGGL(spreadTiledKernel<6>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid);
+  else hipLaunchKernelGGL(spreadTiledKernel<8>, g, b, 0, s, at, natoms, nfftx, nffty, nfftz, xdim, grid);
   MI355NB_HIP(hipGetLastError());
   return MI355NB_OK;
 }

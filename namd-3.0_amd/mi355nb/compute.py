@@ -214,6 +214,19 @@ class ComputeNonbonded:
     def set_timing(self, enable=True):
         L.check(L.lib().mi355nb_compute_set_timing(self._h, int(enable)), "set_timing")
 
+    def pme_atoms(self, recip, origin=(0.0, 0.0, 0.0), out=None):
+        """PME float4 atoms (fractional x, y, z, scaled q) in slot order, on the
+        compute's stream; padding slots have zero charge.  out: cuda float32
+        (atomStorageSize, 4) tensor."""
+        import torch
+        n = self.info()["atomStorageSize"]
+        if out is None:
+            out = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+        L.check(L.lib().mi355nb_compute_pme_atoms(self._h, np.ascontiguousarray(recip, np.float64).reshape(9),
+                                                  np.ascontiguousarray(origin, np.float64), out.data_ptr()),
+                "pme_atoms")
+        return out
+
     def set_force_table(self, enable=True):
         """NAMD's 1/r table mode (True) or direct math (False, the default)."""
         L.check(L.lib().mi355nb_compute_set_force_table(self._h, int(enable)), "set_force_table")
@@ -237,6 +250,13 @@ class ComputeNonbonded:
                     numComputes=i.numComputes, numCandidateLists=i.numCandidateLists,
                     patchDims=tuple(i.patchDims), jtileCapacity=i.jtileCapacity, box=tuple(i.box),
                     numSystems=i.numSystems)
+
+    def system_boxes(self):
+        """(numSystems, 3) box edges the kernels use (rounded to 2^-17 A)."""
+        n = self.info()["numSystems"]
+        b = np.zeros(3 * n)
+        L.check(L.lib().mi355nb_compute_get_system_boxes(self._h, b), "get_system_boxes")
+        return b.reshape(n, 3)
 
     def device_buffers(self):
         d = L.DeviceBuffers()

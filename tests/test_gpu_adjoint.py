@@ -43,6 +43,7 @@ def _reference(base, frames, fm, sig, eps, q):
     nt = sig.size
     qs = q * math.sqrt(COULOMB)
     pos_all = fm.compute.effective_positions()
+    boxes = fm.compute.system_boxes()
     T = fm.target.reshape(len(frames), base.natoms, 3)
     loss = 0.0
     gA = np.zeros((nt, nt))
@@ -50,10 +51,11 @@ def _reference(base, frames, fm, sig, eps, q):
     gq = np.zeros(base.natoms)
     for k, fr in enumerate(frames):
         pos = pos_all[k * base.natoms:(k + 1) * base.natoms]
-        o = orc.nb_fp64(pos, qs, fr.vdw_type, A, B, fr.excl_ptr, fr.excl_idx, fr.box, p.cutoff, p.switchdist, beta)
+        o = orc.nb_fp64(pos, qs, fr.vdw_type, A, B, fr.excl_ptr, fr.excl_idx, boxes[k], p.cutoff, p.switchdist,
+                        beta)
         d = o["f"] + o["fs"] - T[k]
         loss += float((d * d).sum())
-        a, b, c = orc.nb_adjoint_fp64(pos, qs, fr.vdw_type, nt, fr.excl_ptr, fr.excl_idx, fr.box, p.cutoff,
+        a, b, c = orc.nb_adjoint_fp64(pos, qs, fr.vdw_type, nt, fr.excl_ptr, fr.excl_idx, boxes[k], p.cutoff,
                                       p.switchdist, beta, 2.0 * d)
         gA += a
         gB += b
@@ -79,6 +81,7 @@ def test_adjoint_kernel_matches_fp64_adjoint_on_same_residual(problem):
     nt = sig.size
     qs = q * math.sqrt(COULOMB)
     pos_all = fm.compute.effective_positions()
+    boxes = fm.compute.system_boxes()
     T = fm.target.reshape(len(frames), base.natoms, 3)
     gA = np.zeros((nt, nt))
     gB = np.zeros((nt, nt))
@@ -86,12 +89,13 @@ def test_adjoint_kernel_matches_fp64_adjoint_on_same_residual(problem):
     lsum = 0.0
     for k, fr in enumerate(frames):
         pos = pos_all[k * base.natoms:(k + 1) * base.natoms]
-        o = orc.nb_fp64(pos, qs, fr.vdw_type, A, B, fr.excl_ptr, fr.excl_idx, fr.box, p.cutoff, p.switchdist, beta)
+        o = orc.nb_fp64(pos, qs, fr.vdw_type, A, B, fr.excl_ptr, fr.excl_idx, boxes[k], p.cutoff, p.switchdist,
+                        beta)
         ffe = rel_rms(Fg[k], o["f"] + o["fs"])
         print(f"[adjoint] frame {k}: GPU total force vs FP64 rel RMS {ffe:.2e}", flush=True)
         d = Fg[k] - T[k]
         lsum += float((d * d).sum())
-        a, b, c = orc.nb_adjoint_fp64(pos, qs, fr.vdw_type, nt, fr.excl_ptr, fr.excl_idx, fr.box, p.cutoff,
+        a, b, c = orc.nb_adjoint_fp64(pos, qs, fr.vdw_type, nt, fr.excl_ptr, fr.excl_idx, boxes[k], p.cutoff,
                                       p.switchdist, beta, 2.0 * d)
         gA += a
         gB += b

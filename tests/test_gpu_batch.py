@@ -31,17 +31,17 @@ def _alch_for(m, params, lam2_shift=0.05):
     return L.alch_data(params, m.lam, lam2, elec_lambda_start=0.5, vdw_lambda_end=1.0, vdw_shift_coeff=5.0)
 
 
-def _check_system(k, m, r, f, fs, pos, npairs_acc, alch=None):
+def _check_system(k, m, r, f, fs, pos, box, npairs_acc, alch=None):
     s = m.system
     p = s.params
     A, B = L.lj_mix(s.lj_sigma, s.lj_eps)
     beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
     q = s.charge * math.sqrt(COULOMB)
     if alch is None:
-        o = orc.nb_fp64(pos, q, s.vdw_type, A, B, s.excl_ptr, s.excl_idx, s.box, p.cutoff, p.switchdist, beta)
+        o = orc.nb_fp64(pos, q, s.vdw_type, A, B, s.excl_ptr, s.excl_idx, box, p.cutoff, p.switchdist, beta)
         keys = ("e_vdw", "e_elec", "e_slow")
     else:
-        o = orc.nb_fp64_alch(pos, q, s.vdw_type, A, B, s.excl_ptr, s.excl_idx, s.box, p.cutoff, p.switchdist, beta,
+        o = orc.nb_fp64_alch(pos, q, s.vdw_type, A, B, s.excl_ptr, s.excl_idx, box, p.cutoff, p.switchdist, beta,
                              m.partition, alch)
         keys = ("e_vdw", "e_elec", "e_slow", "e_vdw_s", "e_elec_s", "e_slow_s")
     npairs_acc.append(o["npairs"])
@@ -76,11 +76,12 @@ def test_batch_matches_oracle_per_system(torch_cuda, members, alch):
     c.launch_work(do_energy=True, do_virial=True, do_slow=True, do_pairlist=True)
     per, stat, f, fs = c.finish_batch()
     pos = c.effective_positions()
+    boxes = c.system_boxes()
     starts = c.sys_atom_start
     npairs = []
     for k, m in enumerate(members):
         a, b = starts[k], starts[k + 1]
-        _check_system(k, m, per[k], f[a:b], fs[a:b], pos[a:b], npairs,
+        _check_system(k, m, per[k], f[a:b], fs[a:b], pos[a:b], boxes[k], npairs,
                       alch=_alch_for(m, c.params) if alch else None)
     # the exact inclusive cutoff rule: the pair count over the batch matches
     assert stat["numPairsCutoff"] == sum(npairs)
