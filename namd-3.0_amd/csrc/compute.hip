@@ -615,20 +615,19 @@ static int launchImpl(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow
     a.tileLists = c->listsCand.ptr;
     a.tileJatomStart = c->jCand.ptr;
     a.tileExcls = c->exCand.ptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (timing) {
-      e0 = c->takeEvent(); e1 = c->takeEvent();
-      if (e0 && e1) MI355NB_HIP(hipEventRecord(e0, s));
-    }
-    MI355NB_HIP(launchNonbondedForce(nl, a, doEnergy, doVirial, doSlow, true, s));
-    if (timing && e0 && e1) {
-      MI355NB_HIP(hipEventRecord(e1, s));
-      c->evPairlist.emplace_back(e0, e1);
-    }
+  }
+  // the pairlist step is timed from the mask kernel to the end of its force launch
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timing) {
+    e0 = c->takeEvent(); e1 = c->takeEvent();
+    if (e0 && e1) MI355NB_HIP(hipEventRecord(e0, s));
+  }
+  if (doPairlist) {
+    MI355NB_HIP(launchBuildMasks(nl, a, s));
     // refine: depth-ordered, compacted lists
     MI355NB_HIP(launchReduceTileListStat(nl, c->listStat.ptr, c->stat.ptr, s));
     MI355NB_HIP(launchSetupSort(nl, c->listDepth.ptr, c->sortKeys.ptr, c->sortVals.ptr, s));
-    tb = c->tempBytes;
+    size_t tb = c->tempBytes;
     MI355NB_HIP(sortDescending(c->temp.ptr, tb, c->sortKeys.ptr, c->sortKeysOut.ptr, c->sortVals.ptr,
                                c->sortValsOut.ptr, nl, s));
     tb = c->tempBytes;
@@ -637,25 +636,19 @@ static int launchImpl(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow
                              c->listsCand.ptr, c->jCand.ptr, c->exCand.ptr, c->jtileFlags.ptr, c->listsAct.ptr,
                              c->jAct.ptr, c->exAct.ptr, s));
     c->haveLists = true;
-  } else {
-    // lists dropped by the last refinement write no record: clear them all
-    if (doEnergy || doVirial)
-      MI355NB_HIP(launchFill((size_t)nl * sizeof(TileListVirialEnergy) / sizeof(float),
-                             reinterpret_cast<float*>(c->tlve.ptr), 0.0f, s));
-    a.numLists = &c->stat.ptr->numTileLists;
-    a.tileLists = c->listsAct.ptr;
-    a.tileJatomStart = c->jAct.ptr;
-    a.tileExcls = c->exAct.ptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (timing) {
-      e0 = c->takeEvent(); e1 = c->takeEvent();
-      if (e0 && e1) MI355NB_HIP(hipEventRecord(e0, s));
-    }
-    MI355NB_HIP(launchNonbondedForce(nl, a, doEnergy, doVirial, doSlow, false, s));
-    if (timing && e0 && e1) {
-      MI355NB_HIP(hipEventRecord(e1, s));
-      c->evForce.emplace_back(e0, e1);
-    }
+  }
+  // lists dropped by the refinement write no record: clear them all
+  if (doEnergy || doVirial)
+    MI355NB_HIP(launchFill((size_t)nl * sizeof(TileListVirialEnergy) / sizeof(float),
+                           reinterpret_cast<float*>(c->tlve.ptr), 0.0f, s));
+  a.numLists = &c->stat.ptr->numTileLists;
+  a.tileLists = c->listsAct.ptr;
+  a.tileJatomStart = c->jAct.ptr;
+  a.tileExcls = c->exAct.ptr;
+  MI355NB_HIP(launchNonbondedForce(nl, a, doEnergy, doVirial, doSlow, s));
+  if (timing && e0 && e1) {
+    MI355NB_HIP(hipEventRecord(e1, s));
+    (doPairlist ? c->evPairlist : c->evForce).emplace_back(e0, e1);
   }
   MI355NB_HIP(launchTransposeForces(n, n, c->fsoa.ptr, c->force.ptr, c->forceSlow.ptr, doSlow != 0, s));
   if (doEnergy || doVirial) {

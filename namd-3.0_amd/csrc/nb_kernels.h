@@ -51,8 +51,11 @@ hipError_t launchRepack(int maxLists, const int* numActive, const int* sortedVal
                         const TileList* inLists, const int* inJatom, const TileExcl* inExcl,
                         const int* jtileFlags, TileList* outLists, int* outJatom, TileExcl* outExcl,
                         hipStream_t s);
-hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& a, bool doEnergy, bool doVirial,
-                                bool doSlow, bool doPairlist, hipStream_t s);
+// pairlist refinement: masks, j-tile flags, per-list depth and counts
+hipError_t launchBuildMasks(int maxLists, const ForceArgsHost& a, hipStream_t s);
+// the pair kernel on refined lists
+hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& a, bool doEnergy, bool doVirial, bool doSlow,
+                                hipStream_t s);
 hipError_t launchTransposeForces(int n, int stride, float* fsoa, F4* f, F4* fs, bool doSlow, hipStream_t s);
 hipError_t launchFill(size_t n, float* p, float v, hipStream_t s);
 hipError_t launchReduceTileListStat(int numLists, const int4* listStat, TileListStat* st, hipStream_t s);

@@ -343,14 +343,13 @@ constexpr int PAIR_DIRECT = 0;
 constexpr int PAIR_ALCH = 1;
 constexpr int PAIR_TABLE = 2;
 
-template <bool doEnergy, bool doVirial, bool doSlow, bool doPairlist, int mode>
+template <bool doEnergy, bool doVirial, bool doSlow, int mode>
 __global__ void __launch_bounds__(64 * NB_WAVES)
 nonbondedForceKernel(const ForceArgs args) {
   constexpr bool doAlch = mode == PAIR_ALCH;
   extern __shared__ float2 s_lj[];   // vdwWidth^2 (A, B)
   __shared__ F4 s_xyzq[NB_WAVES][TILE];
   __shared__ int s_type[NB_WAVES][TILE];     // vdw type | partition << 16 (alchemical)
-  __shared__ int s_index[NB_WAVES][TILE];
   __shared__ F4 s_jf[NB_WAVES][2][TILE];
   __shared__ F4 s_jfs[NB_WAVES][2][TILE];
 
@@ -366,7 +365,6 @@ nonbondedForceKernel(const ForceArgs args) {
   const float beta3 = args.c.ewald_2 * args.c.ewald_0;
   F4* const sx = s_xyzq[wv];
   int* const st = s_type[wv];
-  int* const sidx = s_index[wv];
   F4* const sjf = s_jf[wv][h];
   F4* const sjfs = s_jfs[wv][h];
 
@@ -392,27 +390,12 @@ nonbondedForceKernel(const ForceArgs args) {
     // FP64 copies of the list's i-force for the virial's shift term
     EnergyAcc<doVirial> vfx, vfy, vfz, vfsx, vfsy, vfsz;
 
-    // pairlist-step state
-    bool iValid = true;
-    int iIndex = 0;
-    I2 iExcl{0, 0};
-    int jEnd = 0;
-    int nPairsCut = 0, nPairsList = 0, nExcl = 0, depth = 0;
-    if (doPairlist) {
-      const PatchRecord pa = args.patches[tl.patchInd[0]];
-      const PatchRecord pb = args.patches[tl.patchInd[1]];
-      iValid = iatom < pa.atomStart + pa.numAtoms;
-      jEnd = pb.atomStart + pb.numAtoms;
-      iIndex = args.atomIndex[iatom];
-      iExcl = args.exclStartMaxdiff[iatom];
-    }
-
     // j-tile starts of the list, 64 at a time, one per lane (broadcast by shuffle)
     const int nJt = tl.jtileEnd - tl.jtileStart + 1;
     int myJ = (lane < nJt) ? args.tileJatomStart[tl.jtileStart + lane] : 0;
     // prefetch registers for the first j-tile (all lanes load; halves share lines)
     float nx = 0.f, ny = 0.f, nz = 0.f, nq = 0.f;
-    int nType = 0, nIdx = 0;
+    int nType = 0;
     unsigned nMask = 0u;
     if (nJt > 0) {
       const int ja0 = __shfl(myJ, 0, 64);
@@ -420,8 +403,7 @@ nonbondedForceKernel(const ForceArgs args) {
       nx = p.x; ny = p.y; nz = p.z; nq = p.w;
       nType = args.vdwTypes[ja0 + i];
       if (doAlch) nType |= (int)args.part[ja0 + i] << 16;
-      if (doPairlist) nIdx = args.atomIndex[ja0 + i];
-      if (!doPairlist) nMask = args.tileExcls[tl.jtileStart].excl[i];
+      nMask = args.tileExcls[tl.jtileStart].excl[i];
     }
 
     for (int k = 0; k < nJt; ++k) {
@@ -430,11 +412,10 @@ nonbondedForceKernel(const ForceArgs args) {
       if (h == 0) {
         sx[i] = F4{nx, ny, nz, nq};
         st[i] = nType;
-        if (doPairlist) sidx[i] = nIdx;
       }
       sjf[i] = F4{0.f, 0.f, 0.f, 0.f};
       if (doSlow) sjfs[i] = F4{0.f, 0.f, 0.f, 0.f};
-      unsigned mask = nMask;
+      const unsigned mask = nMask;
       // prefetch the next j-tile
       if (k + 1 < nJt) {
         if (((k + 1) & 63) == 0) myJ = (k + 1 + lane < nJt) ? args.tileJatomStart[jt + 1 + lane] : 0;
@@ -443,10 +424,8 @@ nonbondedForceKernel(const ForceArgs args) {
         nx = p.x; ny = p.y; nz = p.z; nq = p.w;
         nType = args.vdwTypes[jn + i];
         if (doAlch) nType |= (int)args.part[jn + i] << 16;
-        if (doPairlist) nIdx = args.atomIndex[jn + i];
-        if (!doPairlist) nMask = args.tileExcls[jt + 1].excl[i];
+        nMask = args.tileExcls[jt + 1].excl[i];
       }
-      const bool selfTile = doPairlist && tl.self && (jatomStart == tl.iatomStart);
       waveSync();
 
 #pragma unroll 2
@@ -457,25 +436,7 @@ nonbondedForceKernel(const ForceArgs args) {
         const float dy = xj.y - xi.y;
         const float dz = xj.z - xi.z;
         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        bool active;
-        if (doPairlist) {
-          bool inList = iValid && (jatomStart + j < jEnd) && (!selfTile || j > i) && within(dx, dy, dz, r2, args.plcut);
-          bool excluded = false;
-          if (inList) {
-            const int diff = sidx[j] - iIndex;
-            if (diff >= -iExcl.y && diff <= iExcl.y) {
-              const unsigned b = (unsigned)(iExcl.x + diff);
-              excluded = (args.exclBits[b >> 5] >> (b & 31)) & 1u;
-            }
-          }
-          if (inList && excluded) ++nExcl;
-          inList = inList && !excluded;
-          if (inList) { mask |= 1u << j; ++nPairsList; }
-          active = inList && within(dx, dy, dz, r2, args.cut);
-          if (active) ++nPairsCut;
-        } else {
-          active = ((mask >> j) & 1u) && within(dx, dy, dz, r2, args.cut);
-        }
+        const bool active = ((mask >> j) & 1u) && within(dx, dy, dz, r2, args.cut);
         if (active) {
           float f, fs = 0.f;
           if constexpr (mode == PAIR_ALCH) {
@@ -527,13 +488,6 @@ nonbondedForceKernel(const ForceArgs args) {
           atomicAdd(&args.fsoa[5 * args.stride + ja], c.z + d.z);
         }
       }
-      if (doPairlist) {
-        const unsigned full = mask | __shfl_down(mask, 32, 64);
-        const bool any = __any(full != 0u);
-        if (h == 0) args.tileExcls[jt].excl[i] = full;
-        if (lane == 0) args.jtileFlags[jt] = any ? 1 : 0;
-        depth += any ? 1 : 0;
-      }
       waveSync();
     }
 
@@ -557,15 +511,6 @@ nonbondedForceKernel(const ForceArgs args) {
       }
     }
 
-    if (doPairlist) {
-      const int a = waveSumInt(nPairsCut);
-      const int b = waveSumInt(nPairsList);
-      const int e = waveSumInt(nExcl);
-      if (lane == 0) {
-        args.listDepth[list] = depth;
-        args.listStat[list] = make_int4(depth, a, b, e);
-      }
-    }
     if (doVirial) {
       // per-list i-force sums over all 64 lanes, FP64
       const double sx2 = waveSumD(vfx.v), sy2 = waveSumD(vfy.v), sz2 = waveSumD(vfz.v);
@@ -592,6 +537,92 @@ nonbondedForceKernel(const ForceArgs args) {
         ve.energyVdw_s = a2; ve.energyElec_s = b2; ve.energySlow_s = e2;
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pairlist refinement (the doPairlist part of NAMD's pair kernel,
+// src/CudaComputeNonbondedKernel.cu:930-947, :1129-1149, :1449-1489): one
+// wave per candidate list, the pair kernel's lane / half-wave / rotation
+// layout, every slot of every candidate j-tile tested against pairlistdist
+// (exact inclusive rule) and the exclusion bit table.  Writes the 32-bit
+// masks, a flag per j-tile (any pair kept) and per list {depth, pairs within
+// the cutoff, pairs within pairlistdist, excluded pairs within
+// pairlistdist}.  Forces are not computed here: after the lists are
+// compacted and depth-ordered, the ordinary pair kernel runs on them.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64 * NB_WAVES) buildMasksKernel(const ForceArgs args) {
+  __shared__ F4 s_xyzq[NB_WAVES][TILE];
+  __shared__ int s_index[NB_WAVES][TILE];
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 31;
+  const int h = lane >> 5;
+  const int list = blockIdx.x * NB_WAVES + wv;
+  if (list >= *args.numLists) return;
+  F4* const sx = s_xyzq[wv];
+  int* const sidx = s_index[wv];
+  const TileList tl = args.tileLists[list];
+  const int iatom = tl.iatomStart + i;
+  F4 xi = args.xyzq[iatom];
+  xi.x += tl.offx;
+  xi.y += tl.offy;
+  xi.z += tl.offz;
+  const PatchRecord pa = args.patches[tl.patchInd[0]];
+  const PatchRecord pb = args.patches[tl.patchInd[1]];
+  const bool iValid = iatom < pa.atomStart + pa.numAtoms;
+  const int jEnd = pb.atomStart + pb.numAtoms;
+  const int iIndex = args.atomIndex[iatom];
+  const I2 iExcl = args.exclStartMaxdiff[iatom];
+  int nPairsCut = 0, nPairsList = 0, nExcl = 0, depth = 0;
+
+  for (int jt = tl.jtileStart; jt <= tl.jtileEnd; ++jt) {
+    const int jatomStart = args.tileJatomStart[jt];
+    if (h == 0) {
+      sx[i] = args.xyzq[jatomStart + i];
+      sidx[i] = args.atomIndex[jatomStart + i];
+    }
+    const bool selfTile = tl.self && (jatomStart == tl.iatomStart);
+    waveSync();
+    unsigned mask = 0u;
+#pragma unroll 2
+    for (int t = 0; t < 16; ++t) {
+      const int j = (i + t + 16 * h) & 31;
+      const F4 xj = sx[j];
+      const float dx = xj.x - xi.x;
+      const float dy = xj.y - xi.y;
+      const float dz = xj.z - xi.z;
+      const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+      bool inList = iValid && (jatomStart + j < jEnd) && (!selfTile || j > i) && within(dx, dy, dz, r2, args.plcut);
+      if (inList) {
+        const int diff = sidx[j] - iIndex;
+        if (diff >= -iExcl.y && diff <= iExcl.y) {
+          const unsigned b = (unsigned)(iExcl.x + diff);
+          if ((args.exclBits[b >> 5] >> (b & 31)) & 1u) {
+            ++nExcl;
+            inList = false;
+          }
+        }
+      }
+      if (inList) {
+        mask |= 1u << j;
+        ++nPairsList;
+        if (within(dx, dy, dz, r2, args.cut)) ++nPairsCut;
+      }
+    }
+    const unsigned full = mask | __shfl_down(mask, 32, 64);
+    const bool any = __any(full != 0u);
+    if (h == 0) args.tileExcls[jt].excl[i] = full;
+    if (lane == 0) args.jtileFlags[jt] = any ? 1 : 0;
+    depth += any ? 1 : 0;
+    waveSync();
+  }
+  const int a = waveSumInt(nPairsCut);
+  const int b = waveSumInt(nPairsList);
+  const int e = waveSumInt(nExcl);
+  if (lane == 0) {
+    args.listDepth[list] = depth;
+    args.listStat[list] = make_int4(depth, a, b, e);
   }
 }
 
@@ -862,9 +893,7 @@ hipError_t launchRepack(int maxLists, const int* numActive, const int* sortedVal
   return hipGetLastError();
 }
 
-hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEnergy, bool doVirial,
-                                bool doSlow, bool doPairlist, hipStream_t s) {
-  if (maxLists <= 0) return hipSuccess;
+static ForceArgs toDevice(const ForceArgsHost& h) {
   ForceArgs a;
   a.numLists = h.numLists;
   a.tileLists = h.tileLists;
@@ -892,9 +921,21 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
   a.forceTable = h.forceTable;
   a.energyTable = h.energyTable;
   a.tableSize = h.tableSize;
-  // persistent grid of wgPerCU workgroups per CU, or (wgPerCU == 0) one wave
-  // per list.  Pairlist steps walk the candidate lists in layout order, whose
-  // depths vary widely, so they always take one wave per list.
+  return a;
+}
+
+hipError_t launchBuildMasks(int maxLists, const ForceArgsHost& h, hipStream_t s) {
+  if (maxLists <= 0) return hipSuccess;
+  const ForceArgs a = toDevice(h);
+  hipLaunchKernelGGL(buildMasksKernel, dim3((maxLists + NB_WAVES - 1) / NB_WAVES), dim3(64 * NB_WAVES), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEnergy, bool doVirial, bool doSlow,
+                                hipStream_t s) {
+  if (maxLists <= 0) return hipSuccess;
+  const ForceArgs a = toDevice(h);
+  // persistent grid of wgPerCU workgroups per CU, or (wgPerCU == 0) one wave per list
   static int numCU = 0;
   if (numCU == 0) {
     int dev = 0;
@@ -903,21 +944,19 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
     numCU = prop.multiProcessorCount;
   }
   const int wgNeeded = (maxLists + NB_WAVES - 1) / NB_WAVES;
-  const int wgCap = (doPairlist || h.wgPerCU <= 0) ? wgNeeded : numCU * h.wgPerCU;
+  const int wgCap = h.wgPerCU <= 0 ? wgNeeded : numCU * h.wgPerCU;
   dim3 g(std::min(wgNeeded, wgCap)), b(64 * NB_WAVES);
   const size_t shmem = sizeof(float2) * (size_t)h.vdwWidth * h.vdwWidth;
   const int mode = h.alch ? PAIR_ALCH : (h.forceTable ? PAIR_TABLE : PAIR_DIRECT);
-  const int key = mode * 16 + (doEnergy ? 8 : 0) + (doVirial ? 4 : 0) + (doSlow ? 2 : 0) + (doPairlist ? 1 : 0);
+  const int key = mode * 8 + (doEnergy ? 4 : 0) + (doVirial ? 2 : 0) + (doSlow ? 1 : 0);
   switch (key) {
-#define NB_CASE(M, E, V, S, P)                                                               \
-  case (M * 16 + E * 8 + V * 4 + S * 2 + P):                                                 \
-    hipLaunchKernelGGL((nonbondedForceKernel<E, V, S, P, M>), g, b, shmem, s, a);            \
+#define NB_CASE(M, E, V, S)                                                                  \
+  case (M * 8 + E * 4 + V * 2 + S):                                                          \
+    hipLaunchKernelGGL((nonbondedForceKernel<E, V, S, M>), g, b, shmem, s, a);               \
     break;
-#define NB_MODE(M)                                                                                           \
-  NB_CASE(M, 0, 0, 0, 0) NB_CASE(M, 0, 0, 0, 1) NB_CASE(M, 0, 0, 1, 0) NB_CASE(M, 0, 0, 1, 1)               \
-  NB_CASE(M, 0, 1, 0, 0) NB_CASE(M, 0, 1, 0, 1) NB_CASE(M, 0, 1, 1, 0) NB_CASE(M, 0, 1, 1, 1)               \
-  NB_CASE(M, 1, 0, 0, 0) NB_CASE(M, 1, 0, 0, 1) NB_CASE(M, 1, 0, 1, 0) NB_CASE(M, 1, 0, 1, 1)               \
-  NB_CASE(M, 1, 1, 0, 0) NB_CASE(M, 1, 1, 0, 1) NB_CASE(M, 1, 1, 1, 0) NB_CASE(M, 1, 1, 1, 1)
+#define NB_MODE(M)                                                                           \
+  NB_CASE(M, 0, 0, 0) NB_CASE(M, 0, 0, 1) NB_CASE(M, 0, 1, 0) NB_CASE(M, 0, 1, 1)            \
+  NB_CASE(M, 1, 0, 0) NB_CASE(M, 1, 0, 1) NB_CASE(M, 1, 1, 0) NB_CASE(M, 1, 1, 1)
     NB_MODE(0) NB_MODE(1) NB_MODE(2)
 #undef NB_MODE
 #undef NB_CASE
