@@ -416,6 +416,10 @@ nonbondedForceKernel(const ForceArgs args) {
       sjf[i] = F4{0.f, 0.f, 0.f, 0.f};
       if (doSlow) sjfs[i] = F4{0.f, 0.f, 0.f, 0.f};
       const unsigned mask = nMask;
+      // alchemical kernels: visits with no alchemical atom on either side
+      // (the common case) take the plain pair math (wave-uniform branch)
+      bool visitAlch = false;
+      if (doAlch) visitAlch = __any(iPart != 0 || (nType >> 16) != 0);
       // prefetch the next j-tile
       if (k + 1 < nJt) {
         if (((k + 1) & 63) == 0) myJ = (k + 1 + lane < nJt) ? args.tileJatomStart[jt + 1 + lane] : 0;
@@ -442,8 +446,19 @@ nonbondedForceKernel(const ForceArgs args) {
           if constexpr (mode == PAIR_ALCH) {
             const int tw = st[j];
             const float2 ab = s_lj[vdwRow + (tw & 0xffff)];
-            pairForceAlch<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, iPart, tw >> 16, ap, f,
-                                            fs, eVdw, eElec, eSlow, eVdw2, eElec2, eSlow2);
+            if (visitAlch) {
+              pairForceAlch<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, iPart, tw >> 16, ap, f,
+                                              fs, eVdw, eElec, eSlow, eVdw2, eElec2, eSlow2);
+            } else {
+              // every pair here is (0,0): plain math; its lambda2 terms equal its lambda terms
+              EnergyAcc<doEnergy> tv, te, ts;
+              pairForce<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, tv, te, ts);
+              if (doEnergy) {
+                eVdw.p += tv.p; eVdw2.p += tv.p;
+                eElec.p += te.p; eElec2.p += te.p;
+                eSlow.p += ts.p; eSlow2.p += ts.p;
+              }
+            }
           } else if constexpr (mode == PAIR_TABLE) {
             const float2 ab = s_lj[vdwRow + st[j]];
             pairForceTable<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.forceTable, args.energyTable,
