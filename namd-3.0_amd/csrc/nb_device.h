@@ -23,4 +23,19 @@ __device__ __forceinline__ void waveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Loads a record that every lane of the wave reads at the same address and
+// moves it to scalar registers (v_readfirstlane per 32-bit word), so
+// wave-uniform data does not occupy vector registers.
+template <typename T>
+__device__ __forceinline__ T waveUniformLoad(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "32-bit words");
+  constexpr int W = sizeof(T) / 4;
+  const int* src = reinterpret_cast<const int*>(p);
+  T out;
+  int* dst = reinterpret_cast<int*>(&out);
+#pragma unroll
+  for (int k = 0; k < W; ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+  return out;
+}
+
 }  // namespace mi355nb

@@ -357,7 +357,7 @@ nonbondedForceKernel(const ForceArgs args) {
   for (int k = threadIdx.x; k < nlj; k += blockDim.x) s_lj[k] = args.vdwCoef[k];
   __syncthreads();
 
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int i = lane & 31;
   const int h = lane >> 5;
@@ -369,7 +369,7 @@ nonbondedForceKernel(const ForceArgs args) {
   F4* const sjfs = s_jfs[wv][h];
 
   for (int list = blockIdx.x * NB_WAVES + wv; list < numLists; list += gridDim.x * NB_WAVES) {
-    const TileList tl = args.tileLists[list];
+    const TileList tl = waveUniformLoad(args.tileLists + list);
     const int iatom = tl.iatomStart + i;
     F4 xi = args.xyzq[iatom];
     xi.x += tl.offx;
@@ -380,7 +380,7 @@ nonbondedForceKernel(const ForceArgs args) {
     AlchParams ap;
     if (doAlch) {
       iPart = args.part[iatom];
-      ap = args.alch[tl.sys];
+      ap = waveUniformLoad(args.alch + tl.sys);
     }
 
     float ifx = 0.f, ify = 0.f, ifz = 0.f;
