@@ -5,9 +5,13 @@ One step = the hot path of one MD force evaluation on the 96,000-atom
 solvated box (26,000 TIP3P + 60 chains x 300 beads, 98 A cube, cutoff 12 A,
 switchdist 10 A, pairlistdist 13.5 A, PME 108^3 order 4):
 
-    PME fractional coordinates (from the pair path's patch-ordered slots, as
-    NAMD feeds PME per patch) -> grid clear -> charge spread -> force gather
-    -> nonbonded pair kernel (fast + slow forces) -> force transpose
+    PME stream:       fractional coordinates (from the pair path's patch-
+                      ordered slots, as NAMD feeds PME per patch) -> grid
+                      clear -> charge spread -> force gather
+    nonbonded stream: pair kernel (fast + slow forces) -> force transpose
+
+The two streams run concurrently and join at the end of every step
+(--no-overlap puts both on one stream).
 
 Pairlists are rebuilt (tile-list build + refinement in the pair kernel) every
 20 steps and energies + virial are reduced on those steps too; the other
@@ -51,6 +55,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default pair-kernel launch shape")
+    ap.add_argument("--no-overlap", action="store_true", help="run PME and nonbonded work on one stream")
     ap.add_argument("--wg-per-cu", type=int, default=0,
                     help="fix the pair kernel's persistent workgroups per CU instead of autotuning")
     ap.add_argument("--workload", choices=("solvated96k", "batch64"), default="solvated96k",
@@ -75,7 +80,12 @@ def main():
         run_batch(args)
         return
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
+    # nonbonded on one stream, PME real space on another (they are independent
+    # within a step, as NAMD runs PME and nonbonded work on separate streams);
+    # the step ends when both are done
+    stream = torch.cuda.Stream()
+    pme_stream = torch.cuda.Stream() if not args.no_overlap else stream
+    torch.cuda.set_stream(stream)
     sp = stream.cuda_stream
 
     log("building solvated96k system")
@@ -102,22 +112,26 @@ def main():
         del src, dst
 
     ev = {}
+    pp = pme_stream.cuda_stream
 
     def step(k, timed=False):
         pl = (k % args.pairlist_every) == 0
-        c.pme_atoms(recip, out=atoms)
-        grid.zero_()
-        if timed:
-            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-            e[0].record(stream)
-        mc.spread_charge(atoms, grid, K, order, stream=sp)
-        if timed:
-            e[1].record(stream)
-        mc.gather_force(atoms, grid, K, order, recip=recip, out=fpme, stream=sp)
-        if timed:
-            e[2].record(stream)
-            ev.setdefault("pme", []).append(e)
+        pme_stream.wait_stream(stream)       # positions of this step
+        with torch.cuda.stream(pme_stream):
+            c.pme_atoms(recip, out=atoms, stream=pp)
+            grid.zero_()
+            if timed:
+                e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+                e[0].record(pme_stream)
+            mc.spread_charge(atoms, grid, K, order, stream=pp)
+            if timed:
+                e[1].record(pme_stream)
+            mc.gather_force(atoms, grid, K, order, recip=recip, out=fpme, stream=pp)
+            if timed:
+                e[2].record(pme_stream)
+                ev.setdefault("pme", []).append(e)
         c.launch_work(do_energy=pl, do_virial=pl, do_slow=True, do_pairlist=pl)
+        stream.wait_stream(pme_stream)       # the step's forces are complete
 
     log(f"warmup {args.warmup} steps")
     for k in range(max(1, args.warmup)):
@@ -128,9 +142,12 @@ def main():
         c.set_launch_config(args.wg_per_cu)
         tuned = {"workgroups_per_cu": args.wg_per_cu, "fixed": True}
     elif not args.no_autotune:
-        from mi355nb.autotune import tune_pair_kernel
-        best, table = tune_pair_kernel(c)
-        tuned = {"workgroups_per_cu": best, "ms_by_workgroups_per_cu": {str(k): round(v, 4) for k, v in table.items()}}
+        from mi355nb.autotune import tune_step
+        # force-only steps (k = 1 .. reps never hit a pairlist step at pairlist_every >= 20)
+        best, table = tune_step(c, step, torch.cuda.synchronize, time.perf_counter,
+                                reps=min(10, args.pairlist_every - 1))
+        tuned = {"workgroups_per_cu": best, "tuned_on": "whole step",
+                 "ms_per_step_by_workgroups_per_cu": {str(k): round(v, 4) for k, v in table.items()}}
         log(f"autotune: {tuned}")
         for k in range(max(1, args.warmup)):
             step(k)

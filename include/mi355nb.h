@@ -267,10 +267,11 @@ int mi355nb_compute_set_timing(mi355nb_compute* c, int enable);
  * the scaled charge, for atomStorageSize slots into atoms (device, float4 per
  * slot); padding slots get zero charge.  NAMD feeds PME per patch
  * (ComputePmeCUDA, src/ComputePmeCUDA.C:173-330); patch order keeps each
- * spread workgroup's atoms together.  Enqueued on the compute's stream;
- * single system only. */
+ * spread workgroup's atoms together.  Enqueued on `stream` (PME may run on a
+ * stream of its own, concurrently with the pair kernel); it reads the
+ * compute's positions, so order it after update_positions.  Single system. */
 int mi355nb_compute_pme_atoms(const mi355nb_compute* c, const double recip[9], const double origin[3],
-                              float* atoms);
+                              float* atoms, void* stream);
 
 /* Table mode (doTable of CudaComputeNonbondedKernel::nonbondedForce,
  * src/CudaComputeNonbondedKernel.cu:99-161): later launches take LJ and

@@ -857,13 +857,14 @@ int mi355nb_compute_finish_batch(mi355nb_compute* c, double* forceOut, double* f
 }
 
 int mi355nb_compute_pme_atoms(const mi355nb_compute* c, const double recip[9], const double origin[3],
-                              float* atoms) {
+                              float* atoms, void* stream) {
   CHECK_ARG(c && recip && origin && atoms, "null argument");
   CHECK_STATE(c->haveLayout, "assign_patches must precede pme_atoms");
   CHECK_STATE(c->L.numSystems == 1, "pme_atoms serves a single system");
   MI355NB_HIP(hipSetDevice(c->device));
   MI355NB_HIP(launchFracCoordsFromSlots(c->L.atomStorageSize, c->xyzq.ptr, c->slotPatch.ptr, c->patchCentre.ptr,
-                                        c->patches.ptr, recip, origin, reinterpret_cast<F4*>(atoms), c->stream));
+                                        c->patches.ptr, recip, origin, reinterpret_cast<F4*>(atoms),
+                                        (hipStream_t)stream));
   return MI355NB_OK;
 }
 

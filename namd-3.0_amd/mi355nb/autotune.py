@@ -28,3 +28,29 @@ def tune_pair_kernel(compute, candidates=DEFAULT_CANDIDATES, reps=8, do_slow=Tru
     best = min(results, key=results.get)
     compute.set_launch_config(best)
     return best, results
+
+
+STEP_CANDIDATES = (0, 4, 6, 8, 12, 16, 32)
+
+
+def tune_step(compute, step, sync, clock, candidates=STEP_CANDIDATES, warmup=3, reps=10):
+    """Launch shape chosen on the whole step: step(k) enqueues step k (the pair
+    kernel runs on non-pairlist steps k = 1 .. reps), sync() waits for the
+    device, clock() is a host timer.  With other work running concurrently
+    (PME on its own stream) a smaller persistent grid can leave room for it,
+    so the pair kernel alone does not decide.  Returns (best, {candidate: ms
+    per step}) and leaves the best one set."""
+    results = {}
+    for wg in candidates:
+        compute.set_launch_config(wg)
+        for k in range(1, warmup + 1):
+            step(k)
+        sync()
+        t0 = clock()
+        for k in range(1, reps + 1):
+            step(k)
+        sync()
+        results[wg] = (clock() - t0) * 1e3 / reps
+    best = min(results, key=results.get)
+    compute.set_launch_config(best)
+    return best, results

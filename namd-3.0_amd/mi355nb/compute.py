@@ -27,6 +27,7 @@ class ComputeNonbonded:
             ewaldcof = L.ewald_coefficient(cutoff, ewald_tol)
         self.params = L.NbParams(cutoff, switchdist, pairlistdist, ewaldcof, slow_scale, margin)
         self._h = C.c_void_p()
+        self.stream = stream
         L.check(L.lib().mi355nb_compute_create(C.byref(self.params), int(device), stream, C.byref(self._h)),
                 "compute_create")
         self.natoms = 0
@@ -214,16 +215,17 @@ class ComputeNonbonded:
     def set_timing(self, enable=True):
         L.check(L.lib().mi355nb_compute_set_timing(self._h, int(enable)), "set_timing")
 
-    def pme_atoms(self, recip, origin=(0.0, 0.0, 0.0), out=None):
-        """PME float4 atoms (fractional x, y, z, scaled q) in slot order, on the
-        compute's stream; padding slots have zero charge.  out: cuda float32
-        (atomStorageSize, 4) tensor."""
+    def pme_atoms(self, recip, origin=(0.0, 0.0, 0.0), out=None, stream=None):
+        """PME float4 atoms (fractional x, y, z, scaled q) in slot order;
+        padding slots have zero charge.  out: cuda float32 (atomStorageSize, 4)
+        tensor.  stream: torch stream or raw handle (default: the compute's)."""
         import torch
         n = self.info()["atomStorageSize"]
         if out is None:
             out = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+        st = self.stream if stream is None else _stream_ptr(stream)
         L.check(L.lib().mi355nb_compute_pme_atoms(self._h, np.ascontiguousarray(recip, np.float64).reshape(9),
-                                                  np.ascontiguousarray(origin, np.float64), out.data_ptr()),
+                                                  np.ascontiguousarray(origin, np.float64), out.data_ptr(), st),
                 "pme_atoms")
         return out
 
@@ -318,6 +320,8 @@ def _stream_ptr(stream):
     if stream is None:
         import torch
         return torch.cuda.current_stream().cuda_stream
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
     return stream
 
 

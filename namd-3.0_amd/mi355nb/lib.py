@@ -129,7 +129,7 @@ def lib():
     fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.mi355nb_build_force_energy_tables.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, fp, fp]
     L.mi355nb_compute_set_force_table.argtypes = [vp, C.c_int]
-    L.mi355nb_compute_pme_atoms.argtypes = [vp, dp, dp, vp]
+    L.mi355nb_compute_pme_atoms.argtypes = [vp, dp, dp, vp, vp]
     L.mi355nb_compute_get_system_boxes.argtypes = [vp, dp]
     L.mi355nb_compute_force_matching_grad.argtypes = [vp, dp, C.POINTER(C.c_double), vp, vp, vp]
     _LIB = L
