@@ -58,9 +58,11 @@ def main():
     ap.add_argument("--no-overlap", action="store_true", help="run PME and nonbonded work on one stream")
     ap.add_argument("--wg-per-cu", type=int, default=0,
                     help="fix the pair kernel's persistent workgroups per CU instead of autotuning")
-    ap.add_argument("--workload", choices=("solvated96k", "batch64"), default="solvated96k",
-                    help="solvated96k (default): the 96k-atom step; batch64: 64 uneven alchemical systems in one "
-                         "launch against the same systems run one at a time")
+    ap.add_argument("--workload", choices=("solvated96k", "solvated1m", "batch64", "fit23k"), default="solvated96k",
+                    help="solvated96k (default): the 96k-atom step; solvated1m: the same step on 1.07M atoms; "
+                         "batch64: 64 uneven alchemical systems in one launch against the same systems run one "
+                         "at a time; fit23k: force-matching loss + gradient on 16 frames of the 23k box")
+    ap.add_argument("--force-table", action="store_true", help="NAMD's 1/r table mode instead of direct math")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -79,6 +81,9 @@ def main():
     if args.workload == "batch64":
         run_batch(args)
         return
+    if args.workload == "fit23k":
+        run_fit(args)
+        return
     dev = torch.device("cuda:0")
     # nonbonded on one stream, PME real space on another (they are independent
     # within a step, as NAMD runs PME and nonbonded work on separate streams);
@@ -88,10 +93,13 @@ def main():
     torch.cuda.set_stream(stream)
     sp = stream.cuda_stream
 
-    log("building solvated96k system")
-    s = systems.solvated_96k()
+    big = args.workload == "solvated1m"
+    log(f"building {args.workload} system")
+    s = systems.solvated_1m() if big else systems.solvated_96k()
     p = s.params
     c = mc.ComputeNonbonded.from_system(s, stream=sp)
+    if args.force_table:
+        c.set_force_table(True)
     info = c.info()
     box = np.array(info["box"])
     recip = np.diag(1.0 / box)
@@ -183,7 +191,7 @@ def main():
     valu_tflops = FLOPS_PER_PAIR * npairs / (t_force_ms * 1e-3) / 1e12
 
     out = {
-        "metric": METRIC,
+        "metric": METRIC if not big else METRIC.replace("at 96k atoms", "at 1.07M atoms"),
         "value": round(ns_day, 2),
         "unit": "ns/day",
         "n_gpus": 1,
@@ -196,9 +204,12 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": "solvated96k: 26,000 TIP3P + 60 chains x 300 beads, 98 A cube, rc 12 A, switch 10 A, "
-                        "pairlist 13.5 A, PME 108^3 order 4; step = PME frac coords + spread + gather + "
-                        "nonbonded fast+slow forces, pairlist + energy/virial every 20 steps, dt 2 fs",
+            "workload": (("solvated96k: 26,000 TIP3P + 60 chains x 300 beads, 98 A cube" if not big else
+                          "solvated1m: 340,000 TIP3P + 150 chains x 300 beads, 216 A cube") +
+                         ", rc 12 A, switch 10 A, pairlist 13.5 A, PME " + "x".join(map(str, K)) +
+                         " order 4; step = PME frac coords + spread + gather (own stream) + nonbonded fast+slow "
+                         "forces, pairlist + energy/virial every 20 steps, dt 2 fs" +
+                         ("; 1/r table mode" if args.force_table else "")),
             "natoms": n,
             "pairlist_every": args.pairlist_every,
             "dt_fs": 2.0,
@@ -215,6 +226,9 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": None,
+            "traffic_measured": "rocprofv3 FETCH_SIZE/WRITE_SIZE of this kernel on the 96k box, scaled by a "
+                                "256 MiB copy: 224.6 MB per launch (46.6 MB read + 178.0 MB written, mostly "
+                                "force atomics); profiles/nonbonded_force_solvated96k.md",
             "kernel": "nonbondedForceKernel<doEnergy=0,doVirial=0,doSlow=1,doPairlist=0>",
             "bytes_model": "compulsory",
             "bytes_per_launch": int(bytes_compulsory),
@@ -346,6 +360,54 @@ def run_batch(args):
     for cs in singles:
         cs.close()
     c.close()
+
+
+def run_fit(args):
+    """Force matching (config E): 16 seeded frames of the 23k-atom box (7,000
+    waters + 20 chains x 100 beads, 16 bead types) in one batched compute;
+    one evaluation = forward fast + slow forces + residual + adjoint kernel
+    (dL/dA, dL/dB, dL/dq) + the mixing chain rule to dL/d(sigma, eps)."""
+    import torch
+    from mi355nb import fitting
+    from mi355nb import systems
+
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    log("building 16 frames of fit23k")
+    base = systems.fit_23k()
+    frames = fitting.make_frames(base, n_frames=16, seed=0)
+    fm = fitting.ForceMatching(frames, stream=stream.cuda_stream)
+    sig_t, eps_t, q_t = fitting.perturbed_parameters(base, seed=1)
+    fm.set_target(fm.forces(sig_t, eps_t, q_t))
+    sig, eps, q = base.lj_sigma, base.lj_eps, base.charge
+    for _ in range(max(1, args.warmup // 5)):
+        fm.loss_and_grad(sig, eps, q)
+    n = max(5, args.steps // 10)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        loss, ds, de, dq = fm.loss_and_grad(sig, eps, q)
+    t = (time.perf_counter() - t0) / n
+    fm.compute.set_timing(True)
+    for _ in range(5):
+        fm.compute.launch_work(False, False, True, False)
+    _, _, ms_f, n_f = fm.compute.kernel_time()
+    fm.compute.set_timing(False)
+    natoms = base.natoms * len(frames)
+    out = {
+        "metric": "force-matching loss + gradient dL/d(eps, sigma, q): frames per second",
+        "value": round(len(frames) / t, 2),
+        "unit": "frames/s",
+        "n_gpus": 1, "steps": n, "warmup": max(1, args.warmup // 5), "ms_per_step": round(t * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "fit23k: 16 frames (0.05 A seeded jitter) of 7,000 TIP3P + 20 chains x 100 beads "
+                               "(16 bead types), 61.3 A cube, rc 12 A; loss = sum |F - F_target|^2 with F_target "
+                               "from a seeded 5% perturbation of (eps, sigma, q); step = forward + adjoint "
+                               "+ host copies and chain rule", "natoms": natoms, "frames": len(frames)},
+        "forward_pair_kernel_ms": round(ms_f / max(n_f, 1), 4),
+        "loss": loss,
+    }
+    print(json.dumps(out), flush=True)
+    fm.close()
 
 
 def cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args, gpu_ms_step):
