@@ -411,7 +411,7 @@ def run_fit(args):
 
 
 def cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args, gpu_ms_step):
-    """The oracle timed on the host cores on the full 96k system (one force
+    """The oracle timed on the host cores on the full system (one force
     evaluation + one PME spread/gather), and the GPU force error against it."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as orc
@@ -457,7 +457,8 @@ def cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args,
             "unit": "ns/day",
             "cores": ncores,
             "kind": "port",
-            "sample": "one full evaluation of the 96k system: FP64 oracle nonbonded (all pairs < 12 A, cell list, "
+            "sample": f"one full evaluation of the {args.workload} system: FP64 oracle nonbonded (all pairs "
+                      "within 12 A, cell list, "
                       f"{ncores} OpenMP threads) + FP64 PME spread and gather (1 thread); "
                       f"{cpu_step:.2f} s per step",
         },
