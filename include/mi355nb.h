@@ -335,6 +335,48 @@ int mi355nb_self_energy(const float* atoms, int natoms, double ewaldcof, double*
 int mi355nb_pme_frac_coords(const double* pos, const double* charge, int natoms, const double recip[9],
                             const double origin[3], float* atoms, void* stream);
 
+/* ---- PME reciprocal space (CudaPmeOneDevice, src/CudaPmeSolverUtil.C:1296-1658) ----
+ * One object per grid shape: FFT plans (twiddle tables), the B-spline moduli
+ * d_bm1..3 and the half-complex work grid trans[K3][K2][K1/2+1] (float2),
+ * as CudaPmeOneDevice's constructor sets them up.  Grid sizes up to 1024 with
+ * prime factors 2, 3, 5, 7, 11 (every size NAMD's PMEGridSpacing rule picks,
+ * src/SimParameters.C:4190-4220).  Transforms are unnormalised (cuFFT's
+ * convention).  recip: reciprocal lattice vectors a*, b*, c* as rows
+ * (Lattice::a_r(), b_r(), c_r()); volume in A^3; kappa = ewaldcof.
+ * energyVirial: 7 device doubles, overwritten: {E_recip, v_xx, v_xy, v_xz,
+ * v_yy, v_yz, v_zz} (CudaPmeOneDevice::EnergyVirial,
+ * src/CudaPmeSolverUtil.h:247-250; self energy not included). */
+typedef struct mi355nb_pme_recip mi355nb_pme_recip;
+
+/* compute_b_moduli (src/PmeKSpace.C:21-61): bm[k] = 1/|sum_j M_n(j+1) e^{2 pi i jk/K}|^2, k < K. */
+int mi355nb_pme_b_moduli(int K, int order, double* bm);
+
+int mi355nb_pme_recip_create(int nfftx, int nffty, int nfftz, int order, mi355nb_pme_recip** out);
+int mi355nb_pme_recip_destroy(mi355nb_pme_recip* pr);
+
+/* cufftExecR2C of forwardPlan (src/CudaPmeSolverUtil.C:1584-1590): real grid
+ * data[x + xdim*(y + nffty*z)] -> trans[(z*nffty + y)*(nfftx/2+1) + k1]. */
+int mi355nb_pme_recip_fft_r2c(mi355nb_pme_recip* pr, const float* grid, int xdim, float* trans, void* stream);
+
+/* cufftExecC2R of backwardPlan (src/CudaPmeSolverUtil.C:1616-1620): trans -> real grid
+ * (trans is left unchanged; the object's own work grid is used). */
+int mi355nb_pme_recip_fft_c2r(mi355nb_pme_recip* pr, const float* trans, float* grid, int xdim, void* stream);
+
+/* scalar_sum (src/CudaPmeSolverUtilKernel.cu:73-320, :1434-1499, cX_Y_Z order):
+ * trans *= bm1 bm2 bm3 exp(-pi^2 m^2/kappa^2) / (pi V m^2), trans(0,0,0) = 0;
+ * with doEnergyVirial the energy and virial are written to energyVirial
+ * (FP64 throughout, deterministic reduction order). */
+int mi355nb_pme_recip_scalar_sum(mi355nb_pme_recip* pr, const double recip[9], double volume, double kappa,
+                                 float* trans, int doEnergyVirial, double* energyVirial, void* stream);
+
+/* The reciprocal solve of CudaPmeOneDevice::compute between spread_charge and
+ * gather_force (src/CudaPmeSolverUtil.C:1579-1620): R2C, scalar_sum, C2R, in
+ * place on grid (the charge grid becomes the convolved potential grid that
+ * mi355nb_gather_force reads).  The z transform, the k-space multiply and the
+ * inverse z transform run as one pass. */
+int mi355nb_pme_recip_solve(mi355nb_pme_recip* pr, const double recip[9], double volume, double kappa,
+                            float* grid, int xdim, int doEnergyVirial, double* energyVirial, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

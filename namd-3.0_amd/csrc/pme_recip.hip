@@ -1,0 +1,635 @@
+// PME reciprocal space for gfx950: the 3-D real-to-complex FFT of the charge
+// grid, the k-space multiply with energy and virial, and the complex-to-real
+// FFT back to the grid that gather_force reads.  NAMD's single-device path
+// (src/CudaPmeSolverUtil.C:1493-1658, CudaPmeOneDevice::compute) runs
+// cufftExecR2C, scalar_sum (src/CudaPmeSolverUtilKernel.cu:73-320) and
+// cufftExecC2R; here the FFT is written for the CU:
+//
+//   X pass    real lines of K1 points (contiguous) -> K1/2+1 complex points;
+//   Y pass    complex lines along y, a workgroup owns a tile of T consecutive
+//             x-columns of one z-plane, so every global access is a run of T
+//             consecutive complex values;
+//   Z pass    the same along z, fused with the k-space multiply and the
+//             inverse transform along z (forward, multiply, inverse without
+//             leaving LDS), with per-workgroup FP64 energy/virial partials;
+//   Y, X      inverse passes; the X pass writes the real grid.
+//
+// Each line is transformed by a mixed-radix Stockham FFT in LDS (radices 4,
+// 2, 3, 5, 7, 11: every grid size NAMD's PMEGridSpacing rule produces,
+// src/SimParameters.C:4190-4220), twiddles from an FP64-computed table.
+// Transforms are unnormalised in both directions, as cuFFT's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/mi355nb.h"
+#include "devutil.h"
+
+namespace mi355nb {
+
+constexpr int FFT_THREADS = 256;
+constexpr int FFT_MAX_N = 1024;
+constexpr int FFT_MAX_STAGES = 12;
+constexpr size_t FFT_LDS_BYTES = 60 * 1024;  // dynamic LDS per workgroup
+constexpr int EV_VALUES = 7;  // energy, virial xx xy xz yy yz zz
+
+struct FftAxis {
+  int n;                         // line length
+  int nst;                       // number of stages
+  int radix[FFT_MAX_STAGES];     // stage radices, first stage first
+  int twOff;                     // offset of this axis's twiddle table
+  int lines;                     // lines per workgroup (T)
+  int pitch;                     // LDS line pitch in complex elements (odd)
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mulI(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+// table entry t of exp(-2 pi i t / n), conjugated for the inverse transform
+template <bool INV>
+__device__ __forceinline__ float2 twiddle(const float2* tw, int t) {
+  const float2 w = tw[t];
+  return INV ? make_float2(w.x, -w.y) : w;
+}
+
+// In-place DFT of R points v[0..R-1]; roots from the n-point table (R | n).
+template <int R, bool INV>
+__device__ __forceinline__ void butterfly(float2* v, const float2* tw, int n) {
+  if constexpr (R == 4) {
+    const float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    const float2 c = cadd(v[1], v[3]), d = mulI<INV>(csub(v[1], v[3]));
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cadd(b, d);
+    v[3] = csub(b, d);
+  } else if constexpr (R == 2) {
+    const float2 a = v[0];
+    v[0] = cadd(a, v[1]);
+    v[1] = csub(a, v[1]);
+  } else if constexpr (R == 3) {
+    // y1,2 = a0 - (a1 + a2)/2 -/+ i (sqrt(3)/2)(a1 - a2)  (forward)
+    const float s = 0.86602540378443865f;
+    const float2 t1 = cadd(v[1], v[2]);
+    const float2 t2 = csub(v[1], v[2]);
+    const float2 m = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+    const float2 r = mulI<INV>(make_float2(s * t2.x, s * t2.y));
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  } else {
+    float2 y[R];
+    const int step = n / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) acc = cadd(acc, cmul(v[r], twiddle<INV>(tw, ((r * q) % R) * step)));
+      y[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = y[q];
+  }
+}
+
+// One Stockham stage of radix R after a span Ns (product of the earlier
+// radices) over T lines of N points (line l at a + l*P): element j of N/R
+// takes inputs j + r*N/R, twiddles exp(-+2 pi i (j mod Ns) r / (Ns R)), and
+// writes its outputs to (j / Ns) * Ns * R + (j mod Ns) + q * Ns.
+template <int R, bool INV>
+__device__ __forceinline__ void fftStage(const float2* a, float2* b, const float2* tw, int N, int P, int T, int Ns) {
+  const int NR = N / R;
+  const int step = N / (Ns * R);
+  for (int e = threadIdx.x; e < T * NR; e += FFT_THREADS) {
+    const int line = e / NR;
+    const int j = e - line * NR;
+    const int k = j % Ns;
+    const float2* in = a + line * P;
+    float2* out = b + line * P;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = in[j + r * NR];
+    if (k != 0) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<INV>(tw, k * r * step));
+    }
+    butterfly<R, INV>(v, tw, N);
+    const int o = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) out[o + q * Ns] = v[q];
+  }
+}
+
+// Stockham auto-sort FFT of ax.lines lines held in LDS; returns the buffer
+// holding the result (a or b).
+template <bool INV>
+__device__ float2* fftLines(float2* a, float2* b, const float2* tw, const FftAxis& ax) {
+  const int N = ax.n, P = ax.pitch, T = ax.lines;
+  int Ns = 1;
+  for (int s = 0; s < ax.nst; ++s) {
+    const int R = ax.radix[s];
+    switch (R) {
+      case 4: fftStage<4, INV>(a, b, tw, N, P, T, Ns); break;
+      case 2: fftStage<2, INV>(a, b, tw, N, P, T, Ns); break;
+      case 3: fftStage<3, INV>(a, b, tw, N, P, T, Ns); break;
+      case 5: fftStage<5, INV>(a, b, tw, N, P, T, Ns); break;
+      case 7: fftStage<7, INV>(a, b, tw, N, P, T, Ns); break;
+      default: fftStage<11, INV>(a, b, tw, N, P, T, Ns); break;
+    }
+    __syncthreads();
+    float2* t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+__device__ __forceinline__ void loadTwiddles(float2* s_tw, const float2* tw, const FftAxis& ax) {
+  for (int i = threadIdx.x; i < ax.n; i += FFT_THREADS) s_tw[i] = tw[ax.twOff + i];
+}
+
+// X pass, forward: real lines grid[(line)*xdim + x] -> trans[line*Nh + k].
+__global__ void __launch_bounds__(FFT_THREADS)
+fftXR2CKernel(const float* __restrict__ grid, int xdim, int nlines, FftAxis ax, const float2* __restrict__ tw,
+              float2* __restrict__ trans) {
+  extern __shared__ float2 lds[];
+  const int N = ax.n, P = ax.pitch, T = ax.lines, Nh = N / 2 + 1;
+  float2* s_tw = lds;
+  float2* A = lds + N;
+  float2* B = A + T * P;
+  loadTwiddles(s_tw, tw, ax);
+  const int line0 = blockIdx.x * T;
+  const int nl = min(T, nlines - line0);
+  for (int e = threadIdx.x; e < T * N; e += FFT_THREADS) {
+    const int l = e / N, x = e - l * N;
+    const float v = l < nl ? grid[(size_t)(line0 + l) * xdim + x] : 0.0f;
+    A[l * P + x] = make_float2(v, 0.0f);
+  }
+  __syncthreads();
+  const float2* res = fftLines<false>(A, B, s_tw, ax);
+  for (int e = threadIdx.x; e < nl * Nh; e += FFT_THREADS) {
+    const int l = e / Nh, k = e - l * Nh;
+    trans[(size_t)(line0 + l) * Nh + k] = res[l * P + k];
+  }
+}
+
+// X pass, inverse: Hermitian half lines trans[line*Nh + k] -> real grid.
+// The real part of the inverse transform of the Hermitian extension equals
+// cuFFT's C2R (which ignores the imaginary parts of the k = 0 and k = N/2
+// entries).
+__global__ void __launch_bounds__(FFT_THREADS)
+fftXC2RKernel(const float2* __restrict__ trans, int nlines, FftAxis ax, const float2* __restrict__ tw,
+              float* __restrict__ grid, int xdim) {
+  extern __shared__ float2 lds[];
+  const int N = ax.n, P = ax.pitch, T = ax.lines, Nh = N / 2 + 1;
+  float2* s_tw = lds;
+  float2* A = lds + N;
+  float2* B = A + T * P;
+  loadTwiddles(s_tw, tw, ax);
+  const int line0 = blockIdx.x * T;
+  const int nl = min(T, nlines - line0);
+  for (int e = threadIdx.x; e < T * Nh; e += FFT_THREADS) {
+    const int l = e / Nh, k = e - l * Nh;
+    const float2 v = l < nl ? trans[(size_t)(line0 + l) * Nh + k] : make_float2(0.0f, 0.0f);
+    A[l * P + k] = v;
+    if (k > 0 && k < N - k) A[l * P + N - k] = make_float2(v.x, -v.y);
+  }
+  __syncthreads();
+  const float2* res = fftLines<true>(A, B, s_tw, ax);
+  for (int e = threadIdx.x; e < nl * N; e += FFT_THREADS) {
+    const int l = e / N, x = e - l * N;
+    grid[(size_t)(line0 + l) * xdim + x] = res[l * P + x].x;
+  }
+}
+
+// k-space parameters of the multiply (NAMD's scalar_sum arguments).
+struct KSpace {
+  double recip[9];     // rows a*, b*, c* (Cartesian components)
+  double fac;          // pi^2 / beta^2
+  double pivInv;       // 1 / (pi V)
+  int K1, K2, K3;
+};
+
+// The k-space factor of one half-complex point; accumulates energy/virial
+// terms (before NAMD's final factors 1/2 and -1/2) when ev != nullptr and
+// returns the multiplier theta = bm1*bm2*bm3 * exp(-fac m^2) / (pi V m^2).
+// k1 is in [0, K1/2]; k2, k3 are unsigned grid indices.
+template <bool EV>
+__device__ __forceinline__ float kspaceFactor(const KSpace& ks, const double* __restrict__ bm, int k1, int k2, int k3,
+                                              float2 q, double* ev) {
+  if ((k1 | k2 | k3) == 0) return 0.0f;
+  const int k2s = k2 <= ks.K2 / 2 ? k2 : k2 - ks.K2;
+  const int k3s = k3 <= ks.K3 / 2 ? k3 : k3 - ks.K3;
+  const double m1 = ks.recip[0] * k1 + ks.recip[3] * k2s + ks.recip[6] * k3s;
+  const double m2 = ks.recip[1] * k1 + ks.recip[4] * k2s + ks.recip[7] * k3s;
+  const double m3 = ks.recip[2] * k1 + ks.recip[5] * k2s + ks.recip[8] * k3s;
+  const double msq = m1 * m1 + m2 * m2 + m3 * m3;
+  const double msqInv = 1.0 / msq;
+  const double theta3 = bm[k1] * bm[ks.K1 + k2] * bm[ks.K1 + ks.K2 + k3];
+  const double C = exp(-ks.fac * msq) * ks.pivInv * msqInv;
+  if (EV) {
+    // each stored point stands for itself and its conjugate image, except
+    // the k1 = 0 and k1 = K1/2 planes, which hold both
+    const bool self = (k1 == 0) || ((ks.K1 & 1) == 0 && k1 == ks.K1 / 2);
+    const double q2 = (self ? 1.0 : 2.0) * ((double)q.x * q.x + (double)q.y * q.y) * theta3;
+    const double f = q2 * C;
+    const double vir = 2.0 * (ks.fac + msqInv);
+    ev[0] += f;
+    ev[1] += f * (vir * m1 * m1 - 1.0);
+    ev[2] += f * vir * m1 * m2;
+    ev[3] += f * vir * m1 * m3;
+    ev[4] += f * (vir * m2 * m2 - 1.0);
+    ev[5] += f * vir * m2 * m3;
+    ev[6] += f * (vir * m3 * m3 - 1.0);
+  }
+  return (float)(theta3 * C);
+}
+
+// Workgroup sum of the 7 energy/virial values; thread 0 writes the
+// workgroup's partial record.
+__device__ void writeEvPartial(double* ev, double* __restrict__ partials, int wg) {
+  __shared__ double s_ev[FFT_THREADS / 64][EV_VALUES];
+#pragma unroll
+  for (int c = 0; c < EV_VALUES; ++c) {
+    double v = ev[c];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    ev[c] = v;
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int c = 0; c < EV_VALUES; ++c) s_ev[threadIdx.x >> 6][c] = ev[c];
+  __syncthreads();
+  if (threadIdx.x < EV_VALUES) {
+    double s = 0.0;
+    for (int w = 0; w < FFT_THREADS / 64; ++w) s += s_ev[w][threadIdx.x];
+    partials[(size_t)wg * EV_VALUES + threadIdx.x] = s;
+  }
+}
+
+// Strided pass along y (MODE 0/1: forward/inverse) or z (MODE 0/1, or 2 =
+// forward, k-space multiply, inverse).  Tile: x-columns x0 .. x0+T-1 of one
+// outer index; element (t, k) at data[outer*outerStride + x0 + t + k*stride].
+template <int MODE, bool EV>
+__global__ void __launch_bounds__(FFT_THREADS)
+fftStridedKernel(float2* __restrict__ data, int nx, int stride, int outerStride, FftAxis ax,
+                 const float2* __restrict__ tw, KSpace ks, const double* __restrict__ bm,
+                 double* __restrict__ partials) {
+  extern __shared__ float2 lds[];
+  const int N = ax.n, P = ax.pitch, T = ax.lines;
+  float2* s_tw = lds;
+  float2* A = lds + N;
+  float2* B = A + T * P;
+  loadTwiddles(s_tw, tw, ax);
+  const int x0 = blockIdx.x * T;
+  const int outer = blockIdx.y;
+  const int tc = min(T, nx - x0);
+  float2* base = data + (size_t)outer * outerStride + x0;
+  for (int e = threadIdx.x; e < T * N; e += FFT_THREADS) {
+    const int k = e / T, t = e - k * T;
+    A[t * P + k] = t < tc ? base[t + (size_t)k * stride] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  float2* res;
+  if (MODE == 1) {
+    res = fftLines<true>(A, B, s_tw, ax);
+  } else {
+    res = fftLines<false>(A, B, s_tw, ax);
+  }
+  if (MODE == 2) {
+    // res holds the full 3-D transform of the tile; k1 = x0 + t, k2 = outer, k3 = k
+    double ev[EV_VALUES] = {0, 0, 0, 0, 0, 0, 0};
+    for (int e = threadIdx.x; e < tc * N; e += FFT_THREADS) {
+      const int t = e / N, k = e - t * N;
+      const float2 q = res[t * P + k];
+      const float th = kspaceFactor<EV>(ks, bm, x0 + t, outer, k, q, ev);
+      res[t * P + k] = make_float2(q.x * th, q.y * th);
+    }
+    __syncthreads();
+    float2* other = (res == A) ? B : A;
+    res = fftLines<true>(res, other, s_tw, ax);
+    if (EV) writeEvPartial(ev, partials, blockIdx.y * gridDim.x + blockIdx.x);
+  }
+  for (int e = threadIdx.x; e < T * N; e += FFT_THREADS) {
+    const int k = e / T, t = e - k * T;
+    if (t < tc) base[t + (size_t)k * stride] = res[t * P + k];
+  }
+}
+
+// k-space multiply on a transformed grid trans[k3][k2][k1] (k1 < K1/2+1):
+// NAMD's scalar_sum with the cX_Y_Z permutation, grid-stride over points.
+template <bool EV>
+__global__ void __launch_bounds__(FFT_THREADS)
+scalarSumKernel(float2* __restrict__ data, KSpace ks, const double* __restrict__ bm, double* __restrict__ partials) {
+  const int K1h = ks.K1 / 2 + 1;
+  const size_t total = (size_t)K1h * ks.K2 * ks.K3;
+  double ev[EV_VALUES] = {0, 0, 0, 0, 0, 0, 0};
+  for (size_t p = (size_t)blockIdx.x * FFT_THREADS + threadIdx.x; p < total; p += (size_t)gridDim.x * FFT_THREADS) {
+    const int k1 = (int)(p % K1h);
+    const size_t r = p / K1h;
+    const int k2 = (int)(r % ks.K2);
+    const int k3 = (int)(r / ks.K2);
+    const float2 q = data[p];
+    const float th = kspaceFactor<EV>(ks, bm, k1, k2, k3, q, ev);
+    data[p] = make_float2(q.x * th, q.y * th);
+  }
+  if (EV) writeEvPartial(ev, partials, blockIdx.x);
+}
+
+// Ordered sum of the workgroup partials: energy * 1/2, virial * -1/2
+// (src/CudaPmeSolverUtilKernel.cu:305-316), written as {E, v_xx, v_xy, v_xz,
+// v_yy, v_yz, v_zz}.
+__global__ void evFinalKernel(int nparts, const double* __restrict__ partials, double* __restrict__ out) {
+  const int c = threadIdx.x;
+  if (c < EV_VALUES) {
+    double s = 0.0;
+    for (int p = 0; p < nparts; ++p) s += partials[(size_t)p * EV_VALUES + c];
+    out[c] = (c == 0 ? 0.5 : -0.5) * s;
+  }
+}
+
+// Host: order-n cardinal B-spline values at the integer points 1 .. n-1
+// (Essmann et al. 1995 recursion), NAMD's compute_b_spline at fraction 0
+// (src/PmeBase.inl:86-148) giving M[i] = M_n(i + 1).
+static void splineAtIntegers(int order, std::vector<double>& M) {
+  // M_2(u) = 1 - |u - 1| on [0, 2]: values at u = 1, 2
+  std::vector<double> cur = {1.0, 0.0};
+  for (int n = 3; n <= order; ++n) {
+    std::vector<double> nxt(n, 0.0);
+    // M_n(u) = (u M_{n-1}(u) + (n - u) M_{n-1}(u - 1)) / (n - 1) at u = i + 1
+    for (int i = 0; i < n; ++i) {
+      const double u = i + 1.0;
+      const double a = (i < (int)cur.size()) ? cur[i] : 0.0;       // M_{n-1}(u)
+      const double b = (i >= 1) ? cur[i - 1] : 0.0;                 // M_{n-1}(u - 1)
+      nxt[i] = (u * a + (n - u) * b) / (n - 1);
+    }
+    cur = nxt;
+  }
+  M.assign(cur.begin(), cur.begin() + order);
+}
+
+// bm[k] = 1 / |sum_j M[j] exp(2 pi i j k / K)|^2 (src/PmeKSpace.C:21-61).
+static void bModuli(int K, int order, double* bm) {
+  std::vector<double> M;
+  splineAtIntegers(order, M);
+  for (int k = 0; k < K; ++k) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int j = 0; j < order && j < K; ++j) {
+      const double arg = 2.0 * M_PI * (double)((long long)k * j % K) / K;
+      s1 += M[j] * std::cos(arg);
+      s2 += M[j] * std::sin(arg);
+    }
+    bm[k] = 1.0 / (s1 * s1 + s2 * s2);
+  }
+}
+
+static bool planAxis(int n, FftAxis& ax) {
+  if (n < 2 || n > FFT_MAX_N) return false;
+  ax.n = n;
+  ax.nst = 0;
+  int m = n;
+  for (int r : {4, 2, 3, 5, 7, 11}) {
+    while (m % r == 0) {
+      if (ax.nst == FFT_MAX_STAGES) return false;
+      ax.radix[ax.nst++] = r;
+      m /= r;
+    }
+  }
+  if (m != 1) return false;
+  ax.pitch = n | 1;
+  ax.lines = 16;
+  while (ax.lines > 1 && sizeof(float2) * ((size_t)n + 2 * (size_t)ax.lines * ax.pitch) > FFT_LDS_BYTES) ax.lines /= 2;
+  return true;
+}
+
+static size_t ldsBytes(const FftAxis& ax) { return sizeof(float2) * ((size_t)ax.n + 2 * (size_t)ax.lines * ax.pitch); }
+
+}  // namespace mi355nb
+
+using namespace mi355nb;
+
+struct mi355nb_pme_recip {
+  int K1, K2, K3, order;
+  FftAxis ax1, ax2, ax3;
+  float2* tw = nullptr;        // twiddle tables of the three axes
+  double* bm = nullptr;        // bm1[K1] | bm2[K2] | bm3[K3]
+  float2* trans = nullptr;     // K3 x K2 x (K1/2+1) complex
+  double* partials = nullptr;  // per-workgroup energy/virial records
+  int maxParts = 0;
+};
+
+namespace {
+
+#define RC_CHECK(cond, msg)         \
+  do {                              \
+    if (!(cond)) {                  \
+      setLastError(msg);            \
+      return MI355NB_ERR_ARG;       \
+    }                               \
+  } while (0)
+
+KSpace makeKSpace(const mi355nb_pme_recip* pr, const double* recip, double volume, double kappa) {
+  KSpace ks;
+  for (int k = 0; k < 9; ++k) ks.recip[k] = recip[k];
+  ks.fac = M_PI * M_PI / (kappa * kappa);
+  ks.pivInv = 1.0 / (M_PI * volume);
+  ks.K1 = pr->K1;
+  ks.K2 = pr->K2;
+  ks.K3 = pr->K3;
+  return ks;
+}
+
+dim3 stridedGrid(const mi355nb_pme_recip* pr, const FftAxis& ax, int outer) {
+  const int K1h = pr->K1 / 2 + 1;
+  return dim3((K1h + ax.lines - 1) / ax.lines, outer);
+}
+
+int launchXForward(mi355nb_pme_recip* pr, const float* grid, int xdim, hipStream_t s) {
+  const int nlines = pr->K2 * pr->K3;
+  hipLaunchKernelGGL(fftXR2CKernel, dim3((nlines + pr->ax1.lines - 1) / pr->ax1.lines), dim3(FFT_THREADS),
+                     ldsBytes(pr->ax1), s, grid, xdim, nlines, pr->ax1, pr->tw, pr->trans);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+int launchXInverse(mi355nb_pme_recip* pr, float* grid, int xdim, hipStream_t s) {
+  const int nlines = pr->K2 * pr->K3;
+  hipLaunchKernelGGL(fftXC2RKernel, dim3((nlines + pr->ax1.lines - 1) / pr->ax1.lines), dim3(FFT_THREADS),
+                     ldsBytes(pr->ax1), s, pr->trans, nlines, pr->ax1, pr->tw, grid, xdim);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+template <int MODE, bool EV>
+int launchY(mi355nb_pme_recip* pr, hipStream_t s) {
+  const int K1h = pr->K1 / 2 + 1;
+  KSpace ks{};
+  hipLaunchKernelGGL((fftStridedKernel<MODE, EV>), stridedGrid(pr, pr->ax2, pr->K3), dim3(FFT_THREADS),
+                     ldsBytes(pr->ax2), s, pr->trans, K1h, K1h, K1h * pr->K2, pr->ax2, pr->tw, ks, pr->bm,
+                     pr->partials);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+template <int MODE, bool EV>
+int launchZ(mi355nb_pme_recip* pr, const KSpace& ks, hipStream_t s) {
+  const int K1h = pr->K1 / 2 + 1;
+  hipLaunchKernelGGL((fftStridedKernel<MODE, EV>), stridedGrid(pr, pr->ax3, pr->K2), dim3(FFT_THREADS),
+                     ldsBytes(pr->ax3), s, pr->trans, K1h, K1h * pr->K2, K1h, pr->ax3, pr->tw, ks, pr->bm,
+                     pr->partials);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+int checkKSpace(const double* recip, double volume, double kappa) {
+  RC_CHECK(recip, "null recip");
+  RC_CHECK(volume > 0.0 && kappa > 0.0, "volume and kappa must be positive");
+  return MI355NB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mi355nb_pme_b_moduli(int K, int order, double* bm) {
+  RC_CHECK(bm && K >= order && order >= 3 && order <= 16, "bad b-moduli arguments");
+  bModuli(K, order, bm);
+  return MI355NB_OK;
+}
+
+int mi355nb_pme_recip_create(int K1, int K2, int K3, int order, mi355nb_pme_recip** out) {
+  RC_CHECK(out, "null output");
+  *out = nullptr;
+  RC_CHECK(order == 4 || order == 6 || order == 8, "order must be 4, 6 or 8");
+  RC_CHECK(K1 >= order && K2 >= order && K3 >= order, "grid smaller than the interpolation order");
+  auto* pr = new mi355nb_pme_recip();
+  pr->K1 = K1;
+  pr->K2 = K2;
+  pr->K3 = K3;
+  pr->order = order;
+  if (!planAxis(K1, pr->ax1) || !planAxis(K2, pr->ax2) || !planAxis(K3, pr->ax3)) {
+    delete pr;
+    setLastError("PME grid sizes must be at most 1024 with prime factors 2, 3, 5, 7, 11");
+    return MI355NB_ERR_ARG;
+  }
+  pr->ax1.twOff = 0;
+  pr->ax2.twOff = K1;
+  pr->ax3.twOff = K1 + K2;
+  std::vector<float2> tw(K1 + K2 + K3);
+  const int Ks[3] = {K1, K2, K3};
+  std::vector<double> bm(K1 + K2 + K3);
+  for (int a = 0, off = 0; a < 3; off += Ks[a], ++a) {
+    for (int t = 0; t < Ks[a]; ++t) {
+      const double ang = -2.0 * M_PI * (double)t / Ks[a];
+      tw[off + t] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    bModuli(Ks[a], order, bm.data() + off);
+  }
+  const int K1h = K1 / 2 + 1;
+  // partial records: the fused z pass has ceil(K1h/T3) x K2 workgroups; the
+  // standalone multiply uses at most 1024
+  pr->maxParts = std::max(1024, ((K1h + pr->ax3.lines - 1) / pr->ax3.lines) * K2);
+  hipError_t e = hipMalloc((void**)&pr->tw, tw.size() * sizeof(float2));
+  if (e == hipSuccess) e = hipMalloc((void**)&pr->bm, bm.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&pr->trans, (size_t)K1h * K2 * K3 * sizeof(float2));
+  if (e == hipSuccess) e = hipMalloc((void**)&pr->partials, (size_t)pr->maxParts * EV_VALUES * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(pr->tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(pr->bm, bm.data(), bm.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    setLastError(std::string("PME reciprocal allocation: ") + hipGetErrorString(e));
+    mi355nb_pme_recip_destroy(pr);
+    return -(int)e - 1000;
+  }
+  *out = pr;
+  return MI355NB_OK;
+}
+
+int mi355nb_pme_recip_destroy(mi355nb_pme_recip* pr) {
+  if (!pr) return MI355NB_OK;
+  if (pr->tw) (void)hipFree(pr->tw);
+  if (pr->bm) (void)hipFree(pr->bm);
+  if (pr->trans) (void)hipFree(pr->trans);
+  if (pr->partials) (void)hipFree(pr->partials);
+  delete pr;
+  return MI355NB_OK;
+}
+
+int mi355nb_pme_recip_fft_r2c(mi355nb_pme_recip* pr, const float* grid, int xdim, float* trans, void* stream) {
+  RC_CHECK(pr && grid && trans && xdim >= (pr ? pr->K1 : 0), "bad FFT arguments");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = launchXForward(pr, grid, xdim, s);
+  if (!rc) rc = launchY<0, false>(pr, s);
+  KSpace ks{};
+  if (!rc) rc = launchZ<0, false>(pr, ks, s);
+  if (rc) return rc;
+  const size_t bytes = (size_t)(pr->K1 / 2 + 1) * pr->K2 * pr->K3 * sizeof(float2);
+  if ((void*)trans != (void*)pr->trans) MI355NB_HIP(hipMemcpyAsync(trans, pr->trans, bytes, hipMemcpyDeviceToDevice, s));
+  return MI355NB_OK;
+}
+
+int mi355nb_pme_recip_fft_c2r(mi355nb_pme_recip* pr, const float* trans, float* grid, int xdim, void* stream) {
+  RC_CHECK(pr && grid && trans && xdim >= (pr ? pr->K1 : 0), "bad FFT arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t bytes = (size_t)(pr->K1 / 2 + 1) * pr->K2 * pr->K3 * sizeof(float2);
+  if ((const void*)trans != (void*)pr->trans)
+    MI355NB_HIP(hipMemcpyAsync(pr->trans, trans, bytes, hipMemcpyDeviceToDevice, s));
+  KSpace ks{};
+  int rc = launchZ<1, false>(pr, ks, s);
+  if (!rc) rc = launchY<1, false>(pr, s);
+  if (!rc) rc = launchXInverse(pr, grid, xdim, s);
+  return rc;
+}
+
+int mi355nb_pme_recip_scalar_sum(mi355nb_pme_recip* pr, const double recip[9], double volume, double kappa,
+                                 float* trans, int doEnergyVirial, double* energyVirial, void* stream) {
+  RC_CHECK(pr && trans, "bad scalar-sum arguments");
+  int rc = checkKSpace(recip, volume, kappa);
+  if (rc) return rc;
+  RC_CHECK(!doEnergyVirial || energyVirial, "energyVirial is null");
+  hipStream_t s = (hipStream_t)stream;
+  const KSpace ks = makeKSpace(pr, recip, volume, kappa);
+  const size_t total = (size_t)(pr->K1 / 2 + 1) * pr->K2 * pr->K3;
+  const int nb = (int)std::min<size_t>(1024, (total + FFT_THREADS - 1) / FFT_THREADS);
+  float2* d = reinterpret_cast<float2*>(trans);
+  if (doEnergyVirial) {
+    hipLaunchKernelGGL(scalarSumKernel<true>, dim3(nb), dim3(FFT_THREADS), 0, s, d, ks, pr->bm, pr->partials);
+    MI355NB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(evFinalKernel, dim3(1), dim3(64), 0, s, nb, pr->partials, energyVirial);
+  } else {
+    hipLaunchKernelGGL(scalarSumKernel<false>, dim3(nb), dim3(FFT_THREADS), 0, s, d, ks, pr->bm, pr->partials);
+  }
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+int mi355nb_pme_recip_solve(mi355nb_pme_recip* pr, const double recip[9], double volume, double kappa, float* grid,
+                            int xdim, int doEnergyVirial, double* energyVirial, void* stream) {
+  RC_CHECK(pr && grid && xdim >= (pr ? pr->K1 : 0), "bad solve arguments");
+  int rc = checkKSpace(recip, volume, kappa);
+  if (rc) return rc;
+  RC_CHECK(!doEnergyVirial || energyVirial, "energyVirial is null");
+  hipStream_t s = (hipStream_t)stream;
+  const KSpace ks = makeKSpace(pr, recip, volume, kappa);
+  rc = launchXForward(pr, grid, xdim, s);
+  if (!rc) rc = launchY<0, false>(pr, s);
+  if (!rc) rc = doEnergyVirial ? launchZ<2, true>(pr, ks, s) : launchZ<2, false>(pr, ks, s);
+  if (rc) return rc;
+  if (doEnergyVirial) {
+    const dim3 g = stridedGrid(pr, pr->ax3, pr->K2);
+    hipLaunchKernelGGL(evFinalKernel, dim3(1), dim3(64), 0, s, (int)(g.x * g.y), pr->partials, energyVirial);
+    MI355NB_HIP(hipGetLastError());
+  }
+  rc = launchY<1, false>(pr, s);
+  if (!rc) rc = launchXInverse(pr, grid, xdim, s);
+  return rc;
+}
+
+}  // extern "C"

@@ -363,3 +363,75 @@ def frac_coords(pos, charge, recip, origin=(0.0, 0.0, 0.0), out=None, stream=Non
                                             np.ascontiguousarray(origin, np.float64), out.data_ptr(),
                                             _stream_ptr(stream)), "pme_frac_coords")
     return out
+
+
+class PmeRecip:
+    """PME reciprocal space of one grid shape (K = (K1, K2, K3)): FFTs, the
+    k-space multiply with energy/virial, and the fused solve between
+    spread_charge and gather_force.  Device buffers are torch tensors on the
+    current device; recip holds a*, b*, c* as rows."""
+
+    def __init__(self, K, order=4):
+        self.K = tuple(int(k) for k in K)
+        self.order = int(order)
+        h = C.c_void_p()
+        L.check(L.lib().mi355nb_pme_recip_create(self.K[0], self.K[1], self.K[2], self.order, C.byref(h)),
+                "pme_recip_create")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            L.lib().mi355nb_pme_recip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def trans_shape(self):
+        K1, K2, K3 = self.K
+        return (K3, K2, K1 // 2 + 1)
+
+    def fft_r2c(self, grid, out=None, xdim=None, stream=None):
+        """grid: cuda float32 (K3*K2*xdim); returns complex64 (K3, K2, K1/2+1)."""
+        import torch
+        if out is None:
+            out = torch.empty(self.trans_shape(), dtype=torch.complex64, device=grid.device)
+        L.check(L.lib().mi355nb_pme_recip_fft_r2c(self._h, grid.data_ptr(), int(xdim or self.K[0]), out.data_ptr(),
+                                                  _stream_ptr(stream)), "pme_recip_fft_r2c")
+        return out
+
+    def fft_c2r(self, trans, grid=None, xdim=None, stream=None):
+        import torch
+        K1, K2, K3 = self.K
+        xd = int(xdim or K1)
+        if grid is None:
+            grid = torch.empty(K3 * K2 * xd, dtype=torch.float32, device=trans.device)
+        L.check(L.lib().mi355nb_pme_recip_fft_c2r(self._h, trans.data_ptr(), grid.data_ptr(), xd,
+                                                  _stream_ptr(stream)), "pme_recip_fft_c2r")
+        return grid
+
+    def _ev(self, device):
+        import torch
+        return torch.zeros(7, dtype=torch.float64, device=device)
+
+    def scalar_sum(self, trans, recip, volume, kappa, energy_virial=True, stream=None):
+        """In place on trans; returns the 7-double device tensor {E, vxx, vxy, vxz, vyy, vyz, vzz} or None."""
+        ev = self._ev(trans.device) if energy_virial else None
+        L.check(L.lib().mi355nb_pme_recip_scalar_sum(self._h, np.ascontiguousarray(recip, np.float64).reshape(9),
+                                                     float(volume), float(kappa), trans.data_ptr(),
+                                                     int(energy_virial), ev.data_ptr() if ev is not None else None,
+                                                     _stream_ptr(stream)), "pme_recip_scalar_sum")
+        return ev
+
+    def solve(self, grid, recip, volume, kappa, energy_virial=True, xdim=None, out=None, stream=None):
+        """In place on grid (charges -> convolved potential); returns the energy/virial tensor or None."""
+        ev = (out if out is not None else self._ev(grid.device)) if energy_virial else None
+        L.check(L.lib().mi355nb_pme_recip_solve(self._h, np.ascontiguousarray(recip, np.float64).reshape(9),
+                                                float(volume), float(kappa), grid.data_ptr(),
+                                                int(xdim or self.K[0]), int(energy_virial),
+                                                ev.data_ptr() if ev is not None else None, _stream_ptr(stream)),
+                "pme_recip_solve")
+        return ev

@@ -73,7 +73,9 @@ EXPORTED = [
     "mi355nb_compute_set_alch", "mi355nb_compute_finish_batch", "mi355nb_compute_launch_graph",
     "mi355nb_compute_update_parameters", "mi355nb_compute_force_matching_grad",
     "mi355nb_build_force_energy_tables", "mi355nb_compute_set_force_table", "mi355nb_compute_pme_atoms",
-    "mi355nb_compute_get_system_boxes",
+    "mi355nb_compute_get_system_boxes", "mi355nb_pme_b_moduli", "mi355nb_pme_recip_create",
+    "mi355nb_pme_recip_destroy", "mi355nb_pme_recip_fft_r2c", "mi355nb_pme_recip_fft_c2r",
+    "mi355nb_pme_recip_scalar_sum", "mi355nb_pme_recip_solve",
 ]
 
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -132,6 +134,13 @@ def lib():
     L.mi355nb_compute_pme_atoms.argtypes = [vp, dp, dp, vp, vp]
     L.mi355nb_compute_get_system_boxes.argtypes = [vp, dp]
     L.mi355nb_compute_force_matching_grad.argtypes = [vp, dp, C.POINTER(C.c_double), vp, vp, vp]
+    L.mi355nb_pme_b_moduli.argtypes = [C.c_int, C.c_int, dp]
+    L.mi355nb_pme_recip_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+    L.mi355nb_pme_recip_destroy.argtypes = [vp]
+    L.mi355nb_pme_recip_fft_r2c.argtypes = [vp, vp, C.c_int, vp, vp]
+    L.mi355nb_pme_recip_fft_c2r.argtypes = [vp, vp, vp, C.c_int, vp]
+    L.mi355nb_pme_recip_scalar_sum.argtypes = [vp, dp, C.c_double, C.c_double, vp, C.c_int, vp, vp]
+    L.mi355nb_pme_recip_solve.argtypes = [vp, dp, C.c_double, C.c_double, vp, C.c_int, C.c_int, vp, vp]
     _LIB = L
     return L
 
@@ -204,3 +213,10 @@ def tile_excl_to_namd(masks):
     out = np.zeros_like(m)
     check(lib().mi355nb_tile_excl_to_namd(m.reshape(-1), out.reshape(-1), m.shape[0]), "tile_excl_to_namd")
     return out
+
+
+def pme_b_moduli(K, order):
+    """B-spline moduli bm[k] (length K) of the PME reciprocal sum."""
+    bm = np.zeros(int(K))
+    check(lib().mi355nb_pme_b_moduli(int(K), int(order), bm), "pme_b_moduli")
+    return bm

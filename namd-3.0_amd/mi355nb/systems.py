@@ -388,6 +388,25 @@ def small_box(n_waters: int = 216, n_chains: int = 2, chain_len: int = 20, box: 
     return make_system(f"small{n_waters}w{n_chains}c", n_waters, n_chains, chain_len, box, n_bead_types, seed, p)
 
 
+MADELUNG_NACL = 1.747564594633182  # rock salt, per ion pair, in units of q^2 / nearest-neighbour distance
+
+
+def nacl_crystal(nside: int = 8, spacing: float = 4.0, ewald_tol: float = 1e-8, pme_grid=(48, 48, 48),
+                 pme_order: int = 8) -> System:
+    """Rock-salt lattice of +-1 e ions (nside^3 sites, nside even, nearest
+    neighbour `spacing` A), no Lennard-Jones, no exclusions: its total Ewald
+    energy is -(N/2) * MADELUNG_NACL * COULOMB / spacing."""
+    assert nside % 2 == 0
+    idx = np.stack(np.meshgrid(*[np.arange(nside)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    n = idx.shape[0]
+    L = nside * spacing
+    p = NbParams(pme_grid=tuple(pme_grid), pme_order=pme_order, ewald_tol=ewald_tol)
+    return System(name=f"nacl{nside}", pos=idx * spacing, charge=np.where(idx.sum(1) % 2 == 0, 1.0, -1.0),
+                  vdw_type=np.zeros(n, np.int32), lj_sigma=np.array([1.0]), lj_eps=np.array([0.0]),
+                  excl_ptr=np.zeros(n + 1, np.int64), excl_idx=np.zeros(0, np.int32), box=np.array([L, L, L]),
+                  params=p)
+
+
 def ewald_coefficient(cutoff: float, tol: float) -> float:
     """PMEEwaldCoefficient: bisection on erfc(beta*rc)/rc < tol (SimParameters.C:4243-4256)."""
     from math import erfc

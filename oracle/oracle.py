@@ -288,3 +288,122 @@ def bspline_gpu(w, order):
     dth = np.zeros(order)
     lib().oracle_bspline_gpu(float(w), order, th, dth)
     return th, dth
+
+
+# ---- PME reciprocal space (FP64, numpy's FFT) --------------------------------
+
+def pme_spline_at_integers(order):
+    """NAMD's compute_b_spline at fraction 0 (src/PmeBase.inl:86-125): the
+    order-n cardinal B-spline at u = 1 .. n, by its own recursion (start at
+    n = 3, raise the order one step at a time)."""
+    x = 0.0
+    M = np.zeros(order + 1)          # 1-based like the reference
+    M[1] = 0.5 * (1.0 - x) ** 2
+    M[2] = (1.0 - x) * x + 0.5
+    M[3] = 0.5 * x * x
+    M[order] = 0.0 if order > 3 else M[order]
+    for n in range(4, order + 1):
+        div = 1.0 / (n - 1)
+        M[n] = x * div * M[n - 1]
+        for j in range(1, n - 1):
+            M[n - j] = ((x + j) * M[n - j - 1] + (n - x - j) * M[n - j]) * div
+        M[1] *= (1.0 - x) * div
+    return M[1:order + 1].copy()
+
+
+def pme_b_moduli(K, order):
+    """compute_b_moduli (src/PmeKSpace.C:21-61): bm[k] = 1/|DFT(M)[k]|^2 with
+    dftmod's cos/sin sums."""
+    M = np.zeros(K)
+    M[:order] = pme_spline_at_integers(order)
+    j = np.arange(K)
+    mod = np.empty(K)
+    for k in range(K):
+        arg = 2.0 * np.pi * k * j / K
+        mod[k] = (M * np.cos(arg)).sum() ** 2 + (M * np.sin(arg)).sum() ** 2
+    return 1.0 / mod
+
+
+def _kspace(K, recip, volume, beta, order):
+    """Per half-complex point (k3, k2, k1): theta = bm1 bm2 bm3 exp(-pi^2 m^2/beta^2)/(pi V m^2),
+    m (3 components), m^2 and the multiplicity (2, or 1 on the k1 = 0 and K1/2 planes)."""
+    K1, K2, K3 = K
+    K1h = K1 // 2 + 1
+    recip = np.asarray(recip, np.float64).reshape(3, 3)
+    k1 = np.arange(K1h)
+    k2 = np.arange(K2)
+    k3 = np.arange(K3)
+    k2s = np.where(k2 <= K2 // 2, k2, k2 - K2)
+    k3s = np.where(k3 <= K3 // 2, k3, k3 - K3)
+    m = (k1[None, None, :, None] * recip[0] + k2s[None, :, None, None] * recip[1]
+         + k3s[:, None, None, None] * recip[2])
+    msq = (m * m).sum(axis=-1)
+    msq[0, 0, 0] = 1.0
+    b1, b2, b3 = pme_b_moduli(K1, order), pme_b_moduli(K2, order), pme_b_moduli(K3, order)
+    bm = b3[:, None, None] * b2[None, :, None] * b1[None, None, :K1h]
+    fac = np.pi ** 2 / beta ** 2
+    theta = bm * np.exp(-fac * msq) / (np.pi * volume * msq)
+    theta[0, 0, 0] = 0.0
+    mult = np.full(K1h, 2.0)
+    mult[0] = 1.0
+    if K1 % 2 == 0:
+        mult[K1 // 2] = 1.0
+    return theta, m, msq, mult[None, None, :], fac
+
+
+def pme_scalar_sum(trans, K, recip, volume, beta, order):
+    """scalar_sum (src/CudaPmeSolverUtilKernel.cu:73-320) in FP64 on a
+    half-complex grid (K3, K2, K1/2+1): returns (multiplied grid, energy,
+    virial6 {xx, xy, xz, yy, yz, zz}), energy = 1/2 sum mult*theta*|S|^2,
+    virial = -1/2 sum mult*theta*|S|^2 (2 (fac + 1/m^2) m_a m_b - delta_ab)."""
+    theta, m, msq, mult, fac = _kspace(K, recip, volume, beta, order)
+    S2 = np.abs(trans) ** 2
+    f = mult * theta * S2
+    energy = 0.5 * f.sum()
+    vir = 2.0 * (fac + 1.0 / msq)
+    v = []
+    for a, b in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)):
+        t = vir * m[..., a] * m[..., b] - (1.0 if a == b else 0.0)
+        v.append(-0.5 * (f * t).sum())
+    return trans * theta, energy, np.array(v)
+
+
+def pme_recip(grid, K, recip, volume, beta, order):
+    """FP64 reciprocal solve of CudaPmeOneDevice::compute
+    (src/CudaPmeSolverUtil.C:1579-1620): R2C FFT of grid (K3, K2, K1),
+    scalar_sum, unnormalised C2R.  Returns (potential grid, energy, virial6)."""
+    K1, K2, K3 = K
+    trans = np.fft.rfftn(np.asarray(grid, np.float64).reshape(K3, K2, K1))
+    out, energy, vir = pme_scalar_sum(trans, K, recip, volume, beta, order)
+    phi = np.fft.irfftn(out, s=(K3, K2, K1), axes=(0, 1, 2)) * (K1 * K2 * K3)
+    return phi, energy, vir
+
+
+def ewald_recip_direct(pos, q_scaled, cell, beta, kmax):
+    """Brute-force Ewald reciprocal sum (no grid): E = 1/(2 pi V) sum_{m != 0}
+    exp(-pi^2 m^2/beta^2)/m^2 |S(m)|^2, S(m) = sum_j q_j exp(2 pi i m.r_j),
+    m = n1 a* + n2 b* + n3 c*, |n_i| <= kmax; and F_j = -dE/dr_j."""
+    cell = np.asarray(cell, np.float64).reshape(3, 3)
+    recip = np.linalg.inv(cell).T
+    V = abs(np.linalg.det(cell))
+    pos = np.asarray(pos, np.float64)
+    q = np.asarray(q_scaled, np.float64)
+    rng = np.arange(-kmax, kmax + 1)
+    n = np.stack(np.meshgrid(rng, rng, rng, indexing="ij"), -1).reshape(-1, 3)
+    n = n[np.any(n != 0, axis=1)]
+    m = n @ recip
+    msq = (m * m).sum(1)
+    f = np.exp(-np.pi ** 2 * msq / beta ** 2) / msq
+    E = 0.0
+    F = np.zeros_like(pos)
+    for c0 in range(0, m.shape[0], 4096):
+        mm = m[c0:c0 + 4096]
+        ph = 2.0 * np.pi * (pos @ mm.T)
+        e = np.exp(1j * ph)
+        S = (q[:, None] * e).sum(0)
+        ff = f[c0:c0 + 4096]
+        E += (ff * np.abs(S) ** 2).sum()
+        # dE/dr_j = 1/(2 pi V) sum f * 2 Re(conj(S) q_j 2 pi i m e_j)
+        g = (ff[None, :] * np.real(np.conj(S)[None, :] * 1j * e)) @ mm
+        F -= (2.0 / V) * q[:, None] * g
+    return E / (2.0 * np.pi * V), F

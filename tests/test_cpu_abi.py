@@ -128,3 +128,16 @@ def test_compute_create_without_gpu_fails_cleanly():
     with pytest.raises(L.Mi355nbError):
         from mi355nb.compute import ComputeNonbonded
         ComputeNonbonded()
+
+
+def test_pme_recip_create_rejects_bad_shapes():
+    # rejected before any device allocation, so this runs without a GPU
+    import ctypes
+    out = ctypes.c_void_p()
+    lib = L.lib()
+    assert lib.mi355nb_pme_recip_create(26, 32, 32, 4, ctypes.byref(out)) != 0   # 13 is not a radix
+    assert b"prime factors" in lib.mi355nb_last_error()
+    assert lib.mi355nb_pme_recip_create(32, 32, 2048, 4, ctypes.byref(out)) != 0  # above 1024
+    assert lib.mi355nb_pme_recip_create(32, 32, 32, 5, ctypes.byref(out)) != 0    # order
+    assert not out.value
+    assert lib.mi355nb_pme_b_moduli(3, 4, np.zeros(3)) != 0                       # K < order
