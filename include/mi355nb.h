@@ -273,6 +273,17 @@ int mi355nb_compute_set_timing(mi355nb_compute* c, int enable);
 int mi355nb_compute_pme_atoms(const mi355nb_compute* c, const double recip[9], const double origin[3],
                               float* atoms, void* stream);
 
+/* PME exclusion correction (exclusionForce of ComputeBondedCUDA,
+ * src/ComputeBondedCUDAKernel.cu:633-772): for every fully excluded pair of
+ * the molecule with r^2 < cutoff^2, E += -qq erf(ewaldcof r)/r, evaluated in
+ * FP64, and its forces are added to the slow forces of the last launch (the
+ * forces finish() returns).  Enqueued on the compute's stream after a launch
+ * with doSlow.  energyVirial: device doubles, 10 per system {E, virial[9]}
+ * (virial = sum f_i (x) r_ij, component a*3+b), overwritten; NULL: forces
+ * only.  NAMD reports these in its bonded reduction, so they are kept apart
+ * from the nonbonded VirialEnergy. */
+int mi355nb_compute_exclusion_correction(mi355nb_compute* c, double* energyVirial);
+
 /* Table mode (doTable of CudaComputeNonbondedKernel::nonbondedForce,
  * src/CudaComputeNonbondedKernel.cu:99-161): later launches take LJ and
  * electrostatics from 4096-entry 1/r tables with linear interpolation

@@ -50,6 +50,8 @@ struct mi355nb_compute {
   std::vector<double> qScaled;
   std::vector<I2> exclByAtom;
   std::vector<uint32_t> exclBitsHost;
+  std::vector<int64_t> exclPtrHost;   // the molecule's exclusion CSR (atom indices)
+  std::vector<int32_t> exclIdxHost;
   std::vector<float> vdwCoefHost;  // 2 * ntypes * ntypes
   bool haveMolecule = false;
   bool haveLayout = false;
@@ -104,6 +106,12 @@ struct mi355nb_compute {
   // PME atoms in slot order
   DevBuf<int> slotPatch;
   DevBuf<double> patchCentre;
+  // PME exclusion correction: excluded pairs in slot space, per-system blocks
+  std::vector<int4> exclBlocksHost;
+  std::vector<int> exclSysStartHost;
+  DevBuf<int4> exclPairs, exclBlocks;
+  DevBuf<int> exclSysStart;
+  DevBuf<double> exclPartials, sysBoxDev;
   // NAMD table mode (optional)
   bool tableMode = false;
   int tableSize = 4096;   // FORCE_ENERGY_TABLE_SIZE of the reference
@@ -141,6 +149,7 @@ struct mi355nb_compute {
     dropGraphs();
     alch.release(); part.release(); redBlocks.release(); redSysStart.release();
     forceTable.release(); energyTable.release(); slotPatch.release(); patchCentre.release();
+    exclPairs.release(); exclBlocks.release(); exclSysStart.release(); exclPartials.release(); sysBoxDev.release();
     fmTarget.release(); fmLossPartials.release(); fmLoss.release(); fmGq.release(); fmBinPartials.release();
     fmBins.release(); fmG.release();
     xyzq.release(); force.release(); forceSlow.release();
@@ -296,6 +305,8 @@ int mi355nb_compute_set_molecule(mi355nb_compute* c, int natoms, const int32_t* 
   }
   c->natoms = natoms;
   c->ntypes = ntypes;
+  c->exclPtrHost.assign(excl_ptr, excl_ptr + natoms + 1);
+  c->exclIdxHost.assign(excl_idx, excl_idx + excl_ptr[natoms]);
   c->vdwType.assign(vdw_type, vdw_type + natoms);
   c->qScaled.resize(natoms);
   const double qs = std::sqrt(COULOMB);  // nonbondedScaling = 1, dielectric = 1
@@ -429,6 +440,42 @@ static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomSt
     MI355NB_HIP(c->patchCentre.reserve(c->L.centre.size()));
     MI355NB_HIP(hipMemcpy(c->slotPatch.ptr, sp.data(), n * sizeof(int), hipMemcpyHostToDevice));
     MI355NB_HIP(hipMemcpy(c->patchCentre.ptr, c->L.centre.data(), c->L.centre.size() * sizeof(double),
+                          hipMemcpyHostToDevice));
+  }
+  {
+    // excluded pairs (each once) in slot space, grouped by system: atoms of a
+    // system are contiguous and a pair belongs to the system of its lower atom
+    std::vector<long long> keys;
+    keys.reserve(c->exclIdxHost.size());
+    for (int a = 0; a < c->natoms; ++a)
+      for (int64_t k = c->exclPtrHost[a]; k < c->exclPtrHost[a + 1]; ++k) {
+        const long long b = c->exclIdxHost[k];
+        keys.push_back(std::min<long long>(a, b) * c->natoms + std::max<long long>(a, b));
+      }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    std::vector<int4> pairs(keys.size());
+    std::vector<int> pairSys(keys.size());
+    int sys = 0;
+    for (size_t k = 0; k < keys.size(); ++k) {
+      const int a = (int)(keys[k] / c->natoms), b = (int)(keys[k] % c->natoms);
+      while (a >= c->L.sysAtomStart[sys + 1]) ++sys;
+      pairs[k] = make_int4(c->L.slotOfAtom[a], c->L.slotOfAtom[b], 0, 0);
+      pairSys[k] = sys;
+    }
+    buildExclusionBlocks(pairSys, nsys, c->exclBlocksHost, c->exclSysStartHost);
+    const size_t nb = c->exclBlocksHost.size();
+    MI355NB_HIP(c->exclPairs.reserve(std::max<size_t>(1, pairs.size()), 1.0f));
+    MI355NB_HIP(c->exclBlocks.reserve(std::max<size_t>(1, nb), 1.0f));
+    MI355NB_HIP(c->exclSysStart.reserve(nsys + 1, 1.0f));
+    MI355NB_HIP(c->exclPartials.reserve(std::max<size_t>(1, nb) * EXCL_VALUES_PER_SYSTEM, 1.0f));
+    MI355NB_HIP(c->sysBoxDev.reserve(3 * (size_t)nsys, 1.0f));
+    if (!pairs.empty())
+      MI355NB_HIP(hipMemcpy(c->exclPairs.ptr, pairs.data(), pairs.size() * sizeof(int4), hipMemcpyHostToDevice));
+    if (nb) MI355NB_HIP(hipMemcpy(c->exclBlocks.ptr, c->exclBlocksHost.data(), nb * sizeof(int4), hipMemcpyHostToDevice));
+    MI355NB_HIP(hipMemcpy(c->exclSysStart.ptr, c->exclSysStartHost.data(), (nsys + 1) * sizeof(int),
+                          hipMemcpyHostToDevice));
+    MI355NB_HIP(hipMemcpy(c->sysBoxDev.ptr, c->L.sysBox.data(), 3 * (size_t)nsys * sizeof(double),
                           hipMemcpyHostToDevice));
   }
   MI355NB_HIP(launchFill(c->fsoa.cap, c->fsoa.ptr, 0.0f, c->stream));
@@ -853,6 +900,17 @@ int mi355nb_compute_finish_batch(mi355nb_compute* c, double* forceOut, double* f
   if (vePerSystem)
     MI355NB_HIP(hipMemcpy(vePerSystem, c->ve.ptr, sizeof(VirialEnergy) * c->L.numSystems, hipMemcpyDeviceToHost));
   if (statOut) MI355NB_HIP(hipMemcpy(statOut, c->stat.ptr, sizeof(TileListStat), hipMemcpyDeviceToHost));
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_exclusion_correction(mi355nb_compute* c, double* energyVirial) {
+  CHECK_ARG(c, "null compute");
+  CHECK_STATE(c->haveLayout && c->lastSlow, "a launch with doSlow must precede the exclusion correction");
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(launchExclusionCorrection((int)c->exclBlocksHost.size(), c->L.numSystems, c->exclPairs.ptr,
+                                        c->exclBlocks.ptr, c->exclSysStart.ptr, c->xyzq.ptr, c->slotPatch.ptr,
+                                        c->patchCentre.ptr, c->sysBoxDev.ptr, c->p.cutoff, c->p.ewaldcof,
+                                        c->forceSlow.ptr, c->exclPartials.ptr, energyVirial, c->stream));
   return MI355NB_OK;
 }
 

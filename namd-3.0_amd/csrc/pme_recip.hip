@@ -34,13 +34,17 @@ constexpr int FFT_MAX_N = 1024;
 constexpr int FFT_MAX_STAGES = 12;
 constexpr size_t FFT_LDS_BYTES = 60 * 1024;  // dynamic LDS per workgroup
 constexpr int EV_VALUES = 7;  // energy, virial xx xy xz yy yz zz
+constexpr int FFT_X_LINES = 16;   // lines per workgroup of the x passes
+constexpr int FFT_YZ_LINES = 16;  // x-columns per workgroup of the y and z passes
+constexpr int LD_BATCH = 8;       // global loads in flight per thread
 
 struct FftAxis {
   int n;                         // line length
   int nst;                       // number of stages
   int radix[FFT_MAX_STAGES];     // stage radices, first stage first
   int twOff;                     // offset of this axis's twiddle table
-  int lines;                     // lines per workgroup (T)
+  int lines;                     // lines per workgroup (T, a power of 2)
+  int logLines;
   int pitch;                     // LDS line pitch in complex elements (odd)
 };
 
@@ -60,6 +64,14 @@ __device__ __forceinline__ float2 twiddle(const float2* tw, int t) {
   const float2 w = tw[t];
   return INV ? make_float2(w.x, -w.y) : w;
 }
+
+// Division of small non-negative integers by a runtime divisor d (1..2^16):
+// q = umulhi(n, ceil(2^32 / d)) is exact while n * d < 2^32.
+struct FastDiv {
+  unsigned d, m;
+  __device__ __forceinline__ explicit FastDiv(unsigned dd) : d(dd), m(0xFFFFFFFFu / dd + 1u) {}
+  __device__ __forceinline__ unsigned div(unsigned n) const { return d == 1u ? n : __umulhi(n, m); }
+};
 
 // In-place DFT of R points v[0..R-1]; roots from the n-point table (R | n).
 template <int R, bool INV>
@@ -108,10 +120,12 @@ template <int R, bool INV>
 __device__ __forceinline__ void fftStage(const float2* a, float2* b, const float2* tw, int N, int P, int T, int Ns) {
   const int NR = N / R;
   const int step = N / (Ns * R);
+  const FastDiv dNR(NR), dNs(Ns);
   for (int e = threadIdx.x; e < T * NR; e += FFT_THREADS) {
-    const int line = e / NR;
+    const int line = (int)dNR.div(e);
     const int j = e - line * NR;
-    const int k = j % Ns;
+    const int jq = (int)dNs.div(j);
+    const int k = j - jq * Ns;
     const float2* in = a + line * P;
     float2* out = b + line * P;
     float2 v[R];
@@ -122,7 +136,7 @@ __device__ __forceinline__ void fftStage(const float2* a, float2* b, const float
       for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<INV>(tw, k * r * step));
     }
     butterfly<R, INV>(v, tw, N);
-    const int o = (j / Ns) * Ns * R + k;
+    const int o = jq * Ns * R + k;
 #pragma unroll
     for (int q = 0; q < R; ++q) out[o + q * Ns] = v[q];
   }
@@ -169,15 +183,27 @@ fftXR2CKernel(const float* __restrict__ grid, int xdim, int nlines, FftAxis ax, 
   loadTwiddles(s_tw, tw, ax);
   const int line0 = blockIdx.x * T;
   const int nl = min(T, nlines - line0);
-  for (int e = threadIdx.x; e < T * N; e += FFT_THREADS) {
-    const int l = e / N, x = e - l * N;
-    const float v = l < nl ? grid[(size_t)(line0 + l) * xdim + x] : 0.0f;
-    A[l * P + x] = make_float2(v, 0.0f);
+  const FastDiv dN(N);
+  for (int e0 = threadIdx.x; e0 < T * N; e0 += FFT_THREADS * LD_BATCH) {
+    float v[LD_BATCH];
+#pragma unroll
+    for (int u = 0; u < LD_BATCH; ++u) {
+      const int e = e0 + u * FFT_THREADS;
+      const int l = (int)dN.div(e), x = e - l * N;
+      v[u] = (e < T * N && l < nl) ? grid[(size_t)(line0 + l) * xdim + x] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < LD_BATCH; ++u) {
+      const int e = e0 + u * FFT_THREADS;
+      const int l = (int)dN.div(e), x = e - l * N;
+      if (e < T * N) A[l * P + x] = make_float2(v[u], 0.0f);
+    }
   }
   __syncthreads();
   const float2* res = fftLines<false>(A, B, s_tw, ax);
+  const FastDiv dNh(Nh);
   for (int e = threadIdx.x; e < nl * Nh; e += FFT_THREADS) {
-    const int l = e / Nh, k = e - l * Nh;
+    const int l = (int)dNh.div(e), k = e - l * Nh;
     trans[(size_t)(line0 + l) * Nh + k] = res[l * P + k];
   }
 }
@@ -197,16 +223,29 @@ fftXC2RKernel(const float2* __restrict__ trans, int nlines, FftAxis ax, const fl
   loadTwiddles(s_tw, tw, ax);
   const int line0 = blockIdx.x * T;
   const int nl = min(T, nlines - line0);
-  for (int e = threadIdx.x; e < T * Nh; e += FFT_THREADS) {
-    const int l = e / Nh, k = e - l * Nh;
-    const float2 v = l < nl ? trans[(size_t)(line0 + l) * Nh + k] : make_float2(0.0f, 0.0f);
-    A[l * P + k] = v;
-    if (k > 0 && k < N - k) A[l * P + N - k] = make_float2(v.x, -v.y);
+  const FastDiv dNh(Nh), dN(N);
+  for (int e0 = threadIdx.x; e0 < T * Nh; e0 += FFT_THREADS * LD_BATCH) {
+    float2 v[LD_BATCH];
+#pragma unroll
+    for (int u = 0; u < LD_BATCH; ++u) {
+      const int e = e0 + u * FFT_THREADS;
+      const int l = (int)dNh.div(e), k = e - l * Nh;
+      v[u] = (e < T * Nh && l < nl) ? trans[(size_t)(line0 + l) * Nh + k] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < LD_BATCH; ++u) {
+      const int e = e0 + u * FFT_THREADS;
+      const int l = (int)dNh.div(e), k = e - l * Nh;
+      if (e < T * Nh) {
+        A[l * P + k] = v[u];
+        if (k > 0 && k < N - k) A[l * P + N - k] = make_float2(v[u].x, -v[u].y);
+      }
+    }
   }
   __syncthreads();
   const float2* res = fftLines<true>(A, B, s_tw, ax);
   for (int e = threadIdx.x; e < nl * N; e += FFT_THREADS) {
-    const int l = e / N, x = e - l * N;
+    const int l = (int)dN.div(e), x = e - l * N;
     grid[(size_t)(line0 + l) * xdim + x] = res[l * P + x].x;
   }
 }
@@ -293,9 +332,20 @@ fftStridedKernel(float2* __restrict__ data, int nx, int stride, int outerStride,
   const int outer = blockIdx.y;
   const int tc = min(T, nx - x0);
   float2* base = data + (size_t)outer * outerStride + x0;
-  for (int e = threadIdx.x; e < T * N; e += FFT_THREADS) {
-    const int k = e / T, t = e - k * T;
-    A[t * P + k] = t < tc ? base[t + (size_t)k * stride] : make_float2(0.0f, 0.0f);
+  const int lg = ax.logLines;
+  for (int e0 = threadIdx.x; e0 < T * N; e0 += FFT_THREADS * LD_BATCH) {
+    float2 v[LD_BATCH];
+#pragma unroll
+    for (int u = 0; u < LD_BATCH; ++u) {
+      const int e = e0 + u * FFT_THREADS;
+      const int k = e >> lg, t = e & (T - 1);
+      v[u] = (e < T * N && t < tc) ? base[t + (size_t)k * stride] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < LD_BATCH; ++u) {
+      const int e = e0 + u * FFT_THREADS;
+      if (e < T * N) A[(e & (T - 1)) * P + (e >> lg)] = v[u];
+    }
   }
   __syncthreads();
   float2* res;
@@ -307,19 +357,21 @@ fftStridedKernel(float2* __restrict__ data, int nx, int stride, int outerStride,
   if (MODE == 2) {
     // res holds the full 3-D transform of the tile; k1 = x0 + t, k2 = outer, k3 = k
     double ev[EV_VALUES] = {0, 0, 0, 0, 0, 0, 0};
+    const FastDiv dN(N);
     for (int e = threadIdx.x; e < tc * N; e += FFT_THREADS) {
-      const int t = e / N, k = e - t * N;
+      const int t = (int)dN.div(e), k = e - t * N;
       const float2 q = res[t * P + k];
       const float th = kspaceFactor<EV>(ks, bm, x0 + t, outer, k, q, ev);
       res[t * P + k] = make_float2(q.x * th, q.y * th);
     }
-    __syncthreads();
+    // (writeEvPartial synchronises the workgroup)
+    if (EV) writeEvPartial(ev, partials, blockIdx.y * gridDim.x + blockIdx.x);
+    else __syncthreads();
     float2* other = (res == A) ? B : A;
     res = fftLines<true>(res, other, s_tw, ax);
-    if (EV) writeEvPartial(ev, partials, blockIdx.y * gridDim.x + blockIdx.x);
   }
   for (int e = threadIdx.x; e < T * N; e += FFT_THREADS) {
-    const int k = e / T, t = e - k * T;
+    const int k = e >> lg, t = e & (T - 1);
     if (t < tc) base[t + (size_t)k * stride] = res[t * P + k];
   }
 }
@@ -346,14 +398,25 @@ scalarSumKernel(float2* __restrict__ data, KSpace ks, const double* __restrict__
 
 // Ordered sum of the workgroup partials: energy * 1/2, virial * -1/2
 // (src/CudaPmeSolverUtilKernel.cu:305-316), written as {E, v_xx, v_xy, v_xz,
-// v_yy, v_yz, v_zz}.
-__global__ void evFinalKernel(int nparts, const double* __restrict__ partials, double* __restrict__ out) {
-  const int c = threadIdx.x;
-  if (c < EV_VALUES) {
-    double s = 0.0;
-    for (int p = 0; p < nparts; ++p) s += partials[(size_t)p * EV_VALUES + c];
-    out[c] = (c == 0 ? 0.5 : -0.5) * s;
+// v_yy, v_yz, v_zz}.  One workgroup: thread t sums partials t, t+256, ...,
+// then a fixed-shape tree; the order does not depend on timing.
+__global__ void __launch_bounds__(FFT_THREADS)
+evFinalKernel(int nparts, const double* __restrict__ partials, double* __restrict__ out) {
+  __shared__ double s[EV_VALUES][FFT_THREADS];
+  double acc[EV_VALUES] = {0, 0, 0, 0, 0, 0, 0};
+  for (int p = threadIdx.x; p < nparts; p += FFT_THREADS) {
+#pragma unroll
+    for (int c = 0; c < EV_VALUES; ++c) acc[c] += partials[(size_t)p * EV_VALUES + c];
   }
+#pragma unroll
+  for (int c = 0; c < EV_VALUES; ++c) s[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int w = FFT_THREADS / 2; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < EV_VALUES; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < EV_VALUES) out[threadIdx.x] = (threadIdx.x == 0 ? 0.5 : -0.5) * s[threadIdx.x][0];
 }
 
 // Host: order-n cardinal B-spline values at the integer points 1 .. n-1
@@ -391,7 +454,7 @@ static void bModuli(int K, int order, double* bm) {
   }
 }
 
-static bool planAxis(int n, FftAxis& ax) {
+static bool planAxis(int n, int lines, FftAxis& ax) {
   if (n < 2 || n > FFT_MAX_N) return false;
   ax.n = n;
   ax.nst = 0;
@@ -405,8 +468,10 @@ static bool planAxis(int n, FftAxis& ax) {
   }
   if (m != 1) return false;
   ax.pitch = n | 1;
-  ax.lines = 16;
+  ax.lines = lines;
   while (ax.lines > 1 && sizeof(float2) * ((size_t)n + 2 * (size_t)ax.lines * ax.pitch) > FFT_LDS_BYTES) ax.lines /= 2;
+  ax.logLines = 0;
+  while ((1 << ax.logLines) < ax.lines) ++ax.logLines;
   return true;
 }
 
@@ -515,7 +580,8 @@ int mi355nb_pme_recip_create(int K1, int K2, int K3, int order, mi355nb_pme_reci
   pr->K2 = K2;
   pr->K3 = K3;
   pr->order = order;
-  if (!planAxis(K1, pr->ax1) || !planAxis(K2, pr->ax2) || !planAxis(K3, pr->ax3)) {
+  if (!planAxis(K1, FFT_X_LINES, pr->ax1) || !planAxis(K2, FFT_YZ_LINES, pr->ax2) ||
+      !planAxis(K3, FFT_YZ_LINES, pr->ax3)) {
     delete pr;
     setLastError("PME grid sizes must be at most 1024 with prime factors 2, 3, 5, 7, 11");
     return MI355NB_ERR_ARG;
@@ -602,7 +668,7 @@ int mi355nb_pme_recip_scalar_sum(mi355nb_pme_recip* pr, const double recip[9], d
   if (doEnergyVirial) {
     hipLaunchKernelGGL(scalarSumKernel<true>, dim3(nb), dim3(FFT_THREADS), 0, s, d, ks, pr->bm, pr->partials);
     MI355NB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(evFinalKernel, dim3(1), dim3(64), 0, s, nb, pr->partials, energyVirial);
+    hipLaunchKernelGGL(evFinalKernel, dim3(1), dim3(FFT_THREADS), 0, s, nb, pr->partials, energyVirial);
   } else {
     hipLaunchKernelGGL(scalarSumKernel<false>, dim3(nb), dim3(FFT_THREADS), 0, s, d, ks, pr->bm, pr->partials);
   }
@@ -624,7 +690,8 @@ int mi355nb_pme_recip_solve(mi355nb_pme_recip* pr, const double recip[9], double
   if (rc) return rc;
   if (doEnergyVirial) {
     const dim3 g = stridedGrid(pr, pr->ax3, pr->K2);
-    hipLaunchKernelGGL(evFinalKernel, dim3(1), dim3(64), 0, s, (int)(g.x * g.y), pr->partials, energyVirial);
+    hipLaunchKernelGGL(evFinalKernel, dim3(1), dim3(FFT_THREADS), 0, s, (int)(g.x * g.y), pr->partials,
+                       energyVirial);
     MI355NB_HIP(hipGetLastError());
   }
   rc = launchY<1, false>(pr, s);

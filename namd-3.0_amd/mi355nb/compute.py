@@ -108,6 +108,7 @@ class ComputeNonbonded:
         L.check(L.lib().mi355nb_compute_assign_patches(self._h, np.ascontiguousarray(box, np.float64),
                                                        np.ascontiguousarray(pos, np.float64).reshape(-1)),
                 "assign_patches")
+        self.sys_atom_start = np.array([0, self.natoms], np.int32)
 
     def assign_patches_batch(self, sys_atom_start, boxes, pos):
         starts = np.ascontiguousarray(sys_atom_start, np.int32)
@@ -229,6 +230,18 @@ class ComputeNonbonded:
                 "pme_atoms")
         return out
 
+    def exclusion_correction(self, energy_virial=True):
+        """PME exclusion correction (FP64) for the last launch with slow forces:
+        adds its forces to the slow forces finish() returns; returns a cuda
+        float64 (numSystems, 10) tensor {E, virial[9]} or None."""
+        import torch
+        ev = None
+        if energy_virial:
+            ev = torch.empty((self.info()["numSystems"], 10), dtype=torch.float64, device="cuda")
+        L.check(L.lib().mi355nb_compute_exclusion_correction(self._h, ev.data_ptr() if ev is not None else None),
+                "exclusion_correction")
+        return ev
+
     def set_force_table(self, enable=True):
         """NAMD's 1/r table mode (True) or direct math (False, the default)."""
         L.check(L.lib().mi355nb_compute_set_force_table(self._h, int(enable)), "set_force_table")
@@ -294,6 +307,15 @@ class ComputeNonbonded:
                 "get_tile_lists")
         return dict(iatomStart=ia, jtileStart=js, jtileEnd=je, shift=sh.reshape(-1, 3), jatomStart=ja[: nj.value],
                     excl=ex[: nj.value], atomIndex=slots)
+
+    def slot_of_atom(self):
+        """Storage slot of every atom (padding slots repeat an atom's index
+        after its own slot; the first occurrence is the atom's slot)."""
+        idx = self.tile_lists()["atomIndex"]
+        atoms, first = np.unique(idx, return_index=True)
+        out = np.full(self.natoms, -1, np.int64)
+        out[atoms] = first
+        return out
 
 
 def decode_pairs(tl, natoms_storage_valid=None):

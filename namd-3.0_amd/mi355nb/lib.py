@@ -75,7 +75,7 @@ EXPORTED = [
     "mi355nb_build_force_energy_tables", "mi355nb_compute_set_force_table", "mi355nb_compute_pme_atoms",
     "mi355nb_compute_get_system_boxes", "mi355nb_pme_b_moduli", "mi355nb_pme_recip_create",
     "mi355nb_pme_recip_destroy", "mi355nb_pme_recip_fft_r2c", "mi355nb_pme_recip_fft_c2r",
-    "mi355nb_pme_recip_scalar_sum", "mi355nb_pme_recip_solve",
+    "mi355nb_pme_recip_scalar_sum", "mi355nb_pme_recip_solve", "mi355nb_compute_exclusion_correction",
 ]
 
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -135,6 +135,7 @@ def lib():
     L.mi355nb_compute_get_system_boxes.argtypes = [vp, dp]
     L.mi355nb_compute_force_matching_grad.argtypes = [vp, dp, C.POINTER(C.c_double), vp, vp, vp]
     L.mi355nb_pme_b_moduli.argtypes = [C.c_int, C.c_int, dp]
+    L.mi355nb_compute_exclusion_correction.argtypes = [vp, vp]
     L.mi355nb_pme_recip_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L.mi355nb_pme_recip_destroy.argtypes = [vp]
     L.mi355nb_pme_recip_fft_r2c.argtypes = [vp, vp, C.c_int, vp, vp]

@@ -407,3 +407,36 @@ def ewald_recip_direct(pos, q_scaled, cell, beta, kmax):
         g = (ff[None, :] * np.real(np.conj(S)[None, :] * 1j * e)) @ mm
         F -= (2.0 / V) * q[:, None] * g
     return E / (2.0 * np.pi * V), F
+
+
+def pme_exclusion_correction(pos, q_scaled, excl_ptr, excl_idx, box, beta, cutoff):
+    """PME exclusion correction (exclusionForce, src/ComputeBondedCUDAKernel.cu:633-772)
+    in FP64: for each fully excluded pair (once) with minimum-image r^2 <
+    cutoff^2, E = -qq erf(beta r)/r.  Returns (E, F (n,3), virial 3x3 =
+    sum f_i (x) r_ij)."""
+    from scipy.special import erf
+    pos = np.asarray(pos, np.float64)
+    q = np.asarray(q_scaled, np.float64)
+    box = np.asarray(box, np.float64)
+    n = len(q)
+    ii = np.repeat(np.arange(n), np.diff(np.asarray(excl_ptr)))
+    jj = np.asarray(excl_idx, np.int64)
+    a, b = np.minimum(ii, jj), np.maximum(ii, jj)
+    key = np.unique(a * n + b)
+    a, b = key // n, key % n
+    d = pos[a] - pos[b]
+    d -= box * np.rint(d / box)
+    r2 = (d * d).sum(1)
+    keep = r2 < cutoff * cutoff
+    a, b, d, r2 = a[keep], b[keep], d[keep], r2[keep]
+    r = np.sqrt(r2)
+    qq = q[a] * q[b]
+    e = erf(beta * r) / r
+    E = float(-(qq * e).sum())
+    dedr = -qq * (2.0 / np.sqrt(np.pi) * beta * np.exp(-beta * beta * r2) / r - e / r)
+    f = (-dedr / r)[:, None] * d
+    F = np.zeros((n, 3))
+    np.add.at(F, a, f)
+    np.add.at(F, b, -f)
+    W = np.einsum("ka,kb->ab", f, d)
+    return E, F, W

@@ -1,0 +1,148 @@
+"""GPU parity of the PME exclusion correction (FP64, through the C ABI)
+against the oracle (pinned in test_cpu_exclusion_oracle.py), alone and as
+part of the complete Ewald electrostatics: GPU pair kernel (real space) +
+GPU reciprocal solve + GPU self energy + GPU exclusion correction against the
+same sum from the FP64 oracles.
+
+Tolerances: correction energy and virial within 1e-9 relative (FP64 from the
+same float32 inputs); correction forces within 1e-5 relative RMS (they are
+added to FP32 slow forces); complete electrostatic energy within 1e-6 and
+total slow + reciprocal forces within 1e-5 relative RMS.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mi355nb import compute as mc
+from mi355nb import lib as L
+from mi355nb import systems
+from mi355nb.compute import ComputeNonbonded
+from oracle import oracle as orc
+
+from nbcheck import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+COULOMB = 332.0636
+
+
+def _q32(s):
+    return (s.charge * math.sqrt(COULOMB)).astype(np.float32).astype(np.float64)
+
+
+def _correction_run(torch, c):
+    """Slow forces of one launch without and with the correction, and the
+    correction's per-system energy/virial."""
+    c.launch_work(do_energy=False, do_virial=False, do_slow=True, do_pairlist=True)
+    r0 = c.finish_batch()
+    c.launch_work(do_energy=False, do_virial=False, do_slow=True, do_pairlist=False)
+    ev = c.exclusion_correction()
+    r1 = c.finish_batch()
+    torch.cuda.synchronize()
+    return r0[3], r1[3], ev.cpu().numpy()
+
+
+def _check_correction(tag, dF, ev, pos, q, s, box, beta, cutoff):
+    E, F, W = orc.pme_exclusion_correction(pos, q, s.excl_ptr, s.excl_idx, box, beta, cutoff)
+    ee = abs(ev[0] - E) / abs(E)
+    ew = np.abs(ev[1:] - W.reshape(-1)).max() / np.abs(W).max()
+    ef = rel_rms(dF, F)
+    print(f"    {tag}: E {ev[0]:.12e} vs {E:.12e} rel {ee:.2e}; virial {ew:.2e}; forces {ef:.2e}")
+    assert ee < 1e-9 and ew < 1e-9
+    assert ef < 1e-5
+
+
+@pytest.mark.parametrize("name", ["small", "solvated96k"])
+def test_exclusion_correction_matches_oracle(torch_cuda, name):
+    torch = torch_cuda
+    s = systems.small_box() if name == "small" else systems.solvated_96k()
+    p = s.params
+    c = ComputeNonbonded.from_system(s, stream=torch.cuda.current_stream().cuda_stream)
+    print(f"[exclusion {name}] {s.natoms} atoms", flush=True)
+    fs0, fs1, ev = _correction_run(torch, c)
+    beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+    _check_correction(name, fs1 - fs0, ev[0], c.effective_positions(), _q32(s), s, c.system_boxes()[0], beta,
+                      p.cutoff)
+    c.close()
+
+
+def test_exclusion_correction_batch(torch_cuda):
+    torch = torch_cuda
+    members = systems.batch_systems(n_systems=5, seed=9, n_min=3000, n_max=8000)
+    c = ComputeNonbonded.from_systems([m.system for m in members], stream=torch.cuda.current_stream().cuda_stream)
+    print(f"[exclusion batch] {c.natoms} atoms", flush=True)
+    fs0, fs1, ev = _correction_run(torch, c)
+    pos = c.effective_positions()
+    boxes = c.system_boxes()
+    st = c.sys_atom_start
+    for k, m in enumerate(members):
+        s = m.system
+        beta = systems.ewald_coefficient(s.params.cutoff, s.params.ewald_tol)
+        a, b = st[k], st[k + 1]
+        _check_correction(f"system {k}", fs1[a:b] - fs0[a:b], ev[k], pos[a:b], _q32(s), s, boxes[k], beta,
+                          s.params.cutoff)
+    c.close()
+
+
+def test_complete_ewald_electrostatics(torch_cuda):
+    """Real space (pair kernel) + reciprocal + self + exclusion correction on
+    the GPU against the FP64 oracles for energy and forces."""
+    torch = torch_cuda
+    s = systems.small_box(n_waters=600, n_chains=4, chain_len=30, box=32.0, seed=21, cutoff=12.0,
+                          switchdist=10.0, pairlistdist=13.5, pme_grid=(32, 32, 32))
+    p = s.params
+    c = ComputeNonbonded.from_system(s, stream=torch.cuda.current_stream().cuda_stream)
+    print(f"[complete ewald] {s.natoms} atoms", flush=True)
+    c.launch_work(do_energy=True, do_virial=True, do_slow=True, do_pairlist=True)
+    ev = c.exclusion_correction()
+    r = c.finish()
+    box = c.system_boxes()[0]
+    recip = np.diag(1.0 / box)
+    vol = float(np.prod(box))
+    beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+    K, order = p.pme_grid, p.pme_order
+    at = c.pme_atoms(recip)
+    pr = mc.PmeRecip(K, order)
+    grid = torch.zeros(K[0] * K[1] * K[2], dtype=torch.float32, device="cuda")
+    mc.spread_charge(at, grid, K, order)
+    evr = pr.solve(grid, recip, vol, beta)
+    frec = mc.gather_force(at, grid, K, order, recip=recip)
+    es = mc.self_energy(at, beta)
+    torch.cuda.synchronize()
+    E_gpu = r["e_elec"] + r["e_slow"] + evr.cpu().numpy()[0] + es.cpu().numpy()[0] + ev.cpu().numpy()[0, 0]
+    # recip forces are per slot: map back to atoms
+    slot_of_atom = c.slot_of_atom()
+    F_gpu = r["f"] + r["fs"] + frec.cpu().numpy()[slot_of_atom, :3]
+
+    pos = c.effective_positions()
+    q = _q32(s)
+    A, B = L.lj_mix(s.lj_sigma, s.lj_eps)
+    o = orc.nb_fp64(pos, q, s.vdw_type, A, B, s.excl_ptr, s.excl_idx, box, p.cutoff, p.switchdist, beta)
+    ata = at.cpu().numpy()
+    g = orc.pme_spread_gpu(ata, K, order)
+    phi, Erec, _ = orc.pme_recip(g, K, recip, vol, beta, order)
+    Frec = orc.pme_gather_gpu(ata, K, order, phi, recip)[slot_of_atom]
+    Ecor, Fcor, _ = orc.pme_exclusion_correction(pos, q, s.excl_ptr, s.excl_idx, box, beta, p.cutoff)
+    Eself = -beta / math.sqrt(math.pi) * (ata[:, 3].astype(np.float64) ** 2).sum()
+    E_o = o["e_elec"] + o["e_slow"] + Erec + Eself + Ecor
+    F_o = o["f"] + o["fs"] + Frec + Fcor
+    ee = abs(E_gpu - E_o) / abs(E_o)
+    ef = rel_rms(F_gpu, F_o)
+    print(f"    E gpu {E_gpu:.10e} oracle {E_o:.10e} rel {ee:.2e}; forces {ef:.2e}")
+    assert ee < 1e-6
+    assert ef < 1e-5
+    pr.close()
+    c.close()
+
+
+def test_exclusion_correction_requires_slow_launch(torch_cuda):
+    torch = torch_cuda
+    s = systems.small_box()
+    c = ComputeNonbonded.from_system(s, stream=torch.cuda.current_stream().cuda_stream)
+    with pytest.raises(L.Mi355nbError):
+        c.exclusion_correction()
+    c.launch_work(do_energy=False, do_virial=False, do_slow=False, do_pairlist=True)
+    with pytest.raises(L.Mi355nbError):
+        c.exclusion_correction()
+    c.close()
