@@ -1,22 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark of the nonbonded + PME real-space hot path on one MI355X.
+"""Benchmark of the nonbonded + PME force path on one MI355X.
 
-One step = the hot path of one MD force evaluation on the 96,000-atom
-solvated box (26,000 TIP3P + 60 chains x 300 beads, 98 A cube, cutoff 12 A,
-switchdist 10 A, pairlistdist 13.5 A, PME 108^3 order 4):
+One step = the nonbonded electrostatics + LJ force evaluation of one MD step
+on the 96,000-atom solvated box (26,000 TIP3P + 60 chains x 300 beads, 98 A
+cube, cutoff 12 A, switchdist 10 A, pairlistdist 13.5 A, PME 108^3 order 4):
 
     PME stream:       fractional coordinates (from the pair path's patch-
                       ordered slots, as NAMD feeds PME per patch) -> grid
-                      clear -> charge spread -> force gather
-    nonbonded stream: pair kernel (fast + slow forces) -> force transpose
+                      clear -> charge spread -> reciprocal solve (R2C FFT,
+                      k-space multiply, C2R FFT) -> force gather
+    nonbonded stream: pair kernel (fast + slow forces) -> force transpose ->
+                      PME exclusion correction (FP64)
 
 The two streams run concurrently and join at the end of every step
-(--no-overlap puts both on one stream).
+(--no-overlap puts both on one stream).  --electrostatics realspace drops the
+reciprocal solve and the exclusion correction (the pair path and the PME
+spread/gather that feeds it only).
 
-Pairlists are rebuilt (tile-list build + refinement in the pair kernel) every
-20 steps and energies + virial are reduced on those steps too; the other
-steps are force-only, as in NAMD's GPU-resident defaults.  The PME k-space
-solve (FFT + convolution), bonded terms and integration are not part of this
+Pairlists are rebuilt (tile-list build + refinement) every 20 steps and
+energies + virial (nonbonded, reciprocal, exclusion correction) are reduced
+on those steps too; the other steps are force-only, as in NAMD's
+GPU-resident defaults.  Bonded terms and integration are not part of this
 path and are not timed.  Inputs (positions, charges, tables, tile lists) are
 resident in HBM when the timed region starts.
 
@@ -63,6 +67,9 @@ def main():
                          "batch64: 64 uneven alchemical systems in one launch against the same systems run one "
                          "at a time; fit23k: force-matching loss + gradient on 16 frames of the 23k box")
     ap.add_argument("--force-table", action="store_true", help="NAMD's 1/r table mode instead of direct math")
+    ap.add_argument("--electrostatics", choices=("full", "realspace"), default="full",
+                    help="full (default): PME real space + reciprocal solve + exclusion correction; realspace: "
+                         "pair kernel + PME spread/gather only")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -110,6 +117,12 @@ def main():
     atoms = torch.empty((nslots, 4), dtype=torch.float32, device=dev)
     grid = torch.zeros(K[0] * K[1] * K[2], dtype=torch.float32, device=dev)
     fpme = torch.empty((nslots, 4), dtype=torch.float32, device=dev)
+    full = args.electrostatics == "full"
+    pr = mc.PmeRecip(K, order) if full else None
+    vol = float(np.prod(box))
+    beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+    ev_recip = torch.zeros(7, dtype=torch.float64, device=dev)
+    ev_excl = torch.zeros((1, 10), dtype=torch.float64, device=dev)
 
     if args.calibrate_copy:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
@@ -129,16 +142,22 @@ def main():
             c.pme_atoms(recip, out=atoms, stream=pp)
             grid.zero_()
             if timed:
-                e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+                e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
                 e[0].record(pme_stream)
             mc.spread_charge(atoms, grid, K, order, stream=pp)
             if timed:
                 e[1].record(pme_stream)
-            mc.gather_force(atoms, grid, K, order, recip=recip, out=fpme, stream=pp)
+            if full:
+                pr.solve(grid, recip, vol, beta, energy_virial=pl, out=ev_recip, stream=pp)
             if timed:
                 e[2].record(pme_stream)
+            mc.gather_force(atoms, grid, K, order, recip=recip, out=fpme, stream=pp)
+            if timed:
+                e[3].record(pme_stream)
                 ev.setdefault("pme", []).append(e)
         c.launch_work(do_energy=pl, do_virial=pl, do_slow=True, do_pairlist=pl)
+        if full:
+            c.exclusion_correction(energy_virial=pl, out=ev_excl)
         stream.wait_stream(pme_stream)       # the step's forces are complete
 
     log(f"warmup {args.warmup} steps")
@@ -177,8 +196,9 @@ def main():
     c.set_timing(False)
     ms_step = (t1 - t0) * 1e3 / args.steps
     ns_day = 2e-6 * 86400.0 / (ms_step * 1e-3)
-    spread_ms = np.mean([a.elapsed_time(b) for a, b, _ in ev["pme"]])
-    gather_ms = np.mean([b.elapsed_time(cc) for _, b, cc in ev["pme"]])
+    spread_ms = np.mean([e[0].elapsed_time(e[1]) for e in ev["pme"]])
+    recip_ms = np.mean([e[1].elapsed_time(e[2]) for e in ev["pme"]])
+    gather_ms = np.mean([e[2].elapsed_time(e[3]) for e in ev["pme"]])
     t_force_ms = ms_f / max(n_f, 1)
     t_pl_ms = ms_pl / max(n_pl, 1)
     gpairs = npairs / (t_force_ms * 1e-3) / 1e9
@@ -207,9 +227,13 @@ def main():
             "workload": (("solvated96k: 26,000 TIP3P + 60 chains x 300 beads, 98 A cube" if not big else
                           "solvated1m: 340,000 TIP3P + 150 chains x 300 beads, 216 A cube") +
                          ", rc 12 A, switch 10 A, pairlist 13.5 A, PME " + "x".join(map(str, K)) +
-                         " order 4; step = PME frac coords + spread + gather (own stream) + nonbonded fast+slow "
-                         "forces, pairlist + energy/virial every 20 steps, dt 2 fs" +
+                         " order 4; step = " +
+                         ("PME frac coords + spread + reciprocal solve + gather (own stream) + nonbonded "
+                          "fast+slow forces + FP64 exclusion correction" if full else
+                          "PME frac coords + spread + gather (own stream) + nonbonded fast+slow forces") +
+                         ", pairlist + energy/virial every 20 steps, dt 2 fs" +
                          ("; 1/r table mode" if args.force_table else "")),
+            "electrostatics": args.electrostatics,
             "natoms": n,
             "pairlist_every": args.pairlist_every,
             "dt_fs": 2.0,
@@ -218,7 +242,8 @@ def main():
         "autotune": tuned,
         "pairs_in_cutoff": npairs,
         "kernel_ms": {"nonbonded_force": round(t_force_ms, 4), "nonbonded_pairlist": round(t_pl_ms, 4),
-                      "pme_spread": round(float(spread_ms), 4), "pme_gather": round(float(gather_ms), 4)},
+                      "pme_spread": round(float(spread_ms), 4), "pme_gather": round(float(gather_ms), 4),
+                      "pme_recip_solve": round(float(recip_ms), 4) if full else None},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),
@@ -241,8 +266,10 @@ def main():
     }
 
     if not args.no_cpu_baseline:
-        out.update(cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args, ms_step))
+        out.update(cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args, ms_step, pr))
     print(json.dumps(out), flush=True)
+    if pr is not None:
+        pr.close()
     c.close()
 
 
@@ -410,9 +437,10 @@ def run_fit(args):
     fm.close()
 
 
-def cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args, gpu_ms_step):
+def cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args, gpu_ms_step, pr=None):
     """The oracle timed on the host cores on the full system (one force
-    evaluation + one PME spread/gather), and the GPU force error against it."""
+    evaluation: nonbonded, PME spread, reciprocal solve when pr is given,
+    gather, exclusion correction), and the GPU force error against it."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as orc
     from mi355nb import lib as L
@@ -430,26 +458,52 @@ def cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args,
                     nthreads=ncores)
     t1 = time.perf_counter()
     a = atoms.cpu().numpy()
+    vol = float(np.prod(box))
+    q32 = q.astype(np.float32).astype(np.float64)
     g = orc.pme_spread_gpu(a, K, order)
-    orc.pme_gather_gpu(a, K, order, g, recip)
+    if pr is not None:
+        phi, e_rec, _ = orc.pme_recip(g, K, recip, vol, beta, order)
+        f_rec = orc.pme_gather_gpu(a, K, order, phi, recip)
+        e_cor, f_cor, _ = orc.pme_exclusion_correction(pos, q32, s.excl_ptr, s.excl_idx, box, beta, p.cutoff)
+    else:
+        orc.pme_gather_gpu(a, K, order, g, recip)
     t2 = time.perf_counter()
     cpu_step = (t1 - t0) + (t2 - t1)
     cpu_ns_day = 2e-6 * 86400.0 / cpu_step
 
     # GPU forces of one full evaluation vs the FP64 oracle
     c.launch_work(True, True, True, True)
+    ev_x = c.exclusion_correction() if pr is not None else None
     r = c.finish()
 
     def rel(x, y):
         return float(np.sqrt(((x - y) ** 2).sum() / (y ** 2).sum()))
 
-    err = {"fast": rel(r["f"], o["f"]), "slow": rel(r["fs"], o["fs"]),
-           "total": rel(r["f"] + r["fs"], o["f"] + o["fs"]),
+    # with full electrostatics the slow forces carry the exclusion correction
+    o_fs = o["fs"] + f_cor if pr is not None else o["fs"]
+    err = {"fast": rel(r["f"], o["f"]), "slow": rel(r["fs"], o_fs),
+           "total": rel(r["f"] + r["fs"], o["f"] + o_fs),
            "e_vdw": abs(r["e_vdw"] - o["e_vdw"]) / abs(o["e_vdw"]),
            "e_elec": abs(r["e_elec"] - o["e_elec"]) / abs(o["e_elec"]),
            "e_slow": abs(r["e_slow"] - o["e_slow"]) / abs(o["e_slow"]),
            "virial": float(np.linalg.norm(r["virial"] + r["virial_slow"] - o["virial"] - o["virial_slow"])
                            / np.linalg.norm(o["virial"] + o["virial_slow"]))}
+    if pr is not None:
+        # full electrostatics: reciprocal solve + gather on the GPU from the
+        # same slot-ordered atoms, exclusion correction in the slow forces
+        at = torch.from_numpy(a).cuda()
+        gg = torch.zeros(K[0] * K[1] * K[2], dtype=torch.float32, device="cuda")
+        from mi355nb import compute as mc
+        mc.spread_charge(at, gg, K, order)
+        evr = pr.solve(gg, recip, vol, beta)
+        fr = mc.gather_force(at, gg, K, order, recip=recip)
+        torch.cuda.synchronize()
+        sl = c.slot_of_atom()
+        f_gpu = r["f"] + r["fs"] + fr.cpu().numpy()[sl, :3]
+        f_ora = o["f"] + o_fs + f_rec[sl]
+        err["total_full_electrostatics"] = rel(f_gpu, f_ora)
+        err["e_recip"] = abs(float(evr.cpu().numpy()[0]) - e_rec) / abs(e_rec)
+        err["e_exclusion_correction"] = abs(float(ev_x.cpu().numpy()[0, 0]) - e_cor) / abs(e_cor)
     return {
         "force_rel_rms_err": {k: float(f"{v:.3e}") for k, v in err.items()},
         "cpu_baseline": {
@@ -459,7 +513,9 @@ def cpu_baseline_leg(s, c, box, recip, K, order, atoms, grid, fpme, torch, args,
             "kind": "port",
             "sample": f"one full evaluation of the {args.workload} system: FP64 oracle nonbonded (all pairs "
                       "within 12 A, cell list, "
-                      f"{ncores} OpenMP threads) + FP64 PME spread and gather (1 thread); "
+                      f"{ncores} OpenMP threads) + FP64 PME spread" +
+                      (", reciprocal solve (numpy FFT), gather and exclusion correction (1 thread); "
+                       if pr is not None else " and gather (1 thread); ") +
                       f"{cpu_step:.2f} s per step",
         },
     }

@@ -109,8 +109,8 @@ struct mi355nb_compute {
   // PME exclusion correction: excluded pairs in slot space, per-system blocks
   std::vector<int4> exclBlocksHost;
   std::vector<int> exclSysStartHost;
-  DevBuf<int4> exclPairs, exclBlocks;
-  DevBuf<int> exclSysStart;
+  DevBuf<int4> exclBlocks;
+  DevBuf<int> exclSlotPtr, exclSlotIdx, exclSysStart;
   DevBuf<double> exclPartials, sysBoxDev;
   // NAMD table mode (optional)
   bool tableMode = false;
@@ -149,7 +149,8 @@ struct mi355nb_compute {
     dropGraphs();
     alch.release(); part.release(); redBlocks.release(); redSysStart.release();
     forceTable.release(); energyTable.release(); slotPatch.release(); patchCentre.release();
-    exclPairs.release(); exclBlocks.release(); exclSysStart.release(); exclPartials.release(); sysBoxDev.release();
+    exclSlotPtr.release(); exclSlotIdx.release(); exclBlocks.release(); exclSysStart.release();
+    exclPartials.release(); sysBoxDev.release();
     fmTarget.release(); fmLossPartials.release(); fmLoss.release(); fmGq.release(); fmBinPartials.release();
     fmBins.release(); fmG.release();
     xyzq.release(); force.release(); forceSlow.release();
@@ -443,35 +444,35 @@ static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomSt
                           hipMemcpyHostToDevice));
   }
   {
-    // excluded pairs (each once) in slot space, grouped by system: atoms of a
-    // system are contiguous and a pair belongs to the system of its lower atom
+    // excluded partners of every slot, both directions, in slot space (the
+    // molecule's CSR may list a pair once or twice)
     std::vector<long long> keys;
-    keys.reserve(c->exclIdxHost.size());
+    keys.reserve(2 * c->exclIdxHost.size());
     for (int a = 0; a < c->natoms; ++a)
       for (int64_t k = c->exclPtrHost[a]; k < c->exclPtrHost[a + 1]; ++k) {
         const long long b = c->exclIdxHost[k];
-        keys.push_back(std::min<long long>(a, b) * c->natoms + std::max<long long>(a, b));
+        keys.push_back((long long)c->L.slotOfAtom[a] * n + c->L.slotOfAtom[b]);
+        keys.push_back((long long)c->L.slotOfAtom[b] * n + c->L.slotOfAtom[a]);
       }
     std::sort(keys.begin(), keys.end());
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-    std::vector<int4> pairs(keys.size());
-    std::vector<int> pairSys(keys.size());
-    int sys = 0;
+    std::vector<int> ptr(n + 1, 0), idx(keys.size());
     for (size_t k = 0; k < keys.size(); ++k) {
-      const int a = (int)(keys[k] / c->natoms), b = (int)(keys[k] % c->natoms);
-      while (a >= c->L.sysAtomStart[sys + 1]) ++sys;
-      pairs[k] = make_int4(c->L.slotOfAtom[a], c->L.slotOfAtom[b], 0, 0);
-      pairSys[k] = sys;
+      ++ptr[(int)(keys[k] / n) + 1];
+      idx[k] = (int)(keys[k] % n);
     }
-    buildExclusionBlocks(pairSys, nsys, c->exclBlocksHost, c->exclSysStartHost);
+    for (int k = 0; k < n; ++k) ptr[k + 1] += ptr[k];
+    buildExclusionBlocks(c->L.sysSlotStart.data(), nsys, c->exclBlocksHost, c->exclSysStartHost);
     const size_t nb = c->exclBlocksHost.size();
-    MI355NB_HIP(c->exclPairs.reserve(std::max<size_t>(1, pairs.size()), 1.0f));
+    MI355NB_HIP(c->exclSlotPtr.reserve(n + 1, 1.0f));
+    MI355NB_HIP(c->exclSlotIdx.reserve(std::max<size_t>(1, idx.size()), 1.0f));
     MI355NB_HIP(c->exclBlocks.reserve(std::max<size_t>(1, nb), 1.0f));
     MI355NB_HIP(c->exclSysStart.reserve(nsys + 1, 1.0f));
     MI355NB_HIP(c->exclPartials.reserve(std::max<size_t>(1, nb) * EXCL_VALUES_PER_SYSTEM, 1.0f));
     MI355NB_HIP(c->sysBoxDev.reserve(3 * (size_t)nsys, 1.0f));
-    if (!pairs.empty())
-      MI355NB_HIP(hipMemcpy(c->exclPairs.ptr, pairs.data(), pairs.size() * sizeof(int4), hipMemcpyHostToDevice));
+    MI355NB_HIP(hipMemcpy(c->exclSlotPtr.ptr, ptr.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (!idx.empty())
+      MI355NB_HIP(hipMemcpy(c->exclSlotIdx.ptr, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
     if (nb) MI355NB_HIP(hipMemcpy(c->exclBlocks.ptr, c->exclBlocksHost.data(), nb * sizeof(int4), hipMemcpyHostToDevice));
     MI355NB_HIP(hipMemcpy(c->exclSysStart.ptr, c->exclSysStartHost.data(), (nsys + 1) * sizeof(int),
                           hipMemcpyHostToDevice));
@@ -907,8 +908,9 @@ int mi355nb_compute_exclusion_correction(mi355nb_compute* c, double* energyViria
   CHECK_ARG(c, "null compute");
   CHECK_STATE(c->haveLayout && c->lastSlow, "a launch with doSlow must precede the exclusion correction");
   MI355NB_HIP(hipSetDevice(c->device));
-  MI355NB_HIP(launchExclusionCorrection((int)c->exclBlocksHost.size(), c->L.numSystems, c->exclPairs.ptr,
-                                        c->exclBlocks.ptr, c->exclSysStart.ptr, c->xyzq.ptr, c->slotPatch.ptr,
+  MI355NB_HIP(launchExclusionCorrection((int)c->exclBlocksHost.size(), c->L.numSystems, c->exclSlotPtr.ptr,
+                                        c->exclSlotIdx.ptr, c->exclBlocks.ptr, c->exclSysStart.ptr, c->xyzq.ptr,
+                                        c->slotPatch.ptr,
                                         c->patchCentre.ptr, c->sysBoxDev.ptr, c->p.cutoff, c->p.ewaldcof,
                                         c->forceSlow.ptr, c->exclPartials.ptr, energyVirial, c->stream));
   return MI355NB_OK;

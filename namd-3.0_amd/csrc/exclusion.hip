@@ -7,11 +7,14 @@
 // it is evaluated in FP64 (erf and exp of the device library), with the pair
 // displacement taken in FP64 from patch centre + patch-centred position.
 //
-// Work layout: the excluded pairs of a batch are stored by system in slot
-// space; a block of at most EXCL_THREADS pairs of one system is one
-// workgroup.  Forces go to the float4 slow-force output with float atomics;
-// energy and virial (sum f_i (x) r_ij, component a*3+b = f_a r_b) are summed
-// per workgroup and then per system in a fixed order.
+// Work layout: one thread per storage slot walks that atom's excluded
+// partners (a slot-space CSR built when patches are assigned) and adds the
+// summed correction force to its own slot with a plain read-modify-write: no
+// atomics, and the force of every slot is summed in a fixed order.  Each pair
+// is evaluated from both ends (FP64 arithmetic is cheap here), so energy and
+// virial take half of each evaluation.  Blocks of slots never straddle two
+// systems; energy and virial (sum f_i (x) r_ij, component a*3+b = f_a r_b)
+// are summed per workgroup and then per system in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -38,48 +41,57 @@ __device__ __forceinline__ double blockSum256(double v, double* sh) {
 }
 
 __global__ void __launch_bounds__(EXCL_THREADS)
-exclusionCorrectionKernel(const int4* __restrict__ pairs, const int4* __restrict__ blocks,
-                          const F4* __restrict__ xyzq, const int* __restrict__ slotPatch,
-                          const double* __restrict__ centre, const double* __restrict__ sysBox, double cutoff2,
-                          double beta, F4* __restrict__ forceSlow, int doEnergyVirial,
-                          double* __restrict__ partials) {
+exclusionCorrectionKernel(const int* __restrict__ exclPtr, const int* __restrict__ exclSlot,
+                          const int4* __restrict__ blocks, const F4* __restrict__ xyzq,
+                          const int* __restrict__ slotPatch, const double* __restrict__ centre,
+                          const double* __restrict__ sysBox, double cutoff2, double beta, F4* __restrict__ forceSlow,
+                          int doEnergyVirial, double* __restrict__ partials) {
   __shared__ double sh[EXCL_THREADS / 64];
-  const int4 bk = blocks[blockIdx.x];   // {sys, begin, end, -}
+  const int4 bk = blocks[blockIdx.x];   // {sys, begin slot, end slot, -}
   const int k = bk.y + threadIdx.x;
   double v[EXCL_VALUES];
 #pragma unroll
   for (int u = 0; u < EXCL_VALUES; ++u) v[u] = 0.0;
   if (k < bk.z) {
-    const int4 pr = pairs[k];           // {slot i, slot j, -, -}
-    const F4 a = xyzq[pr.x], b = xyzq[pr.y];
-    const int pa = slotPatch[pr.x], pb = slotPatch[pr.y];
-    const double* L = sysBox + 3 * bk.x;
-    double d[3] = {(centre[3 * pa] + (double)a.x) - (centre[3 * pb] + (double)b.x),
-                   (centre[3 * pa + 1] + (double)a.y) - (centre[3 * pb + 1] + (double)b.y),
-                   (centre[3 * pa + 2] + (double)a.z) - (centre[3 * pb + 2] + (double)b.z)};
+    const int m0 = exclPtr[k], m1 = exclPtr[k + 1];
+    if (m0 < m1) {
+      const F4 a = xyzq[k];
+      const int pa = slotPatch[k];
+      const double L[3] = {sysBox[3 * bk.x], sysBox[3 * bk.x + 1], sysBox[3 * bk.x + 2]};
+      const double xa[3] = {centre[3 * pa] + (double)a.x, centre[3 * pa + 1] + (double)a.y,
+                            centre[3 * pa + 2] + (double)a.z};
+      double f[3] = {0.0, 0.0, 0.0};
+      for (int m = m0; m < m1; ++m) {
+        const int j = exclSlot[m];
+        const F4 b = xyzq[j];
+        const int pb = slotPatch[j];
+        double d[3] = {xa[0] - (centre[3 * pb] + (double)b.x), xa[1] - (centre[3 * pb + 1] + (double)b.y),
+                       xa[2] - (centre[3 * pb + 2] + (double)b.z)};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) d[c] -= L[c] * rint(d[c] / L[c]);
-    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    if (r2 < cutoff2 && r2 > 0.0) {
-      const double qq = (double)a.w * (double)b.w;
-      const double r = sqrt(r2);
-      const double rinv = 1.0 / r;
-      const double e = erf(beta * r) * rinv;
-      // E = -qq erf(beta r)/r;  f_i = -dE/dr * d/r
-      const double dedr = -qq * (1.1283791670955126 * beta * exp(-beta * beta * r2) * rinv - e * rinv);
-      const double fr = -dedr * rinv;
-      const double f[3] = {fr * d[0], fr * d[1], fr * d[2]};
-      atomicAdd(&forceSlow[pr.x].x, (float)f[0]);
-      atomicAdd(&forceSlow[pr.x].y, (float)f[1]);
-      atomicAdd(&forceSlow[pr.x].z, (float)f[2]);
-      atomicAdd(&forceSlow[pr.y].x, (float)-f[0]);
-      atomicAdd(&forceSlow[pr.y].y, (float)-f[1]);
-      atomicAdd(&forceSlow[pr.y].z, (float)-f[2]);
-      v[0] = -qq * e;
+        for (int c = 0; c < 3; ++c) d[c] -= L[c] * rint(d[c] / L[c]);
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (r2 < cutoff2 && r2 > 0.0) {
+          const double qq = (double)a.w * (double)b.w;
+          const double r = sqrt(r2);
+          const double rinv = 1.0 / r;
+          const double e = erf(beta * r) * rinv;
+          // E = -qq erf(beta r)/r;  f_k = -dE/dr * d/r
+          const double dedr = -qq * (1.1283791670955126 * beta * exp(-beta * beta * r2) * rinv - e * rinv);
+          const double fr = -dedr * rinv;
 #pragma unroll
-      for (int p = 0; p < 3; ++p)
+          for (int c = 0; c < 3; ++c) f[c] += fr * d[c];
+          v[0] -= 0.5 * qq * e;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) v[1 + 3 * p + q] = f[p] * d[q];
+          for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) v[1 + 3 * p + q] += 0.5 * fr * d[p] * d[q];
+        }
+      }
+      F4 o = forceSlow[k];
+      o.x += (float)f[0];
+      o.y += (float)f[1];
+      o.z += (float)f[2];
+      forceSlow[k] = o;
     }
   }
   if (doEnergyVirial) {
@@ -104,29 +116,26 @@ exclusionFinalKernel(const int* __restrict__ sysBlockStart, const double* __rest
   }
 }
 
-void buildExclusionBlocks(const std::vector<int>& pairSys, int numSystems, std::vector<int4>& blocks,
+void buildExclusionBlocks(const int* sysSlotStart, int numSystems, std::vector<int4>& blocks,
                           std::vector<int>& sysBlockStart) {
   blocks.clear();
   sysBlockStart.assign(numSystems + 1, 0);
-  size_t k = 0;
   for (int s = 0; s < numSystems; ++s) {
     sysBlockStart[s] = (int)blocks.size();
-    size_t e = k;
-    while (e < pairSys.size() && pairSys[e] == s) ++e;
-    for (size_t b = k; b < e; b += EXCL_THREADS)
-      blocks.push_back(make_int4(s, (int)b, (int)std::min(b + EXCL_THREADS, e), 0));
-    k = e;
+    for (int k = sysSlotStart[s]; k < sysSlotStart[s + 1]; k += EXCL_THREADS)
+      blocks.push_back(make_int4(s, k, std::min(k + EXCL_THREADS, sysSlotStart[s + 1]), 0));
   }
   sysBlockStart[numSystems] = (int)blocks.size();
 }
 
-hipError_t launchExclusionCorrection(int numBlocks, int numSystems, const int4* pairs, const int4* blocks,
-                                     const int* sysBlockStart, const F4* xyzq, const int* slotPatch,
-                                     const double* centre, const double* sysBox, double cutoff, double beta,
-                                     F4* forceSlow, double* partials, double* out, hipStream_t s) {
+hipError_t launchExclusionCorrection(int numBlocks, int numSystems, const int* exclPtr, const int* exclSlot,
+                                     const int4* blocks, const int* sysBlockStart, const F4* xyzq,
+                                     const int* slotPatch, const double* centre, const double* sysBox, double cutoff,
+                                     double beta, F4* forceSlow, double* partials, double* out, hipStream_t s) {
   if (numBlocks > 0) {
-    hipLaunchKernelGGL(exclusionCorrectionKernel, dim3(numBlocks), dim3(EXCL_THREADS), 0, s, pairs, blocks, xyzq,
-                       slotPatch, centre, sysBox, cutoff * cutoff, beta, forceSlow, out ? 1 : 0, partials);
+    hipLaunchKernelGGL(exclusionCorrectionKernel, dim3(numBlocks), dim3(EXCL_THREADS), 0, s, exclPtr, exclSlot,
+                       blocks, xyzq, slotPatch, centre, sysBox, cutoff * cutoff, beta, forceSlow, out ? 1 : 0,
+                       partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -73,16 +73,17 @@ hipError_t launchFracCoordsFromSlots(int nslots, const F4* xyzq, const int* slot
                                      const PatchRecord* patches, const double recip[9], const double origin[3],
                                      F4* out, hipStream_t s);
 
-// PME exclusion correction in FP64 (exclusion.hip).  pairs: {slot i, slot j, -, -}
-// grouped by system; blocks: {system, begin, end, -}; out: 10 doubles per
-// system {E, virial[9]} (null: forces only).
+// PME exclusion correction in FP64 (exclusion.hip).  exclPtr/exclSlot: CSR
+// of every slot's excluded partner slots (both directions); blocks: {system,
+// begin slot, end slot, -}; out: 10 doubles per system {E, virial[9]} (null:
+// forces only).
 constexpr int EXCL_VALUES_PER_SYSTEM = 10;
-void buildExclusionBlocks(const std::vector<int>& pairSys, int numSystems, std::vector<int4>& blocks,
+void buildExclusionBlocks(const int* sysSlotStart, int numSystems, std::vector<int4>& blocks,
                           std::vector<int>& sysBlockStart);
-hipError_t launchExclusionCorrection(int numBlocks, int numSystems, const int4* pairs, const int4* blocks,
-                                     const int* sysBlockStart, const F4* xyzq, const int* slotPatch,
-                                     const double* centre, const double* sysBox, double cutoff, double beta,
-                                     F4* forceSlow, double* partials, double* out, hipStream_t s);
+hipError_t launchExclusionCorrection(int numBlocks, int numSystems, const int* exclPtr, const int* exclSlot,
+                                     const int4* blocks, const int* sysBlockStart, const F4* xyzq,
+                                     const int* slotPatch, const double* centre, const double* sysBox, double cutoff,
+                                     double beta, F4* forceSlow, double* partials, double* out, hipStream_t s);
 
 // Force-matching adjoint (adjoint.hip)
 struct AdjointArgsHost {

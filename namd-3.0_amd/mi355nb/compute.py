@@ -230,14 +230,15 @@ class ComputeNonbonded:
                 "pme_atoms")
         return out
 
-    def exclusion_correction(self, energy_virial=True):
+    def exclusion_correction(self, energy_virial=True, out=None):
         """PME exclusion correction (FP64) for the last launch with slow forces:
         adds its forces to the slow forces finish() returns; returns a cuda
-        float64 (numSystems, 10) tensor {E, virial[9]} or None."""
+        float64 (numSystems, 10) tensor {E, virial[9]} (out, if given) or None."""
         import torch
         ev = None
         if energy_virial:
-            ev = torch.empty((self.info()["numSystems"], 10), dtype=torch.float64, device="cuda")
+            ev = out if out is not None else torch.empty((self.info()["numSystems"], 10), dtype=torch.float64,
+                                                         device="cuda")
         L.check(L.lib().mi355nb_compute_exclusion_correction(self._h, ev.data_ptr() if ev is not None else None),
                 "exclusion_correction")
         return ev
