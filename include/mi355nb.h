@@ -273,6 +273,19 @@ int mi355nb_compute_set_timing(mi355nb_compute* c, int enable);
 int mi355nb_compute_pme_atoms(const mi355nb_compute* c, const double recip[9], const double origin[3],
                               float* atoms, void* stream);
 
+/* Device-side position update (the GPU-resident packing of
+ * SequencerCUDAKernel, src/SequencerCUDAKernel.cu:1952-2041): pos is a device
+ * array of FP64 positions (3 per atom, molecule order); every slot gets the
+ * image of its atom closest to its patch centre, rounded to 2^-17 A, exactly
+ * as mi355nb_compute_update_positions does on the host.  Enqueued on the
+ * compute's stream; the patch assignment is kept (atoms must stay within
+ * their patch's pairlist margin until the next assign_patches, as between
+ * NAMD's atom migrations). */
+int mi355nb_compute_update_positions_device(mi355nb_compute* c, const double* pos);
+
+/* Storage slot of every atom of the current layout (host array, natoms). */
+int mi355nb_compute_get_slot_of_atom(const mi355nb_compute* c, int32_t* slotOfAtom);
+
 /* PME exclusion correction (exclusionForce of ComputeBondedCUDA,
  * src/ComputeBondedCUDAKernel.cu:633-772): for every fully excluded pair of
  * the molecule with r^2 < cutoff^2, E += -qq erf(ewaldcof r)/r, evaluated in
@@ -387,6 +400,38 @@ int mi355nb_pme_recip_scalar_sum(mi355nb_pme_recip* pr, const double recip[9], d
  * inverse z transform run as one pass. */
 int mi355nb_pme_recip_solve(mi355nb_pme_recip* pr, const double recip[9], double volume, double kappa,
                             float* grid, int xdim, int doEnergyVirial, double* energyVirial, void* stream);
+
+/* ---- FP64 integration with rigid waters (GPU-resident step) ----
+ * pos, vel: device doubles, 3 per atom; mass/invMass: device doubles per
+ * atom; all enqueued on `stream`. */
+
+/* addForceToMomentum (src/SequencerCUDAKernel.cu:1870-1905, :2471-2515):
+ * vel += halfDt * invMass * (force0 + force1 + force2)[slotOfAtom[a]], the
+ * forces being the pair path's float4-per-slot outputs (any may be NULL). */
+int mi355nb_integrate_kick(int natoms, double halfDt, const double* invMass, const int32_t* slotOfAtom,
+                           const float* force0, const float* force1, const float* force2, double* vel, void* stream);
+
+/* addVelocityToPosition (src/SequencerCUDAKernel.cu:2517-2608): pos += dt *
+ * vel; when posRef is not NULL the positions before the drift are saved
+ * there (the SETTLE reference). */
+int mi355nb_integrate_drift(int natoms, double dt, const double* vel, double* pos, double* posRef, void* stream);
+
+/* settle1 (src/Settle.C:46-290), Miyamoto & Kollman SETTLE for waters whose
+ * atoms are O, H1, H2 at firstAtom[w], +1, +2: pos (unconstrained, after the
+ * drift) is replaced by the rigid geometry (rOH, rHH) that keeps each water's
+ * centre of mass, the displacement taken along the old (posRef) molecular
+ * plane; with vel != NULL, vel += (constrained - unconstrained) / dt. */
+int mi355nb_settle_positions(int nwater, const int32_t* firstAtom, double mO, double mH, double rOH, double rHH,
+                             const double* posRef, double* pos, double* vel, double dt, void* stream);
+
+/* settle2 (src/Settle.C:1420-1486): removes the velocity components that
+ * change the three distances of each water (bond impulses from a 3x3 solve). */
+int mi355nb_settle_velocities(int nwater, const int32_t* firstAtom, double mO, double mH, const double* pos,
+                              double* vel, void* stream);
+
+/* Kinetic energy sum 1/2 m v^2 (FP64, fixed summation order) into *energy
+ * (device double, overwritten). */
+int mi355nb_kinetic_energy(int natoms, const double* mass, const double* vel, double* energy, void* stream);
 
 #ifdef __cplusplus
 }

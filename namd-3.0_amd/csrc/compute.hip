@@ -112,6 +112,7 @@ struct mi355nb_compute {
   DevBuf<int4> exclBlocks;
   DevBuf<int> exclSlotPtr, exclSlotIdx, exclSysStart;
   DevBuf<double> exclPartials, sysBoxDev;
+  DevBuf<int> patchSystemDev;
   // NAMD table mode (optional)
   bool tableMode = false;
   int tableSize = 4096;   // FORCE_ENERGY_TABLE_SIZE of the reference
@@ -150,7 +151,7 @@ struct mi355nb_compute {
     alch.release(); part.release(); redBlocks.release(); redSysStart.release();
     forceTable.release(); energyTable.release(); slotPatch.release(); patchCentre.release();
     exclSlotPtr.release(); exclSlotIdx.release(); exclBlocks.release(); exclSysStart.release();
-    exclPartials.release(); sysBoxDev.release();
+    exclPartials.release(); sysBoxDev.release(); patchSystemDev.release();
     fmTarget.release(); fmLossPartials.release(); fmLoss.release(); fmGq.release(); fmBinPartials.release();
     fmBins.release(); fmG.release();
     xyzq.release(); force.release(); forceSlow.release();
@@ -477,6 +478,9 @@ static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomSt
     MI355NB_HIP(hipMemcpy(c->exclSysStart.ptr, c->exclSysStartHost.data(), (nsys + 1) * sizeof(int),
                           hipMemcpyHostToDevice));
     MI355NB_HIP(hipMemcpy(c->sysBoxDev.ptr, c->L.sysBox.data(), 3 * (size_t)nsys * sizeof(double),
+                          hipMemcpyHostToDevice));
+    MI355NB_HIP(c->patchSystemDev.reserve(std::max<size_t>(1, c->L.patchSystem.size()), 1.0f));
+    MI355NB_HIP(hipMemcpy(c->patchSystemDev.ptr, c->L.patchSystem.data(), c->L.patchSystem.size() * sizeof(int),
                           hipMemcpyHostToDevice));
   }
   MI355NB_HIP(launchFill(c->fsoa.cap, c->fsoa.ptr, 0.0f, c->stream));
@@ -901,6 +905,22 @@ int mi355nb_compute_finish_batch(mi355nb_compute* c, double* forceOut, double* f
   if (vePerSystem)
     MI355NB_HIP(hipMemcpy(vePerSystem, c->ve.ptr, sizeof(VirialEnergy) * c->L.numSystems, hipMemcpyDeviceToHost));
   if (statOut) MI355NB_HIP(hipMemcpy(statOut, c->stat.ptr, sizeof(TileListStat), hipMemcpyDeviceToHost));
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_update_positions_device(mi355nb_compute* c, const double* pos) {
+  CHECK_ARG(c && pos, "null argument");
+  CHECK_STATE(c->haveLayout, "assign_patches must precede update_positions_device");
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(launchPackSlots(c->L.atomStorageSize, c->atomIndex.ptr, c->slotPatch.ptr, c->patchCentre.ptr,
+                              c->patchSystemDev.ptr, c->sysBoxDev.ptr, POS_QUANTUM, pos, c->xyzq.ptr, c->stream));
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_get_slot_of_atom(const mi355nb_compute* c, int32_t* slotOfAtom) {
+  CHECK_ARG(c && slotOfAtom, "null argument");
+  CHECK_STATE(c->haveLayout, "no layout yet");
+  for (int a = 0; a < c->natoms; ++a) slotOfAtom[a] = c->L.slotOfAtom[a];
   return MI355NB_OK;
 }
 

@@ -1,0 +1,329 @@
+// FP64 velocity-Verlet integration with rigid waters for gfx950: the
+// half-step kick and the drift of NAMD's GPU-resident integrator
+// (addForceToMomentum / addVelocityToPosition,
+// src/SequencerCUDAKernel.cu:1870-2608), SETTLE for the water positions and
+// the matching velocity constraint (settle1 / settle2, src/Settle.C:46-290,
+// :1420-1486), the kinetic energy, and the device-side packing of FP64
+// positions into the pair path's patch-centred slots (so a step never leaves
+// the GPU between pairlist rebuilds).
+//
+// Every kernel is one thread per atom (kick, drift, energy), per water
+// (SETTLE) or per slot (packing); positions and velocities are FP64 arrays of
+// 3 doubles per atom in the molecule's atom order.  Forces are read per slot
+// (the pair path's float4 outputs) through slotOfAtom.
+//
+// SETTLE (Miyamoto & Kollman, J. Comput. Chem. 13, 952 (1992)) is written
+// from the paper's construction: the unconstrained water is expressed in a
+// frame built from the old molecular plane, the canonical triangle is
+// rotated by phi, psi (out of plane) and theta (in plane) so that the
+// constrained positions keep the centre of mass of the unconstrained ones and
+// satisfy the three distances exactly.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "../../include/mi355nb.h"
+#include "devutil.h"
+#include "nb_kernels.h"
+#include "nbtypes.h"
+
+namespace mi355nb {
+
+constexpr int INT_THREADS = 256;
+
+__global__ void __launch_bounds__(INT_THREADS)
+kickKernel(int natoms, double halfDt, const double* __restrict__ invMass, const int* __restrict__ slotOfAtom,
+           const F4* __restrict__ f0, const F4* __restrict__ f1, const F4* __restrict__ f2, double* __restrict__ vel) {
+  const int a = blockIdx.x * INT_THREADS + threadIdx.x;
+  if (a >= natoms) return;
+  const int s = slotOfAtom[a];
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if (f0) { const F4 g = f0[s]; fx += g.x; fy += g.y; fz += g.z; }
+  if (f1) { const F4 g = f1[s]; fx += g.x; fy += g.y; fz += g.z; }
+  if (f2) { const F4 g = f2[s]; fx += g.x; fy += g.y; fz += g.z; }
+  const double c = halfDt * invMass[a];
+  vel[3 * a] += c * fx;
+  vel[3 * a + 1] += c * fy;
+  vel[3 * a + 2] += c * fz;
+}
+
+__global__ void __launch_bounds__(INT_THREADS)
+driftKernel(int natoms, double dt, const double* __restrict__ vel, double* __restrict__ pos,
+            double* __restrict__ posRef) {
+  const int k = blockIdx.x * INT_THREADS + threadIdx.x;
+  if (k >= 3 * natoms) return;
+  const double x = pos[k];
+  if (posRef) posRef[k] = x;
+  pos[k] = x + dt * vel[k];
+}
+
+struct V3 { double x, y, z; };
+__device__ __forceinline__ V3 ld3(const double* p) { return V3{p[0], p[1], p[2]}; }
+__device__ __forceinline__ void st3(double* p, V3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 mul(V3 a, double s) { return V3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+  return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ V3 unit(V3 a) { return mul(a, 1.0 / sqrt(dot(a, a))); }
+
+// Rigid-water geometry: masses, and the canonical triangle in its centre-of-
+// mass frame: O at (0, ra, 0), H at (-+rc, -rb, 0).
+struct SettleGeom {
+  double mO, mH;
+  double ra, rb, rc;
+};
+
+SettleGeom makeSettleGeom(double mO, double mH, double rOH, double rHH) {
+  SettleGeom g;
+  g.mO = mO;
+  g.mH = mH;
+  g.rc = 0.5 * rHH;
+  const double h = std::sqrt(rOH * rOH - g.rc * g.rc);   // O to the H-H midpoint
+  // centre of mass on the bisector: mO ra = 2 mH rb, ra + rb = h
+  g.ra = h * 2.0 * mH / (mO + 2.0 * mH);
+  g.rb = h - g.ra;
+  return g;
+}
+
+// Position SETTLE for water w (atoms O, H1, H2 at first[w], +1, +2).
+// ref: constrained positions before the drift; pos: unconstrained positions,
+// overwritten with the constrained ones; vel += (constrained - unconstrained)
+// / dt when invDt != 0.
+__global__ void __launch_bounds__(INT_THREADS)
+settlePositionsKernel(int nwater, const int* __restrict__ first, SettleGeom g, const double* __restrict__ ref,
+                      double* __restrict__ pos, double* __restrict__ vel, double invDt) {
+  const int w = blockIdx.x * INT_THREADS + threadIdx.x;
+  if (w >= nwater) return;
+  const int o = first[w];
+  const V3 A0 = ld3(ref + 3 * o), B0 = ld3(ref + 3 * o + 3), C0 = ld3(ref + 3 * o + 6);
+  const V3 A1 = ld3(pos + 3 * o), B1 = ld3(pos + 3 * o + 3), C1 = ld3(pos + 3 * o + 6);
+  const double M = g.mO + 2.0 * g.mH;
+  // centre of mass of the unconstrained water; positions relative to it
+  const V3 d0 = mul(add(mul(A1, g.mO), mul(add(B1, C1), g.mH)), 1.0 / M);
+  const V3 a1 = sub(A1, d0), b1 = sub(B1, d0), c1 = sub(C1, d0);
+  // frame: ez normal to the old plane, ex perpendicular to ez and a1, ey = ez x ex
+  const V3 b0 = sub(B0, A0), c0 = sub(C0, A0);
+  const V3 ez = unit(cross(b0, c0));
+  const V3 ex = unit(cross(a1, ez));
+  const V3 ey = cross(ez, ex);
+  // old in-plane vectors and new positions in that frame
+  const double xb0 = dot(b0, ex), yb0 = dot(b0, ey);
+  const double xc0 = dot(c0, ex), yc0 = dot(c0, ey);
+  const double za1 = dot(a1, ez);
+  const double xb1 = dot(b1, ex), yb1 = dot(b1, ey), zb1 = dot(b1, ez);
+  const double xc1 = dot(c1, ex), yc1 = dot(c1, ey), zc1 = dot(c1, ez);
+  // out-of-plane tilts: O height fixes phi, the H1-H2 height difference psi
+  const double sinPhi = za1 / g.ra;
+  const double cosPhi = sqrt(fmax(0.0, 1.0 - sinPhi * sinPhi));
+  const double sinPsi = (zb1 - zc1) / (2.0 * g.rc * cosPhi);
+  const double cosPsi = sqrt(fmax(0.0, 1.0 - sinPsi * sinPsi));
+  // canonical triangle after phi and psi (in-plane components)
+  const double ya2 = g.ra * cosPhi;
+  const double xb2 = -g.rc * cosPsi;
+  const double yb2 = -g.rb * cosPhi - g.rc * sinPsi * sinPhi;
+  const double yc2 = -g.rb * cosPhi + g.rc * sinPsi * sinPhi;
+  // in-plane rotation theta: zero net angular momentum change about ez
+  const double alpha = xb2 * (xb0 - xc0) + yb0 * yb2 + yc0 * yc2;
+  const double beta = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2;
+  const double gamma = xb0 * yb1 - xb1 * yb0 + xc0 * yc1 - xc1 * yc0;
+  const double ab2 = alpha * alpha + beta * beta;
+  const double sinTh = (alpha * gamma - beta * sqrt(fmax(0.0, ab2 - gamma * gamma))) / ab2;
+  const double cosTh = sqrt(fmax(0.0, 1.0 - sinTh * sinTh));
+  const double xa3 = -ya2 * sinTh, ya3 = ya2 * cosTh;
+  const double xb3 = xb2 * cosTh - yb2 * sinTh, yb3 = xb2 * sinTh + yb2 * cosTh;
+  const double xc3 = -xb2 * cosTh - yc2 * sinTh, yc3 = -xb2 * sinTh + yc2 * cosTh;
+  const double za3 = g.ra * sinPhi;
+  const double zb3 = -g.rb * sinPhi + g.rc * sinPsi * cosPhi;
+  const double zc3 = -g.rb * sinPhi - g.rc * sinPsi * cosPhi;
+  const V3 A3 = add(d0, add(add(mul(ex, xa3), mul(ey, ya3)), mul(ez, za3)));
+  const V3 B3 = add(d0, add(add(mul(ex, xb3), mul(ey, yb3)), mul(ez, zb3)));
+  const V3 C3 = add(d0, add(add(mul(ex, xc3), mul(ey, yc3)), mul(ez, zc3)));
+  if (invDt != 0.0) {
+    st3(vel + 3 * o, add(ld3(vel + 3 * o), mul(sub(A3, A1), invDt)));
+    st3(vel + 3 * o + 3, add(ld3(vel + 3 * o + 3), mul(sub(B3, B1), invDt)));
+    st3(vel + 3 * o + 6, add(ld3(vel + 3 * o + 6), mul(sub(C3, C1), invDt)));
+  }
+  st3(pos + 3 * o, A3);
+  st3(pos + 3 * o + 3, B3);
+  st3(pos + 3 * o + 6, C3);
+}
+
+// Velocity constraint for water w: the impulses along the three bonds that
+// make every bond length stationary ((v_i - v_j) . r_ij = 0), from the 3x3
+// linear system in the three multipliers (solved by Cramer's rule).
+__global__ void __launch_bounds__(INT_THREADS)
+settleVelocitiesKernel(int nwater, const int* __restrict__ first, SettleGeom g, const double* __restrict__ pos,
+                       double* __restrict__ vel) {
+  const int w = blockIdx.x * INT_THREADS + threadIdx.x;
+  if (w >= nwater) return;
+  const int o = first[w];
+  const V3 A = ld3(pos + 3 * o), B = ld3(pos + 3 * o + 3), C = ld3(pos + 3 * o + 6);
+  V3 vA = ld3(vel + 3 * o), vB = ld3(vel + 3 * o + 3), vC = ld3(vel + 3 * o + 6);
+  // bonds: 0 = O-H1 (A-B), 1 = O-H2 (A-C), 2 = H1-H2 (B-C); r_k = r_i - r_j
+  const V3 r0 = sub(A, B), r1 = sub(A, C), r2 = sub(B, C);
+  const double iO = 1.0 / g.mO, iH = 1.0 / g.mH;
+  // v_i += lam_k r_k / m_i, v_j -= lam_k r_k / m_j; require (v_i - v_j) . r_k = 0
+  const double m00 = (iO + iH) * dot(r0, r0), m01 = iO * dot(r1, r0), m02 = -iH * dot(r2, r0);
+  const double m10 = iO * dot(r0, r1), m11 = (iO + iH) * dot(r1, r1), m12 = iH * dot(r2, r1);
+  const double m20 = -iH * dot(r0, r2), m21 = iH * dot(r1, r2), m22 = 2.0 * iH * dot(r2, r2);
+  const double b0 = -dot(sub(vA, vB), r0), b1 = -dot(sub(vA, vC), r1), b2 = -dot(sub(vB, vC), r2);
+  const double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+  const double l0 = (b0 * (m11 * m22 - m12 * m21) - m01 * (b1 * m22 - m12 * b2) + m02 * (b1 * m21 - m11 * b2)) / det;
+  const double l1 = (m00 * (b1 * m22 - m12 * b2) - b0 * (m10 * m22 - m12 * m20) + m02 * (m10 * b2 - b1 * m20)) / det;
+  const double l2 = (m00 * (m11 * b2 - b1 * m21) - m01 * (m10 * b2 - b1 * m20) + b0 * (m10 * m21 - m11 * m20)) / det;
+  vA = add(vA, mul(add(mul(r0, l0), mul(r1, l1)), iO));
+  vB = add(vB, mul(sub(mul(r2, l2), mul(r0, l0)), iH));
+  vC = sub(vC, mul(add(mul(r1, l1), mul(r2, l2)), iH));
+  st3(vel + 3 * o, vA);
+  st3(vel + 3 * o + 3, vB);
+  st3(vel + 3 * o + 6, vC);
+}
+
+__global__ void __launch_bounds__(INT_THREADS)
+kineticEnergyKernel(int natoms, const double* __restrict__ mass, const double* __restrict__ vel,
+                    double* __restrict__ partials) {
+  __shared__ double sh[INT_THREADS / 64];
+  double e = 0.0;
+  for (int a = blockIdx.x * INT_THREADS + threadIdx.x; a < natoms; a += gridDim.x * INT_THREADS)
+    e += 0.5 * mass[a] * (vel[3 * a] * vel[3 * a] + vel[3 * a + 1] * vel[3 * a + 1] + vel[3 * a + 2] * vel[3 * a + 2]);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) e += __shfl_xor(e, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < INT_THREADS / 64; ++k) s += sh[k];
+    partials[blockIdx.x] = s;
+  }
+}
+
+__global__ void sumPartialsKernel(int n, const double* __restrict__ partials, double scale, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += partials[k];
+    *out = scale * s;
+  }
+}
+
+// Patch-centred positions of every slot from FP64 atom positions: the image
+// of the atom closest to its patch centre, rounded to POS_QUANTUM, exactly as
+// the host packing does; padding slots (atomIndex < 0) are left alone.
+__global__ void __launch_bounds__(INT_THREADS)
+packSlotsKernel(int nslots, const int* __restrict__ atomIndex, const int* __restrict__ slotPatch,
+                const double* __restrict__ centre, const int* __restrict__ patchSystem,
+                const double* __restrict__ sysBox, double quantum, const double* __restrict__ pos,
+                F4* __restrict__ xyzq) {
+  const int s = blockIdx.x * INT_THREADS + threadIdx.x;
+  if (s >= nslots) return;
+  const int a = atomIndex[s];
+  if (a < 0) return;
+  const int p = slotPatch[s];
+  const double* L = sysBox + 3 * patchSystem[p];
+  double q3[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double x = pos[3 * a + d] - centre[3 * p + d];
+    x -= L[d] * rint(x / L[d]);
+    q3[d] = rint(x / quantum) * quantum;
+  }
+  F4 v = xyzq[s];
+  v.x = (float)q3[0];
+  v.y = (float)q3[1];
+  v.z = (float)q3[2];
+  xyzq[s] = v;
+}
+
+hipError_t launchPackSlots(int nslots, const int* atomIndex, const int* slotPatch, const double* centre,
+                           const int* patchSystem, const double* sysBox, double quantum, const double* pos, F4* xyzq,
+                           hipStream_t s) {
+  if (nslots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(packSlotsKernel, dim3((nslots + INT_THREADS - 1) / INT_THREADS), dim3(INT_THREADS), 0, s,
+                     nslots, atomIndex, slotPatch, centre, patchSystem, sysBox, quantum, pos, xyzq);
+  return hipGetLastError();
+}
+
+}  // namespace mi355nb
+
+using namespace mi355nb;
+
+namespace {
+int blocksFor(long long n) { return (int)((n + INT_THREADS - 1) / INT_THREADS); }
+}  // namespace
+
+extern "C" {
+
+int mi355nb_integrate_kick(int natoms, double halfDt, const double* invMass, const int32_t* slotOfAtom,
+                           const float* force0, const float* force1, const float* force2, double* vel, void* stream) {
+  if (natoms < 0 || (natoms > 0 && (!invMass || !slotOfAtom || !vel))) {
+    setLastError("bad kick arguments");
+    return MI355NB_ERR_ARG;
+  }
+  if (natoms == 0) return MI355NB_OK;
+  hipLaunchKernelGGL(kickKernel, dim3(blocksFor(natoms)), dim3(INT_THREADS), 0, (hipStream_t)stream, natoms, halfDt,
+                     invMass, slotOfAtom, reinterpret_cast<const F4*>(force0), reinterpret_cast<const F4*>(force1),
+                     reinterpret_cast<const F4*>(force2), vel);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+int mi355nb_integrate_drift(int natoms, double dt, const double* vel, double* pos, double* posRef, void* stream) {
+  if (natoms < 0 || (natoms > 0 && (!vel || !pos))) {
+    setLastError("bad drift arguments");
+    return MI355NB_ERR_ARG;
+  }
+  if (natoms == 0) return MI355NB_OK;
+  hipLaunchKernelGGL(driftKernel, dim3(blocksFor(3LL * natoms)), dim3(INT_THREADS), 0, (hipStream_t)stream, natoms,
+                     dt, vel, pos, posRef);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+int mi355nb_settle_positions(int nwater, const int32_t* firstAtom, double mO, double mH, double rOH, double rHH,
+                             const double* posRef, double* pos, double* vel, double dt, void* stream) {
+  if (nwater < 0 || (nwater > 0 && (!firstAtom || !posRef || !pos)) || mO <= 0 || mH <= 0 || rOH <= 0 ||
+      rHH <= 0 || rHH >= 2 * rOH || (vel && dt <= 0)) {
+    setLastError("bad SETTLE arguments");
+    return MI355NB_ERR_ARG;
+  }
+  if (nwater == 0) return MI355NB_OK;
+  const SettleGeom g = makeSettleGeom(mO, mH, rOH, rHH);
+  hipLaunchKernelGGL(settlePositionsKernel, dim3(blocksFor(nwater)), dim3(INT_THREADS), 0, (hipStream_t)stream,
+                     nwater, firstAtom, g, posRef, pos, vel, vel ? 1.0 / dt : 0.0);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+int mi355nb_settle_velocities(int nwater, const int32_t* firstAtom, double mO, double mH, const double* pos,
+                              double* vel, void* stream) {
+  if (nwater < 0 || (nwater > 0 && (!firstAtom || !pos || !vel)) || mO <= 0 || mH <= 0) {
+    setLastError("bad SETTLE velocity arguments");
+    return MI355NB_ERR_ARG;
+  }
+  if (nwater == 0) return MI355NB_OK;
+  const SettleGeom g = makeSettleGeom(mO, mH, 1.0, 1.0);
+  hipLaunchKernelGGL(settleVelocitiesKernel, dim3(blocksFor(nwater)), dim3(INT_THREADS), 0, (hipStream_t)stream,
+                     nwater, firstAtom, g, pos, vel);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+int mi355nb_kinetic_energy(int natoms, const double* mass, const double* vel, double* energy, void* stream) {
+  if (natoms < 0 || !energy || (natoms > 0 && (!mass || !vel))) {
+    setLastError("bad kinetic-energy arguments");
+    return MI355NB_ERR_ARG;
+  }
+  static thread_local DevBuf<double> partials;
+  const int nb = std::max(1, std::min(256, blocksFor(natoms)));
+  MI355NB_HIP(partials.reserve(nb, 1.0f));
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(kineticEnergyKernel, dim3(nb), dim3(INT_THREADS), 0, s, natoms, mass, vel, partials.ptr);
+  MI355NB_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sumPartialsKernel, dim3(1), dim3(64), 0, s, nb, partials.ptr, 1.0, energy);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+}  // extern "C"

@@ -85,6 +85,11 @@ hipError_t launchExclusionCorrection(int numBlocks, int numSystems, const int* e
                                      const int* slotPatch, const double* centre, const double* sysBox, double cutoff,
                                      double beta, F4* forceSlow, double* partials, double* out, hipStream_t s);
 
+// FP64 positions (3 per atom) -> patch-centred slot coordinates (integrate.hip)
+hipError_t launchPackSlots(int nslots, const int* atomIndex, const int* slotPatch, const double* centre,
+                           const int* patchSystem, const double* sysBox, double quantum, const double* pos, F4* xyzq,
+                           hipStream_t s);
+
 // Force-matching adjoint (adjoint.hip)
 struct AdjointArgsHost {
   int nslots;

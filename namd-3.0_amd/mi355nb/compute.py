@@ -310,13 +310,15 @@ class ComputeNonbonded:
                     excl=ex[: nj.value], atomIndex=slots)
 
     def slot_of_atom(self):
-        """Storage slot of every atom (padding slots repeat an atom's index
-        after its own slot; the first occurrence is the atom's slot)."""
-        idx = self.tile_lists()["atomIndex"]
-        atoms, first = np.unique(idx, return_index=True)
-        out = np.full(self.natoms, -1, np.int64)
-        out[atoms] = first
+        """Storage slot of every atom (int32, natoms)."""
+        out = np.zeros(self.natoms, np.int32)
+        L.check(L.lib().mi355nb_compute_get_slot_of_atom(self._h, out), "get_slot_of_atom")
         return out
+
+    def update_positions_device(self, pos):
+        """pos: cuda float64 (natoms, 3) tensor; packed into the slots on the compute's stream."""
+        L.check(L.lib().mi355nb_compute_update_positions_device(self._h, pos.data_ptr()),
+                "update_positions_device")
 
 
 def decode_pairs(tl, natoms_storage_valid=None):

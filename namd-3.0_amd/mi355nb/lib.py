@@ -76,6 +76,8 @@ EXPORTED = [
     "mi355nb_compute_get_system_boxes", "mi355nb_pme_b_moduli", "mi355nb_pme_recip_create",
     "mi355nb_pme_recip_destroy", "mi355nb_pme_recip_fft_r2c", "mi355nb_pme_recip_fft_c2r",
     "mi355nb_pme_recip_scalar_sum", "mi355nb_pme_recip_solve", "mi355nb_compute_exclusion_correction",
+    "mi355nb_compute_update_positions_device", "mi355nb_compute_get_slot_of_atom", "mi355nb_integrate_kick",
+    "mi355nb_integrate_drift", "mi355nb_settle_positions", "mi355nb_settle_velocities", "mi355nb_kinetic_energy",
 ]
 
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -136,6 +138,14 @@ def lib():
     L.mi355nb_compute_force_matching_grad.argtypes = [vp, dp, C.POINTER(C.c_double), vp, vp, vp]
     L.mi355nb_pme_b_moduli.argtypes = [C.c_int, C.c_int, dp]
     L.mi355nb_compute_exclusion_correction.argtypes = [vp, vp]
+    L.mi355nb_compute_update_positions_device.argtypes = [vp, vp]
+    L.mi355nb_compute_get_slot_of_atom.argtypes = [vp, i32p]
+    L.mi355nb_integrate_kick.argtypes = [C.c_int, C.c_double, vp, vp, vp, vp, vp, vp, vp]
+    L.mi355nb_integrate_drift.argtypes = [C.c_int, C.c_double, vp, vp, vp, vp]
+    L.mi355nb_settle_positions.argtypes = [C.c_int, vp, C.c_double, C.c_double, C.c_double, C.c_double, vp, vp, vp,
+                                           C.c_double, vp]
+    L.mi355nb_settle_velocities.argtypes = [C.c_int, vp, C.c_double, C.c_double, vp, vp, vp]
+    L.mi355nb_kinetic_energy.argtypes = [C.c_int, vp, vp, vp, vp]
     L.mi355nb_pme_recip_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L.mi355nb_pme_recip_destroy.argtypes = [vp]
     L.mi355nb_pme_recip_fft_r2c.argtypes = [vp, vp, C.c_int, vp, vp]

@@ -1,0 +1,212 @@
+"""NVE molecular dynamics of rigid-water boxes on one GPU: velocity Verlet in
+FP64 with SETTLE, forces from the nonbonded pair path + PME (real space,
+reciprocal solve, self energy, exclusion correction), all through the C ABI.
+
+The step follows NAMD's GPU-resident velocity Verlet (SequencerCUDA:
+addForceToMomentum, addVelocityToPosition, rattle1/settle, the second half
+kick, settle2): positions and velocities stay on the device; between pairlist
+rebuilds the positions are packed into the pair path's slots on the device;
+every `pairlist_every` steps the patches are reassigned on the host (NAMD's
+atom migration) and the pairlists rebuilt.
+
+Units as NAMD's: A, kcal/mol, amu; time in units of TIMEFACTOR fs
+(src/common.h), so that v += dt F / m and KE = 1/2 m v^2 need no factors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import compute as mc
+from . import lib as L
+from . import systems as S
+
+TIMEFACTOR = 48.88821      # fs per internal time unit (NAMD src/common.h)
+BOLTZMANN = 0.001987191    # kcal/mol/K (NAMD src/common.h)
+
+
+class RigidWaterNVE:
+    """NVE of a water-only System (atoms O, H1, H2 per water)."""
+
+    def __init__(self, s: S.System, dt_fs=2.0, temperature=300.0, seed=0, pairlist_every=20, stream=None):
+        import torch
+        if s.n_beads:
+            raise ValueError("RigidWaterNVE integrates water-only systems (no bonded terms are built)")
+        self.torch = torch
+        self.s = s
+        self.n = s.natoms
+        self.nw = s.n_waters
+        self.dt = dt_fs / TIMEFACTOR
+        self.dt_fs = dt_fs
+        self.pairlist_every = int(pairlist_every)
+        self.stream = stream if stream is not None else torch.cuda.Stream()
+        self.sp = self.stream.cuda_stream
+        p = s.params
+        self.beta = S.ewald_coefficient(p.cutoff, p.ewald_tol)
+        self.box = np.asarray(s.box, np.float64)
+        dev = torch.device("cuda")
+        mass = np.tile([S.MASS_O, S.MASS_H, S.MASS_H], self.nw)
+        with torch.cuda.stream(self.stream):
+            self.mass = torch.tensor(mass, dtype=torch.float64, device=dev)
+            self.inv_mass = torch.tensor(1.0 / mass, dtype=torch.float64, device=dev)
+            self.first = torch.tensor(3 * np.arange(self.nw), dtype=torch.int32, device=dev)
+            self.pos = torch.tensor(np.mod(s.pos, self.box), dtype=torch.float64, device=dev).contiguous()
+            self.pos_ref = torch.empty_like(self.pos)
+            self.ev_recip = torch.zeros(7, dtype=torch.float64, device=dev)
+            self.ev_excl = torch.zeros((1, 10), dtype=torch.float64, device=dev)
+            self.ke = torch.zeros(1, dtype=torch.float64, device=dev)
+            K = p.pme_grid
+            self.K = K
+            self.order = p.pme_order
+            self.grid = torch.zeros(K[0] * K[1] * K[2], dtype=torch.float32, device=dev)
+        self.c = mc.ComputeNonbonded.from_system(s, stream=self.sp)
+        self.pr = mc.PmeRecip(K, self.order)
+        self.recip = np.diag(1.0 / self.box)
+        self.volume = float(np.prod(self.box))
+        self._assign()
+        self._init_velocities(temperature, seed)
+        # PME self energy (constant: the charges do not change), from the
+        # float32 scaled charges the kernels use
+        q32 = (s.charge * math.sqrt(mc.COULOMB)).astype(np.float32).astype(np.float64)
+        self.self_energy = -self.beta / math.sqrt(math.pi) * float((q32 ** 2).sum())
+        self.dof = self.dof_of()
+        self._forces(energy=True)
+
+    # --- layout -------------------------------------------------------------
+    def _assign(self):
+        """Reassign patches from the current positions (host) and refresh the
+        slot maps and per-slot buffers."""
+        torch = self.torch
+        self.stream.synchronize()
+        host = np.mod(self.pos.cpu().numpy(), self.box)
+        self.c.assign_patches(self.box, host)
+        n_slots = self.c.info()["atomStorageSize"]
+        with torch.cuda.stream(self.stream):
+            self.slot_of_atom = torch.tensor(self.c.slot_of_atom(), dtype=torch.int32, device="cuda")
+            self.atoms = torch.empty((n_slots, 4), dtype=torch.float32, device="cuda")
+            self.frec = torch.zeros((n_slots, 4), dtype=torch.float32, device="cuda")
+        self.bufs = self.c.device_buffers()
+        self.need_pairlist = True
+
+    def _init_velocities(self, temperature, seed):
+        torch = self.torch
+        rng = np.random.default_rng(seed)
+        m = self.mass.cpu().numpy()
+        v = rng.normal(size=(self.n, 3)) * np.sqrt(BOLTZMANN * temperature / m)[:, None]
+        with torch.cuda.stream(self.stream):
+            self.vel = torch.tensor(v, dtype=torch.float64, device="cuda").contiguous()
+        self._settle_velocities()
+        self.stream.synchronize()
+        v = self.vel.cpu().numpy()
+        v -= (m[:, None] * v).sum(0) / m.sum()            # zero total momentum
+        ke = 0.5 * (m[:, None] * v * v).sum()
+        v *= math.sqrt(0.5 * self.dof_of() * BOLTZMANN * temperature / ke)
+        with torch.cuda.stream(self.stream):
+            self.vel.copy_(torch.from_numpy(v))
+
+    def dof_of(self):
+        return 6 * self.nw - 3
+
+    # --- pieces of a step ---------------------------------------------------
+    def _forces(self, energy):
+        torch = self.torch
+        c = self.c
+        c.launch_work(do_energy=energy, do_virial=energy, do_slow=True, do_pairlist=self.need_pairlist)
+        self.need_pairlist = False
+        c.exclusion_correction(energy_virial=energy, out=self.ev_excl)
+        with torch.cuda.stream(self.stream):
+            c.pme_atoms(self.recip, out=self.atoms, stream=self.sp)
+            self.grid.zero_()
+            mc.spread_charge(self.atoms, self.grid, self.K, self.order, stream=self.sp)
+            self.pr.solve(self.grid, self.recip, self.volume, self.beta, energy_virial=energy, out=self.ev_recip,
+                          stream=self.sp)
+            mc.gather_force(self.atoms, self.grid, self.K, self.order, recip=self.recip, out=self.frec,
+                            stream=self.sp)
+
+    def _kick(self):
+        L.check(L.lib().mi355nb_integrate_kick(self.n, 0.5 * self.dt, self.inv_mass.data_ptr(),
+                                               self.slot_of_atom.data_ptr(), self.bufs["force"],
+                                               self.bufs["forceSlow"], self.frec.data_ptr(), self.vel.data_ptr(),
+                                               self.sp), "integrate_kick")
+
+    def _settle_velocities(self):
+        L.check(L.lib().mi355nb_settle_velocities(self.nw, self.first.data_ptr(), S.MASS_O, S.MASS_H,
+                                                  self.pos.data_ptr(), self.vel.data_ptr(), self.sp),
+                "settle_velocities")
+
+    def step(self, k, energy=False):
+        """One velocity-Verlet step (k counts from 0); returns energies when asked."""
+        lib = L.lib()
+        self._kick()
+        L.check(lib.mi355nb_integrate_drift(self.n, self.dt, self.vel.data_ptr(), self.pos.data_ptr(),
+                                            self.pos_ref.data_ptr(), self.sp), "integrate_drift")
+        L.check(lib.mi355nb_settle_positions(self.nw, self.first.data_ptr(), S.MASS_O, S.MASS_H, S.TIP3P_ROH,
+                                             S.TIP3P_RHH, self.pos_ref.data_ptr(), self.pos.data_ptr(),
+                                             self.vel.data_ptr(), self.dt, self.sp), "settle_positions")
+        if (k + 1) % self.pairlist_every == 0:
+            self._assign()
+        else:
+            self.c.update_positions_device(self.pos)
+        self._forces(energy)
+        self._kick()
+        self._settle_velocities()
+        return self.energies() if energy else None
+
+    def energies(self):
+        """Energies of the current state (after a force evaluation with energies)."""
+        L.check(L.lib().mi355nb_kinetic_energy(self.n, self.mass.data_ptr(), self.vel.data_ptr(), self.ke.data_ptr(),
+                                               self.sp), "kinetic_energy")
+        r = self.c.finish(forces=False)
+        self.stream.synchronize()
+        ke = float(self.ke.cpu().numpy()[0])
+        e_rec = float(self.ev_recip.cpu().numpy()[0])
+        e_exc = float(self.ev_excl.cpu().numpy()[0, 0])
+        pe = r["e_vdw"] + r["e_elec"] + r["e_slow"] + e_rec + self.self_energy + e_exc
+        return dict(kinetic=ke, potential=pe, total=ke + pe, temperature=2.0 * ke / (self.dof * BOLTZMANN),
+                    vdw=r["e_vdw"], elec_real=r["e_elec"] + r["e_slow"], elec_recip=e_rec,
+                    elec_self=self.self_energy, elec_excl=e_exc)
+
+    def equilibrate(self, nsteps, temperature, every=10, start=0):
+        """Velocity rescaling to `temperature` every `every` steps (NAMD's
+        rescaleFreq / rescaleTemp); returns the step count reached."""
+        k = start
+        for _ in range(nsteps):
+            e = self.step(k, energy=((k + 1) % every == 0))
+            k += 1
+            if e is not None and e["temperature"] > 0:
+                with self.torch.cuda.stream(self.stream):
+                    self.vel.mul_(math.sqrt(temperature / e["temperature"]))
+        return k
+
+    def run(self, nsteps, energy_every=10, start=0):
+        out = []
+        for k in range(start, start + nsteps):
+            e = self.step(k, energy=((k + 1) % energy_every == 0))
+            if e is not None:
+                e["step"] = k + 1
+                out.append(e)
+        return out
+
+    def positions(self):
+        self.stream.synchronize()
+        return self.pos.cpu().numpy()
+
+    def close(self):
+        self.pr.close()
+        self.c.close()
+
+
+def drift_fit(records, dt_fs, dof):
+    """Least-squares slope of the total energy: kcal/mol per ns, and per ns per
+    degree of freedom in kT units at the mean temperature; RMS fluctuation of
+    total and kinetic energy."""
+    t = np.array([r["step"] for r in records], np.float64) * dt_fs * 1e-6   # ns
+    e = np.array([r["total"] for r in records])
+    ke = np.array([r["kinetic"] for r in records])
+    temp = float(np.mean([r["temperature"] for r in records]))
+    slope = float(np.polyfit(t, e, 1)[0]) if len(t) > 1 else 0.0
+    return dict(slope_kcal_per_mol_ns=slope,
+                slope_kT_per_ns_per_dof=slope / (dof * BOLTZMANN * temp) if temp > 0 else float("nan"),
+                total_rms=float(np.std(e)), kinetic_rms=float(np.std(ke)), mean_temperature=temp)

@@ -36,6 +36,8 @@ TIP3P_QO = -0.834
 TIP3P_QH = 0.417
 TIP3P_O_EPS, TIP3P_O_RMIN2 = 0.1521, 1.7682
 TIP3P_H_EPS, TIP3P_H_RMIN2 = 0.046, 0.2245
+TIP3P_RHH = 2.0 * TIP3P_ROH * math.sin(0.5 * TIP3P_THETA)
+MASS_O, MASS_H = 15.9994, 1.008
 RMIN_TO_SIGMA = 2.0 / 2.0 ** (1.0 / 6.0)  # sigma = Rmin / 2^(1/6) = 2*(Rmin/2)/2^(1/6)
 
 
@@ -328,6 +330,14 @@ def water_box_9k(seed: int = 1) -> System:
     """
     p = NbParams(cutoff=12.0, switchdist=10.0, pairlistdist=13.5, pme_grid=(32, 32, 32))
     return make_system("water9k", 3000, 0, 0, 31.0, 0, seed, p, lattice_shape=(15, 15, 14), min_contact=1.0)
+
+
+def water_box_liquid(n_side=(15, 15, 14), seed: int = 1, pme_grid=(48, 48, 48)) -> System:
+    """The 9,000-atom TIP3P box at liquid density: the same 3,000-site lattice
+    as water_box_9k, box scaled by 3^(1/3) to 44.71 A (0.1007 atoms/A^3)."""
+    box = round(31.0 * 3.0 ** (1.0 / 3.0), 2)
+    p = NbParams(cutoff=12.0, switchdist=10.0, pairlistdist=13.5, pme_grid=tuple(pme_grid))
+    return make_system("water9k_liquid", 3000, 0, 0, box, 0, seed, p, lattice_shape=tuple(n_side), min_contact=1.6)
 
 
 def solvated_96k(seed: int = 2) -> System:

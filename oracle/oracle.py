@@ -440,3 +440,111 @@ def pme_exclusion_correction(pos, q_scaled, excl_ptr, excl_idx, box, beta, cutof
     np.add.at(F, b, -f)
     W = np.einsum("ka,kb->ab", f, d)
     return E, F, W
+
+
+# ---- rigid water constraints (FP64) -------------------------------------------
+
+def settle_geometry(mO, mH, rOH, rHH):
+    """Canonical triangle in the centre-of-mass frame (settle1init,
+    src/Settle.C:46-61): O at (0, ra), H at (-+rc, -rb)."""
+    rc = 0.5 * rHH
+    h = np.sqrt(rOH * rOH - rc * rc)
+    ra = h * 2.0 * mH / (mO + 2.0 * mH)
+    return ra, h - ra, rc
+
+
+def settle_positions(ref, pos, mO, mH, rOH, rHH):
+    """SETTLE (settle1, src/Settle.C:63-290; Miyamoto & Kollman 1992) for
+    waters given as (nw, 3, 3) arrays [water][O, H1, H2][xyz]: returns the
+    constrained positions."""
+    ra, rb, rc = settle_geometry(mO, mH, rOH, rHH)
+    M = mO + 2 * mH
+    A0, B0, C0 = ref[:, 0], ref[:, 1], ref[:, 2]
+    A1, B1, C1 = pos[:, 0], pos[:, 1], pos[:, 2]
+    d0 = (mO * A1 + mH * (B1 + C1)) / M
+    a1, b1, c1 = A1 - d0, B1 - d0, C1 - d0
+    b0, c0 = B0 - A0, C0 - A0
+
+    def unit(v):
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    ez = unit(np.cross(b0, c0))
+    ex = unit(np.cross(a1, ez))
+    ey = np.cross(ez, ex)
+
+    def dot(u, v):
+        return (u * v).sum(1)
+
+    xb0, yb0, xc0, yc0 = dot(b0, ex), dot(b0, ey), dot(c0, ex), dot(c0, ey)
+    za1 = dot(a1, ez)
+    xb1, yb1, zb1 = dot(b1, ex), dot(b1, ey), dot(b1, ez)
+    xc1, yc1, zc1 = dot(c1, ex), dot(c1, ey), dot(c1, ez)
+    sphi = za1 / ra
+    cphi = np.sqrt(np.maximum(0.0, 1 - sphi ** 2))
+    spsi = (zb1 - zc1) / (2 * rc * cphi)
+    cpsi = np.sqrt(np.maximum(0.0, 1 - spsi ** 2))
+    ya2 = ra * cphi
+    xb2 = -rc * cpsi
+    yb2 = -rb * cphi - rc * spsi * sphi
+    yc2 = -rb * cphi + rc * spsi * sphi
+    al = xb2 * (xb0 - xc0) + yb0 * yb2 + yc0 * yc2
+    be = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2
+    ga = xb0 * yb1 - xb1 * yb0 + xc0 * yc1 - xc1 * yc0
+    ab2 = al * al + be * be
+    sth = (al * ga - be * np.sqrt(np.maximum(0.0, ab2 - ga * ga))) / ab2
+    cth = np.sqrt(np.maximum(0.0, 1 - sth ** 2))
+    out = np.empty_like(pos)
+    comps = [(-ya2 * sth, ya2 * cth, ra * sphi),
+             (xb2 * cth - yb2 * sth, xb2 * sth + yb2 * cth, -rb * sphi + rc * spsi * cphi),
+             (-xb2 * cth - yc2 * sth, -xb2 * sth + yc2 * cth, -rb * sphi - rc * spsi * cphi)]
+    for k, (x, y, z) in enumerate(comps):
+        out[:, k] = d0 + x[:, None] * ex + y[:, None] * ey + z[:, None] * ez
+    return out
+
+
+def shake_positions(ref, pos, mO, mH, rOH, rHH, tol=1e-14, iters=10000):
+    """Iterative SHAKE (Ryckaert et al. 1977) for the same rigid waters:
+    corrections along the reference bond vectors, mass weighted, until every
+    distance matches to tol.  An independent solver of the problem SETTLE
+    solves in closed form."""
+    x = pos.copy()
+    m = np.array([mO, mH, mH])
+    bonds = [(0, 1, rOH), (0, 2, rOH), (1, 2, rHH)]
+    for _ in range(iters):
+        worst = 0.0
+        for i, j, d in bonds:
+            rij = x[:, i] - x[:, j]
+            r0 = ref[:, i] - ref[:, j]
+            diff = (rij * rij).sum(1) - d * d
+            worst = max(worst, np.abs(diff).max())
+            g = diff / (2.0 * (1 / m[i] + 1 / m[j]) * (rij * r0).sum(1))
+            x[:, i] -= (g / m[i])[:, None] * r0
+            x[:, j] += (g / m[j])[:, None] * r0
+        if worst < tol:
+            break
+    return x
+
+
+def settle_velocities(pos, vel, mO, mH):
+    """settle2 (src/Settle.C:1420-1486) restated: bond impulses making the
+    three distances of each water stationary, from the 3x3 system solved by
+    numpy.  pos, vel: (nw, 3, 3)."""
+    m = np.array([mO, mH, mH])
+    bonds = [(0, 1), (0, 2), (1, 2)]
+    out = vel.copy()
+    for w in range(pos.shape[0]):
+        r = [pos[w, i] - pos[w, j] for i, j in bonds]
+        Mx = np.zeros((3, 3))
+        b = np.zeros(3)
+        for k, (i, j) in enumerate(bonds):
+            b[k] = -np.dot(vel[w, i] - vel[w, j], r[k])
+            for l, (p, q) in enumerate(bonds):
+                # d(v_i - v_j)/d lam_l . r_k
+                ci = (1 / m[i] if p == i else 0) - (1 / m[i] if q == i else 0)
+                cj = (1 / m[j] if p == j else 0) - (1 / m[j] if q == j else 0)
+                Mx[k, l] = (ci - cj) * np.dot(r[l], r[k])
+        lam = np.linalg.solve(Mx, b)
+        for l, (p, q) in enumerate(bonds):
+            out[w, p] += lam[l] * r[l] / m[p]
+            out[w, q] -= lam[l] * r[l] / m[q]
+    return out
