@@ -62,10 +62,14 @@ def main():
     ap.add_argument("--no-overlap", action="store_true", help="run PME and nonbonded work on one stream")
     ap.add_argument("--wg-per-cu", type=int, default=0,
                     help="fix the pair kernel's persistent workgroups per CU instead of autotuning")
-    ap.add_argument("--workload", choices=("solvated96k", "solvated1m", "batch64", "fit23k"), default="solvated96k",
+    ap.add_argument("--workload", choices=("solvated96k", "solvated1m", "batch64", "fit23k", "nve9k"),
+                    default="solvated96k",
                     help="solvated96k (default): the 96k-atom step; solvated1m: the same step on 1.07M atoms; "
                          "batch64: 64 uneven alchemical systems in one launch against the same systems run one "
-                         "at a time; fit23k: force-matching loss + gradient on 16 frames of the 23k box")
+                         "at a time; fit23k: force-matching loss + gradient on 16 frames of the 23k box; nve9k: "
+                         "rigid-water NVE (FP64 velocity Verlet + SETTLE) on the 9,000-atom box at liquid density, "
+                         "energy drift, and the 3x-density box as specified")
+    ap.add_argument("--nve-equil", type=int, default=2000, help="nve9k: velocity-rescaling steps before NVE")
     ap.add_argument("--force-table", action="store_true", help="NAMD's 1/r table mode instead of direct math")
     ap.add_argument("--electrostatics", choices=("full", "realspace"), default="full",
                     help="full (default): PME real space + reciprocal solve + exclusion correction; realspace: "
@@ -90,6 +94,9 @@ def main():
         return
     if args.workload == "fit23k":
         run_fit(args)
+        return
+    if args.workload == "nve9k":
+        run_nve(args)
         return
     dev = torch.device("cuda:0")
     # nonbonded on one stream, PME real space on another (they are independent
@@ -387,6 +394,84 @@ def run_batch(args):
     for cs in singles:
         cs.close()
     c.close()
+
+
+def run_nve(args):
+    """Rigid-water NVE: the timed steps are complete MD steps (half kick,
+    drift, SETTLE, position packing or patch reassignment every 20 steps,
+    nonbonded + PME forces with energies every 20 steps, half kick, velocity
+    SETTLE); the drift is fitted over the same run."""
+    import torch
+    from mi355nb import md
+    from mi355nb import systems
+
+    dt = 2.0
+    s = systems.water_box_liquid()
+    log(f"nve9k: liquid-density box ({s.natoms} atoms, {s.box[0]} A), equilibrate {args.nve_equil} steps")
+    sim = md.RigidWaterNVE(s, dt_fs=dt, temperature=300.0, seed=1)
+    k = sim.equilibrate(args.nve_equil, 300.0, every=10)
+    for _ in range(args.warmup):
+        sim.step(k)
+        k += 1
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} NVE steps")
+    t0 = time.perf_counter()
+    rec = sim.run(args.steps, energy_every=20, start=k)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    fit = md.drift_fit(rec, dt, sim.dof)
+    sim.close()
+
+    # the 9,000-atom box as specified (3x liquid density): reported as it comes out
+    log("nve9k: the 3x-density box as specified, 1000 NVE steps from 300 K")
+    a = systems.water_box_9k()
+    simA = md.RigidWaterNVE(a, dt_fs=dt, temperature=300.0, seed=1)
+    recA = []
+    outcome = "completed"
+    for j in range(1000):
+        try:
+            e = simA.step(j, energy=((j + 1) % 50 == 0))
+        except Exception as exc:       # a library error (e.g. a non-finite position) ends the run
+            outcome = f"stopped at step {j + 1}: {exc}"
+            break
+        if e is not None:
+            e["step"] = j + 1
+            recA.append(e)
+            if not np.isfinite(e["total"]):
+                outcome = f"non-finite energy at step {j + 1}"
+                break
+    simA.close()
+    a_summary = {"outcome": outcome,
+                 "temperature_first_last_K": [round(recA[0]["temperature"], 1), round(recA[-1]["temperature"], 1)]
+                 if recA else None,
+                 "total_energy_first_last": [round(recA[0]["total"], 2), round(recA[-1]["total"], 2)]
+                 if recA else None}
+
+    out = {
+        "metric": "ns/day of a complete rigid-water NVE step (FP64 integration, full electrostatics); "
+                  "energy drift",
+        "value": round(2e-6 * 86400.0 / (ms_step * 1e-3), 2),
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 forces, fp64 integration",
+        "data": "synthetic",
+        "config": {"workload": "nve9k: 3,000 rigid TIP3P waters at liquid density, 44.71 A cube, rc 12 A, "
+                               "PME 48^3 order 4, dt 2 fs, SETTLE, pairlist + patch reassignment every 20 steps, "
+                               f"after {args.nve_equil} steps of velocity rescaling to 300 K",
+                   "natoms": s.natoms},
+        "nve_drift": {k2: (round(v, 6) if isinstance(v, float) else v) for k2, v in fit.items()},
+        "nve_steps": args.steps,
+        "nve_total_over_kinetic_rms": round(fit["total_rms"] / fit["kinetic_rms"], 5),
+        "config_a_3x_density": a_summary,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def run_fit(args):
