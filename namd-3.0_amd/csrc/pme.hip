@@ -132,10 +132,26 @@ spreadTiledKernel(const F4* __restrict__ atoms, int natoms, int K1, int K2, int 
       if (2 * u > K[d]) u -= K[d];
       if (2 * u < -K[d]) u += K[d];
       s_u[d][tid] = u;
-      atomicMin(&s_lo[d], u);
-      atomicMax(&s_hi[d], u);
 #pragma unroll
       for (int k = 0; k < ORDER; ++k) s_t[d][tid * ORDER + k] = (d == 0 ? at.w : 1.0f) * th[k];
+    }
+  }
+  // group bounds: a wave-level min/max, then one LDS atomic per wave
+  {
+    const bool live = tid < na;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int lo = live ? s_u[d][tid] : 0x7fffffff;
+      int hi = live ? s_u[d][tid] : -0x7fffffff;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+      }
+      if ((tid & 63) == 0) {
+        atomicMin(&s_lo[d], lo);
+        atomicMax(&s_hi[d], hi);
+      }
     }
   }
   __syncthreads();
@@ -156,15 +172,28 @@ spreadTiledKernel(const F4* __restrict__ atoms, int natoms, int K1, int K2, int 
         }
     }
     __syncthreads();
+    // p -> (px, py, pz) by multiply-high division (exact: p * e < 2^32;
+    // e >= ORDER); the unwrapped index lies in [-K/2, 2.5K) since K >= ORDER,
+    // so one add and two subtracts bring it into [0, K)
+    const unsigned m0 = 0xFFFFFFFFu / (unsigned)e0 + 1u, m1 = 0xFFFFFFFFu / (unsigned)e1 + 1u;
+    const int ox = s_ref[0] + s_lo[0], oy = s_ref[1] + s_lo[1], oz = s_ref[2] + s_lo[2];
     for (int p = tid; p < total; p += SP_ATOMS) {
       const float v = tile[p];
       if (v == 0.0f) continue;
-      int x = s_ref[0] + s_lo[0] + p % e0;
-      int y = s_ref[1] + s_lo[1] + (p / e0) % e1;
-      int z = s_ref[2] + s_lo[2] + p / (e0 * e1);
-      x %= K1; if (x < 0) x += K1;
-      y %= K2; if (y < 0) y += K2;
-      z %= K3; if (z < 0) z += K3;
+      const int q = (int)__umulhi((unsigned)p, m0);
+      const int pz = (int)__umulhi((unsigned)q, m1);
+      int x = ox + (p - q * e0);
+      int y = oy + (q - pz * e1);
+      int z = oz + pz;
+      if (x < 0) x += K1;
+      if (x >= K1) x -= K1;
+      if (x >= K1) x -= K1;
+      if (y < 0) y += K2;
+      if (y >= K2) y -= K2;
+      if (y >= K2) y -= K2;
+      if (z < 0) z += K3;
+      if (z >= K3) z -= K3;
+      if (z >= K3) z -= K3;
       atomicAdd(&grid[x + (size_t)xdim * (y + (size_t)K2 * z)], v);
     }
   } else {
@@ -222,23 +251,49 @@ gatherForceKernel(const F4* __restrict__ atoms, int natoms, int K1, int K2, int 
   __syncthreads();
   const int na = min(64, natoms - base);
   constexpr int P = ORDER * ORDER * ORDER;
-  for (int k = 0; k < na; ++k) {
-    float f1 = 0.f, f2 = 0.f, f3 = 0.f;
-    for (int p = lane; p < P; p += 64) {
-      const int x0 = p % ORDER, y0 = (p / ORDER) % ORDER, z0 = p / (ORDER * ORDER);
-      int x = s_i[0][k] + x0; if (x >= K1) x -= K1;
-      int y = s_i[1][k] + y0; if (y >= K2) y -= K2;
-      int z = s_i[2][k] + z0; if (z >= K3) z -= K3;
-      const float Q = grid[x + (size_t)xdim * (y + (size_t)K2 * z)];
-      const float tx = s_t[0][k * ORDER + x0], ty = s_t[1][k * ORDER + y0], tz = s_t[2][k * ORDER + z0];
-      f1 += s_d[0][k * ORDER + x0] * ty * tz * Q;
-      f2 += tx * s_d[1][k * ORDER + y0] * tz * Q;
-      f3 += tx * ty * s_d[2][k * ORDER + z0] * Q;
+  constexpr int PL = (P + 63) / 64;   // stencil points per lane
+  // GB atoms at a time: all their grid loads are issued before any is used,
+  // so one memory latency is spent per GB atoms instead of per atom
+  constexpr int GB = 8;
+  for (int k0 = 0; k0 < na; k0 += GB) {
+    float Q[GB][PL];
+#pragma unroll
+    for (int u = 0; u < GB; ++u) {
+      const int k = min(k0 + u, na - 1);
+#pragma unroll
+      for (int r = 0; r < PL; ++r) {
+        const int p = lane + 64 * r;
+        Q[u][r] = 0.0f;
+        if (p < P) {
+          const int x0 = p % ORDER, y0 = (p / ORDER) % ORDER, z0 = p / (ORDER * ORDER);
+          int x = s_i[0][k] + x0; if (x >= K1) x -= K1;
+          int y = s_i[1][k] + y0; if (y >= K2) y -= K2;
+          int z = s_i[2][k] + z0; if (z >= K3) z -= K3;
+          Q[u][r] = grid[x + (size_t)xdim * (y + (size_t)K2 * z)];
+        }
+      }
     }
-    f1 = waveSum64(f1);
-    f2 = waveSum64(f2);
-    f3 = waveSum64(f3);
-    if (lane == 0) { s_f[0][k] = f1; s_f[1][k] = f2; s_f[2][k] = f3; }
+#pragma unroll
+    for (int u = 0; u < GB; ++u) {
+      const int k = k0 + u;
+      if (k >= na) break;
+      float f1 = 0.f, f2 = 0.f, f3 = 0.f;
+#pragma unroll
+      for (int r = 0; r < PL; ++r) {
+        const int p = lane + 64 * r;
+        if (p < P) {
+          const int x0 = p % ORDER, y0 = (p / ORDER) % ORDER, z0 = p / (ORDER * ORDER);
+          const float tx = s_t[0][k * ORDER + x0], ty = s_t[1][k * ORDER + y0], tz = s_t[2][k * ORDER + z0];
+          f1 += s_d[0][k * ORDER + x0] * ty * tz * Q[u][r];
+          f2 += tx * s_d[1][k * ORDER + y0] * tz * Q[u][r];
+          f3 += tx * ty * s_d[2][k * ORDER + z0] * Q[u][r];
+        }
+      }
+      f1 = waveSum64(f1);
+      f2 = waveSum64(f2);
+      f3 = waveSum64(f3);
+      if (lane == 0) { s_f[0][k] = f1; s_f[1][k] = f2; s_f[2][k] = f3; }
+    }
   }
   __syncthreads();
   if (a < natoms) {
