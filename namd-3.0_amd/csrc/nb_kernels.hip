@@ -415,7 +415,9 @@ nonbondedForceKernel(const ForceArgs args) {
       }
       sjf[i] = F4{0.f, 0.f, 0.f, 0.f};
       if (doSlow) sjfs[i] = F4{0.f, 0.f, 0.f, 0.f};
-      const unsigned mask = nMask;
+      // mask bit j of lane i, rotated so that bit t is step t's j = (i + t + 16h) mod 32
+      const int rot = (i + 16 * h) & 31;
+      unsigned mrot = rot ? ((nMask >> rot) | (nMask << (32 - rot))) : nMask;
       // alchemical kernels: visits with no alchemical atom on either side
       // (the common case) take the plain pair math (wave-uniform branch)
       bool visitAlch = false;
@@ -440,7 +442,8 @@ nonbondedForceKernel(const ForceArgs args) {
         const float dy = xj.y - xi.y;
         const float dz = xj.z - xi.z;
         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        const bool active = ((mask >> j) & 1u) && within(dx, dy, dz, r2, args.cut);
+        const bool active = (mrot & 1u) && within(dx, dy, dz, r2, args.cut);
+        mrot >>= 1;
         if (active) {
           float f, fs = 0.f;
           if constexpr (mode == PAIR_ALCH) {
