@@ -406,10 +406,37 @@ int mi355nb_pme_recip_solve(mi355nb_pme_recip* pr, const double recip[9], double
  * atom; all enqueued on `stream`. */
 
 /* addForceToMomentum (src/SequencerCUDAKernel.cu:1870-1905, :2471-2515):
- * vel += halfDt * invMass * (force0 + force1 + force2)[slotOfAtom[a]], the
- * forces being the pair path's float4-per-slot outputs (any may be NULL). */
+ * vel += halfDt * invMass * ((force0 + force1 + force2)[slotOfAtom[a]] +
+ * forceAtom[a]): the pair path's float4-per-slot outputs and a FP64 per-atom
+ * force (3 doubles, e.g. the bonded terms); any of them may be NULL. */
 int mi355nb_integrate_kick(int natoms, double halfDt, const double* invMass, const int32_t* slotOfAtom,
-                           const float* force0, const float* force1, const float* force2, double* vel, void* stream);
+                           const float* force0, const float* force1, const float* force2, const double* forceAtom,
+                           double* vel, void* stream);
+
+/* One capped steepest-descent move (NAMD's minimizer moves along the force
+ * with a step limit, src/Sequencer.C minimize): posRef = pos, then each atom
+ * moves along its total force F (same inputs as the kick) by
+ * min(gamma |F|, maxStep).  A caller follows it with SETTLE (posRef as the
+ * reference) to keep waters rigid. */
+int mi355nb_minimize_step(int natoms, double gamma, double maxStep, const int32_t* slotOfAtom, const float* force0,
+                          const float* force1, const float* force2, const double* forceAtom, double* pos,
+                          double* posRef, void* stream);
+
+/* ---- Harmonic bonds and angles (ComputeBondedCUDA bondForce / angleForce,
+ * src/ComputeBondedCUDAKernel.cu:194-280, :775-900), CHARMM form
+ * E = k (r - r0)^2, E = k (theta - theta0)^2, FP64 ----
+ * bondAtoms: 2 ints per bond, angleAtoms: 3 per angle (vertex in the middle),
+ * theta0 in radians.  Host arrays; the object owns the device copies. */
+typedef struct mi355nb_bonded mi355nb_bonded;
+int mi355nb_bonded_create(int natoms, int nbonds, const int32_t* bondAtoms, const double* bondK, const double* bondR0,
+                          int nangles, const int32_t* angleAtoms, const double* angleK, const double* angleTheta0,
+                          mi355nb_bonded** out);
+int mi355nb_bonded_destroy(mi355nb_bonded* b);
+/* force (device, 3 doubles per atom) is overwritten with the bonded forces of
+ * pos (device FP64, minimum image in the orthorhombic box); energy (device, 2
+ * doubles {bond, angle}, overwritten) may be NULL. */
+int mi355nb_bonded_forces(mi355nb_bonded* b, const double* pos, const double box[3], double* force, double* energy,
+                          void* stream);
 
 /* addVelocityToPosition (src/SequencerCUDAKernel.cu:2517-2608): pos += dt *
  * vel; when posRef is not NULL the positions before the drift are saved

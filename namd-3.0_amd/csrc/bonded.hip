@@ -1,0 +1,228 @@
+// Harmonic bonds and angles of the chains, FP64, for gfx950: NAMD's
+// bondForce / angleForce of ComputeBondedCUDA
+// (src/ComputeBondedCUDAKernel.cu:194-280, :775-900) in CHARMM form,
+//   bond  E = k (r - r0)^2,   angle  E = k (theta - theta0)^2,
+// without Urey-Bradley, alchemical scaling or the lower-bound bond form.
+//
+// Work layout: one thread per atom walks the bonded terms the atom takes part
+// in (a per-atom list built on the host) and sums the force on itself, so
+// nothing is accumulated with atomics and every force is summed in a fixed
+// order; each term's energy is added by its first atom only.  Positions and
+// forces are FP64, 3 per atom, in the molecule's atom order; displacements
+// use the minimum image of an orthorhombic box.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/mi355nb.h"
+#include "devutil.h"
+
+namespace mi355nb {
+
+constexpr int BD_THREADS = 256;
+
+struct BondTerm { int i, j; double k, r0; };
+struct AngleTerm { int i, j, k; int pad; double kTheta, theta0; };
+
+__device__ __forceinline__ void minImage(double* d, const double* L) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) d[c] -= L[c] * rint(d[c] / L[c]);
+}
+
+// term list entry: (kind << 30) | index, kind 0 = bond, 1 = angle
+__global__ void __launch_bounds__(BD_THREADS)
+bondedKernel(int natoms, const int* __restrict__ termPtr, const int* __restrict__ termIdx,
+             const BondTerm* __restrict__ bonds, const AngleTerm* __restrict__ angles, double Lx, double Ly,
+             double Lz, const double* __restrict__ pos, double* __restrict__ force, double* __restrict__ partials) {
+  __shared__ double sh[2][BD_THREADS / 64];
+  const int a = blockIdx.x * BD_THREADS + threadIdx.x;
+  const double L[3] = {Lx, Ly, Lz};
+  double eb = 0.0, ea = 0.0;
+  if (a < natoms) {
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int m = termPtr[a]; m < termPtr[a + 1]; ++m) {
+      const int e = termIdx[m];
+      const int kind = e >> 30, idx = e & 0x3fffffff;
+      if (kind == 0) {
+        const BondTerm b = bonds[idx];
+        double d[3];
+        for (int c = 0; c < 3; ++c) d[c] = pos[3 * b.i + c] - pos[3 * b.j + c];
+        minImage(d, L);
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double db = r - b.r0;
+        // force on i: -2 k db d / r; on j the opposite
+        const double g = (a == b.i ? -2.0 : 2.0) * b.k * db / r;
+        for (int c = 0; c < 3; ++c) f[c] += g * d[c];
+        if (a == b.i) eb += b.k * db * db;
+      } else {
+        const AngleTerm t = angles[idx];
+        double u[3], v[3];
+        for (int c = 0; c < 3; ++c) {
+          u[c] = pos[3 * t.i + c] - pos[3 * t.j + c];
+          v[c] = pos[3 * t.k + c] - pos[3 * t.j + c];
+        }
+        minImage(u, L);
+        minImage(v, L);
+        const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        const double ru = sqrt(uu), rv = sqrt(vv);
+        double cs = (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / (ru * rv);
+        cs = fmin(1.0, fmax(-1.0, cs));
+        const double th = acos(cs);
+        const double sn = fmax(sqrt(fmax(0.0, 1.0 - cs * cs)), 1e-8);
+        const double dE = 2.0 * t.kTheta * (th - t.theta0);
+        // d theta / d u = -(v/(|u||v|) - cos u/|u|^2) / sin, likewise for v
+        double fi[3], fk[3];
+        for (int c = 0; c < 3; ++c) {
+          fi[c] = dE / sn * (v[c] / (ru * rv) - cs * u[c] / uu);
+          fk[c] = dE / sn * (u[c] / (ru * rv) - cs * v[c] / vv);
+        }
+        if (a == t.i) {
+          for (int c = 0; c < 3; ++c) f[c] += fi[c];
+          const double dt = th - t.theta0;
+          ea += t.kTheta * dt * dt;
+        } else if (a == t.k) {
+          for (int c = 0; c < 3; ++c) f[c] += fk[c];
+        } else {
+          for (int c = 0; c < 3; ++c) f[c] -= fi[c] + fk[c];
+        }
+      }
+    }
+    for (int c = 0; c < 3; ++c) force[3 * a + c] = f[c];
+  }
+  if (partials) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      eb += __shfl_xor(eb, o, 64);
+      ea += __shfl_xor(ea, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      sh[0][threadIdx.x >> 6] = eb;
+      sh[1][threadIdx.x >> 6] = ea;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+      double s = 0.0;
+      for (int w = 0; w < BD_THREADS / 64; ++w) s += sh[threadIdx.x][w];
+      partials[2 * blockIdx.x + threadIdx.x] = s;
+    }
+  }
+}
+
+__global__ void bondedEnergyFinalKernel(int nb, const double* __restrict__ partials, double* __restrict__ out) {
+  if (threadIdx.x < 2) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += partials[2 * b + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+}  // namespace mi355nb
+
+using namespace mi355nb;
+
+struct mi355nb_bonded {
+  int natoms = 0;
+  DevBuf<int> termPtr, termIdx;
+  DevBuf<BondTerm> bonds;
+  DevBuf<AngleTerm> angles;
+  DevBuf<double> partials;
+};
+
+extern "C" {
+
+int mi355nb_bonded_create(int natoms, int nbonds, const int32_t* bondAtoms, const double* bondK, const double* bondR0,
+                          int nangles, const int32_t* angleAtoms, const double* angleK, const double* angleTheta0,
+                          mi355nb_bonded** out) {
+  if (!out || natoms <= 0 || nbonds < 0 || nangles < 0 || (nbonds && (!bondAtoms || !bondK || !bondR0)) ||
+      (nangles && (!angleAtoms || !angleK || !angleTheta0))) {
+    setLastError("bad bonded-term arguments");
+    return MI355NB_ERR_ARG;
+  }
+  *out = nullptr;
+  std::vector<BondTerm> b(nbonds);
+  std::vector<AngleTerm> t(nangles);
+  std::vector<std::vector<int>> per(natoms);
+  for (int k = 0; k < nbonds; ++k) {
+    const int i = bondAtoms[2 * k], j = bondAtoms[2 * k + 1];
+    if (i < 0 || j < 0 || i >= natoms || j >= natoms || i == j || !(bondR0[k] >= 0)) {
+      setLastError("bond term with a bad atom index or length");
+      return MI355NB_ERR_ARG;
+    }
+    b[k] = BondTerm{i, j, bondK[k], bondR0[k]};
+    per[i].push_back(k);
+    per[j].push_back(k);
+  }
+  for (int k = 0; k < nangles; ++k) {
+    const int i = angleAtoms[3 * k], j = angleAtoms[3 * k + 1], m = angleAtoms[3 * k + 2];
+    if (i < 0 || j < 0 || m < 0 || i >= natoms || j >= natoms || m >= natoms || i == j || j == m || i == m) {
+      setLastError("angle term with a bad atom index");
+      return MI355NB_ERR_ARG;
+    }
+    t[k] = AngleTerm{i, j, m, 0, angleK[k], angleTheta0[k]};
+    const int e = (1 << 30) | k;
+    per[i].push_back(e);
+    per[j].push_back(e);
+    per[m].push_back(e);
+  }
+  std::vector<int> ptr(natoms + 1, 0), idx;
+  for (int a = 0; a < natoms; ++a) {
+    ptr[a + 1] = ptr[a] + (int)per[a].size();
+    idx.insert(idx.end(), per[a].begin(), per[a].end());
+  }
+  auto* h = new mi355nb_bonded();
+  h->natoms = natoms;
+  const int nb = (natoms + BD_THREADS - 1) / BD_THREADS;
+  hipError_t e = h->termPtr.reserve(natoms + 1, 1.0f);
+  if (e == hipSuccess) e = h->termIdx.reserve(std::max<size_t>(1, idx.size()), 1.0f);
+  if (e == hipSuccess) e = h->bonds.reserve(std::max(1, nbonds), 1.0f);
+  if (e == hipSuccess) e = h->angles.reserve(std::max(1, nangles), 1.0f);
+  if (e == hipSuccess) e = h->partials.reserve(2 * (size_t)nb, 1.0f);
+  if (e == hipSuccess) e = hipMemcpy(h->termPtr.ptr, ptr.data(), (natoms + 1) * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !idx.empty())
+    e = hipMemcpy(h->termIdx.ptr, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nbonds) e = hipMemcpy(h->bonds.ptr, b.data(), nbonds * sizeof(BondTerm), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nangles)
+    e = hipMemcpy(h->angles.ptr, t.data(), nangles * sizeof(AngleTerm), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    setLastError(std::string("bonded-term upload: ") + hipGetErrorString(e));
+    mi355nb_bonded_destroy(h);
+    return -(int)e - 1000;
+  }
+  *out = h;
+  return MI355NB_OK;
+}
+
+int mi355nb_bonded_destroy(mi355nb_bonded* b) {
+  if (!b) return MI355NB_OK;
+  b->termPtr.release();
+  b->termIdx.release();
+  b->bonds.release();
+  b->angles.release();
+  b->partials.release();
+  delete b;
+  return MI355NB_OK;
+}
+
+int mi355nb_bonded_forces(mi355nb_bonded* b, const double* pos, const double box[3], double* force, double* energy,
+                          void* stream) {
+  if (!b || !pos || !box || !force || !(box[0] > 0 && box[1] > 0 && box[2] > 0)) {
+    setLastError("bad bonded-force arguments");
+    return MI355NB_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = (b->natoms + BD_THREADS - 1) / BD_THREADS;
+  hipLaunchKernelGGL(bondedKernel, dim3(nb), dim3(BD_THREADS), 0, s, b->natoms, b->termPtr.ptr, b->termIdx.ptr,
+                     b->bonds.ptr, b->angles.ptr, box[0], box[1], box[2], pos, force,
+                     energy ? b->partials.ptr : nullptr);
+  MI355NB_HIP(hipGetLastError());
+  if (energy) {
+    hipLaunchKernelGGL(bondedEnergyFinalKernel, dim3(1), dim3(64), 0, s, nb, b->partials.ptr, energy);
+    MI355NB_HIP(hipGetLastError());
+  }
+  return MI355NB_OK;
+}
+
+}  // extern "C"

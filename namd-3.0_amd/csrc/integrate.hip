@@ -31,20 +31,49 @@ namespace mi355nb {
 
 constexpr int INT_THREADS = 256;
 
+struct ForceInputs {
+  const int* slotOfAtom;
+  const F4* f0;
+  const F4* f1;
+  const F4* f2;
+  const double* fAtom;
+};
+
+__device__ __forceinline__ void totalForce(const ForceInputs& in, int a, double* f) {
+  f[0] = f[1] = f[2] = 0.0;
+  const int s = in.slotOfAtom ? in.slotOfAtom[a] : a;
+  if (in.f0) { const F4 g = in.f0[s]; f[0] += g.x; f[1] += g.y; f[2] += g.z; }
+  if (in.f1) { const F4 g = in.f1[s]; f[0] += g.x; f[1] += g.y; f[2] += g.z; }
+  if (in.f2) { const F4 g = in.f2[s]; f[0] += g.x; f[1] += g.y; f[2] += g.z; }
+  if (in.fAtom) { f[0] += in.fAtom[3 * a]; f[1] += in.fAtom[3 * a + 1]; f[2] += in.fAtom[3 * a + 2]; }
+}
+
 __global__ void __launch_bounds__(INT_THREADS)
-kickKernel(int natoms, double halfDt, const double* __restrict__ invMass, const int* __restrict__ slotOfAtom,
-           const F4* __restrict__ f0, const F4* __restrict__ f1, const F4* __restrict__ f2, double* __restrict__ vel) {
+kickKernel(int natoms, double halfDt, const double* __restrict__ invMass, ForceInputs in, double* __restrict__ vel) {
   const int a = blockIdx.x * INT_THREADS + threadIdx.x;
   if (a >= natoms) return;
-  const int s = slotOfAtom[a];
-  double fx = 0.0, fy = 0.0, fz = 0.0;
-  if (f0) { const F4 g = f0[s]; fx += g.x; fy += g.y; fz += g.z; }
-  if (f1) { const F4 g = f1[s]; fx += g.x; fy += g.y; fz += g.z; }
-  if (f2) { const F4 g = f2[s]; fx += g.x; fy += g.y; fz += g.z; }
+  double f[3];
+  totalForce(in, a, f);
   const double c = halfDt * invMass[a];
-  vel[3 * a] += c * fx;
-  vel[3 * a + 1] += c * fy;
-  vel[3 * a + 2] += c * fz;
+  vel[3 * a] += c * f[0];
+  vel[3 * a + 1] += c * f[1];
+  vel[3 * a + 2] += c * f[2];
+}
+
+__global__ void __launch_bounds__(INT_THREADS)
+minimizeStepKernel(int natoms, double gamma, double maxStep, ForceInputs in, double* __restrict__ pos,
+                   double* __restrict__ posRef) {
+  const int a = blockIdx.x * INT_THREADS + threadIdx.x;
+  if (a >= natoms) return;
+  double f[3];
+  totalForce(in, a, f);
+  const double fn = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+  const double step = (fn > 0.0 && isfinite(fn)) ? fmin(gamma, maxStep / fn) : 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const double x = pos[3 * a + c];
+    posRef[3 * a + c] = x;
+    pos[3 * a + c] = x + step * f[c];
+  }
 }
 
 __global__ void __launch_bounds__(INT_THREADS)
@@ -256,15 +285,34 @@ int blocksFor(long long n) { return (int)((n + INT_THREADS - 1) / INT_THREADS); 
 extern "C" {
 
 int mi355nb_integrate_kick(int natoms, double halfDt, const double* invMass, const int32_t* slotOfAtom,
-                           const float* force0, const float* force1, const float* force2, double* vel, void* stream) {
-  if (natoms < 0 || (natoms > 0 && (!invMass || !slotOfAtom || !vel))) {
+                           const float* force0, const float* force1, const float* force2, const double* forceAtom,
+                           double* vel, void* stream) {
+  if (natoms < 0 || (natoms > 0 && (!invMass || !vel || (!slotOfAtom && (force0 || force1 || force2))))) {
     setLastError("bad kick arguments");
     return MI355NB_ERR_ARG;
   }
   if (natoms == 0) return MI355NB_OK;
+  const ForceInputs in{slotOfAtom, reinterpret_cast<const F4*>(force0), reinterpret_cast<const F4*>(force1),
+                       reinterpret_cast<const F4*>(force2), forceAtom};
   hipLaunchKernelGGL(kickKernel, dim3(blocksFor(natoms)), dim3(INT_THREADS), 0, (hipStream_t)stream, natoms, halfDt,
-                     invMass, slotOfAtom, reinterpret_cast<const F4*>(force0), reinterpret_cast<const F4*>(force1),
-                     reinterpret_cast<const F4*>(force2), vel);
+                     invMass, in, vel);
+  MI355NB_HIP(hipGetLastError());
+  return MI355NB_OK;
+}
+
+int mi355nb_minimize_step(int natoms, double gamma, double maxStep, const int32_t* slotOfAtom, const float* force0,
+                          const float* force1, const float* force2, const double* forceAtom, double* pos,
+                          double* posRef, void* stream) {
+  if (natoms < 0 || (natoms > 0 && (!pos || !posRef || (!slotOfAtom && (force0 || force1 || force2)))) ||
+      !(gamma >= 0) || !(maxStep >= 0)) {
+    setLastError("bad minimization arguments");
+    return MI355NB_ERR_ARG;
+  }
+  if (natoms == 0) return MI355NB_OK;
+  const ForceInputs in{slotOfAtom, reinterpret_cast<const F4*>(force0), reinterpret_cast<const F4*>(force1),
+                       reinterpret_cast<const F4*>(force2), forceAtom};
+  hipLaunchKernelGGL(minimizeStepKernel, dim3(blocksFor(natoms)), dim3(INT_THREADS), 0, (hipStream_t)stream, natoms,
+                     gamma, maxStep, in, pos, posRef);
   MI355NB_HIP(hipGetLastError());
   return MI355NB_OK;
 }

@@ -78,6 +78,7 @@ EXPORTED = [
     "mi355nb_pme_recip_scalar_sum", "mi355nb_pme_recip_solve", "mi355nb_compute_exclusion_correction",
     "mi355nb_compute_update_positions_device", "mi355nb_compute_get_slot_of_atom", "mi355nb_integrate_kick",
     "mi355nb_integrate_drift", "mi355nb_settle_positions", "mi355nb_settle_velocities", "mi355nb_kinetic_energy",
+    "mi355nb_minimize_step", "mi355nb_bonded_create", "mi355nb_bonded_destroy", "mi355nb_bonded_forces",
 ]
 
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -140,7 +141,11 @@ def lib():
     L.mi355nb_compute_exclusion_correction.argtypes = [vp, vp]
     L.mi355nb_compute_update_positions_device.argtypes = [vp, vp]
     L.mi355nb_compute_get_slot_of_atom.argtypes = [vp, i32p]
-    L.mi355nb_integrate_kick.argtypes = [C.c_int, C.c_double, vp, vp, vp, vp, vp, vp, vp]
+    L.mi355nb_integrate_kick.argtypes = [C.c_int, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.mi355nb_minimize_step.argtypes = [C.c_int, C.c_double, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.mi355nb_bonded_create.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, C.POINTER(vp)]
+    L.mi355nb_bonded_destroy.argtypes = [vp]
+    L.mi355nb_bonded_forces.argtypes = [vp, vp, dp, vp, vp, vp]
     L.mi355nb_integrate_drift.argtypes = [C.c_int, C.c_double, vp, vp, vp, vp]
     L.mi355nb_settle_positions.argtypes = [C.c_int, vp, C.c_double, C.c_double, C.c_double, C.c_double, vp, vp, vp,
                                            C.c_double, vp]

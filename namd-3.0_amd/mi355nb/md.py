@@ -1,6 +1,7 @@
-"""NVE molecular dynamics of rigid-water boxes on one GPU: velocity Verlet in
-FP64 with SETTLE, forces from the nonbonded pair path + PME (real space,
-reciprocal solve, self energy, exclusion correction), all through the C ABI.
+"""NVE molecular dynamics on one GPU: velocity Verlet in FP64 with SETTLE
+for the waters and harmonic bonds + angles for the chains, forces from the
+nonbonded pair path + PME (real space, reciprocal solve, self energy,
+exclusion correction), all through the C ABI.
 
 The step follows NAMD's GPU-resident velocity Verlet (SequencerCUDA:
 addForceToMomentum, addVelocityToPosition, rattle1/settle, the second half
@@ -16,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import time
 
 import numpy as np
 
@@ -27,13 +29,13 @@ TIMEFACTOR = 48.88821      # fs per internal time unit (NAMD src/common.h)
 BOLTZMANN = 0.001987191    # kcal/mol/K (NAMD src/common.h)
 
 
-class RigidWaterNVE:
-    """NVE of a water-only System (atoms O, H1, H2 per water)."""
+class NVE:
+    """NVE of a System of chains (stored first) and rigid waters (O, H1, H2)."""
 
     def __init__(self, s: S.System, dt_fs=2.0, temperature=300.0, seed=0, pairlist_every=20, stream=None):
         import torch
-        if s.n_beads:
-            raise ValueError("RigidWaterNVE integrates water-only systems (no bonded terms are built)")
+        if s.n_beads and not s.chain_len:
+            raise ValueError("chain beads without chain_len: bonded terms unknown")
         self.torch = torch
         self.s = s
         self.n = s.natoms
@@ -47,11 +49,13 @@ class RigidWaterNVE:
         self.beta = S.ewald_coefficient(p.cutoff, p.ewald_tol)
         self.box = np.asarray(s.box, np.float64)
         dev = torch.device("cuda")
-        mass = np.tile([S.MASS_O, S.MASS_H, S.MASS_H], self.nw)
+        mass = S.masses(s)
         with torch.cuda.stream(self.stream):
             self.mass = torch.tensor(mass, dtype=torch.float64, device=dev)
             self.inv_mass = torch.tensor(1.0 / mass, dtype=torch.float64, device=dev)
-            self.first = torch.tensor(3 * np.arange(self.nw), dtype=torch.int32, device=dev)
+            self.first = torch.tensor(s.n_beads + 3 * np.arange(self.nw), dtype=torch.int32, device=dev)
+            self.fb = torch.zeros((self.n, 3), dtype=torch.float64, device=dev)
+            self.e_bonded = torch.zeros(2, dtype=torch.float64, device=dev)
             self.pos = torch.tensor(np.mod(s.pos, self.box), dtype=torch.float64, device=dev).contiguous()
             self.pos_ref = torch.empty_like(self.pos)
             self.ev_recip = torch.zeros(7, dtype=torch.float64, device=dev)
@@ -63,6 +67,18 @@ class RigidWaterNVE:
             self.grid = torch.zeros(K[0] * K[1] * K[2], dtype=torch.float32, device=dev)
         self.c = mc.ComputeNonbonded.from_system(s, stream=self.sp)
         self.pr = mc.PmeRecip(K, self.order)
+        self.assign_seconds = 0.0     # host time spent in patch reassignment
+        self.bonded = None
+        if s.n_beads:
+            bonds, bk, br0, angles, ak, at0 = S.chain_bonded_terms(s)
+            h = C.c_void_p()
+            L.check(L.lib().mi355nb_bonded_create(self.n, len(bonds), np.ascontiguousarray(bonds).ctypes.data,
+                                                  np.ascontiguousarray(bk).ctypes.data,
+                                                  np.ascontiguousarray(br0).ctypes.data, len(angles),
+                                                  np.ascontiguousarray(angles).ctypes.data,
+                                                  np.ascontiguousarray(ak).ctypes.data,
+                                                  np.ascontiguousarray(at0).ctypes.data, C.byref(h)), "bonded_create")
+            self.bonded = h
         self.recip = np.diag(1.0 / self.box)
         self.volume = float(np.prod(self.box))
         self._assign()
@@ -79,6 +95,7 @@ class RigidWaterNVE:
         """Reassign patches from the current positions (host) and refresh the
         slot maps and per-slot buffers."""
         torch = self.torch
+        t0 = time.perf_counter()
         self.stream.synchronize()
         host = np.mod(self.pos.cpu().numpy(), self.box)
         self.c.assign_patches(self.box, host)
@@ -89,6 +106,7 @@ class RigidWaterNVE:
             self.frec = torch.zeros((n_slots, 4), dtype=torch.float32, device="cuda")
         self.bufs = self.c.device_buffers()
         self.need_pairlist = True
+        self.assign_seconds += time.perf_counter() - t0
 
     def _init_velocities(self, temperature, seed):
         torch = self.torch
@@ -107,12 +125,16 @@ class RigidWaterNVE:
             self.vel.copy_(torch.from_numpy(v))
 
     def dof_of(self):
-        return 6 * self.nw - 3
+        return 6 * self.nw + 3 * self.s.n_beads - 3
 
     # --- pieces of a step ---------------------------------------------------
     def _forces(self, energy):
         torch = self.torch
         c = self.c
+        if self.bonded is not None:
+            L.check(L.lib().mi355nb_bonded_forces(self.bonded, self.pos.data_ptr(), self.box, self.fb.data_ptr(),
+                                                  self.e_bonded.data_ptr() if energy else None, self.sp),
+                    "bonded_forces")
         c.launch_work(do_energy=energy, do_virial=energy, do_slow=True, do_pairlist=self.need_pairlist)
         self.need_pairlist = False
         c.exclusion_correction(energy_virial=energy, out=self.ev_excl)
@@ -125,11 +147,14 @@ class RigidWaterNVE:
             mc.gather_force(self.atoms, self.grid, self.K, self.order, recip=self.recip, out=self.frec,
                             stream=self.sp)
 
+    def _fatom(self):
+        return self.fb.data_ptr() if self.bonded is not None else None
+
     def _kick(self):
         L.check(L.lib().mi355nb_integrate_kick(self.n, 0.5 * self.dt, self.inv_mass.data_ptr(),
                                                self.slot_of_atom.data_ptr(), self.bufs["force"],
-                                               self.bufs["forceSlow"], self.frec.data_ptr(), self.vel.data_ptr(),
-                                               self.sp), "integrate_kick")
+                                               self.bufs["forceSlow"], self.frec.data_ptr(), self._fatom(),
+                                               self.vel.data_ptr(), self.sp), "integrate_kick")
 
     def _settle_velocities(self):
         L.check(L.lib().mi355nb_settle_velocities(self.nw, self.first.data_ptr(), S.MASS_O, S.MASS_H,
@@ -163,10 +188,30 @@ class RigidWaterNVE:
         ke = float(self.ke.cpu().numpy()[0])
         e_rec = float(self.ev_recip.cpu().numpy()[0])
         e_exc = float(self.ev_excl.cpu().numpy()[0, 0])
-        pe = r["e_vdw"] + r["e_elec"] + r["e_slow"] + e_rec + self.self_energy + e_exc
+        e_b = self.e_bonded.cpu().numpy() if self.bonded is not None else np.zeros(2)
+        pe = r["e_vdw"] + r["e_elec"] + r["e_slow"] + e_rec + self.self_energy + e_exc + float(e_b.sum())
         return dict(kinetic=ke, potential=pe, total=ke + pe, temperature=2.0 * ke / (self.dof * BOLTZMANN),
                     vdw=r["e_vdw"], elec_real=r["e_elec"] + r["e_slow"], elec_recip=e_rec,
-                    elec_self=self.self_energy, elec_excl=e_exc)
+                    elec_self=self.self_energy, elec_excl=e_exc, bond=float(e_b[0]), angle=float(e_b[1]))
+
+    def minimize(self, nsteps, max_step=0.02, gamma=1e-3):
+        """Capped steepest descent (waters kept rigid by SETTLE against the
+        previous positions); patches reassigned every pairlist_every moves."""
+        lib = L.lib()
+        for k in range(nsteps):
+            L.check(lib.mi355nb_minimize_step(self.n, gamma, max_step, self.slot_of_atom.data_ptr(),
+                                              self.bufs["force"], self.bufs["forceSlow"], self.frec.data_ptr(),
+                                              self._fatom(), self.pos.data_ptr(), self.pos_ref.data_ptr(), self.sp),
+                    "minimize_step")
+            L.check(lib.mi355nb_settle_positions(self.nw, self.first.data_ptr(), S.MASS_O, S.MASS_H, S.TIP3P_ROH,
+                                                 S.TIP3P_RHH, self.pos_ref.data_ptr(), self.pos.data_ptr(), None,
+                                                 self.dt, self.sp), "settle_positions")
+            if (k + 1) % self.pairlist_every == 0:
+                self._assign()
+            else:
+                self.c.update_positions_device(self.pos)
+            self._forces(energy=(k + 1 == nsteps))
+        return self.energies()
 
     def equilibrate(self, nsteps, temperature, every=10, start=0):
         """Velocity rescaling to `temperature` every `every` steps (NAMD's
@@ -194,8 +239,14 @@ class RigidWaterNVE:
         return self.pos.cpu().numpy()
 
     def close(self):
+        if self.bonded is not None:
+            L.lib().mi355nb_bonded_destroy(self.bonded)
+            self.bonded = None
         self.pr.close()
         self.c.close()
+
+
+RigidWaterNVE = NVE
 
 
 def drift_fit(records, dt_fs, dof):

@@ -38,6 +38,12 @@ TIP3P_O_EPS, TIP3P_O_RMIN2 = 0.1521, 1.7682
 TIP3P_H_EPS, TIP3P_H_RMIN2 = 0.046, 0.2245
 TIP3P_RHH = 2.0 * TIP3P_ROH * math.sin(0.5 * TIP3P_THETA)
 MASS_O, MASS_H = 15.9994, 1.008
+# chain beads (a choice of this build, the configs leave them open): united
+# CH2-like mass, harmonic bonds at the random walk's 1.5 A step, angles held
+# at their built value
+MASS_BEAD = 14.027
+BEAD_BOND_K, BEAD_BOND_R0 = 300.0, 1.5       # kcal/mol/A^2, A
+BEAD_ANGLE_K = 40.0                           # kcal/mol/rad^2
 RMIN_TO_SIGMA = 2.0 / 2.0 ** (1.0 / 6.0)  # sigma = Rmin / 2^(1/6) = 2*(Rmin/2)/2^(1/6)
 
 
@@ -75,6 +81,7 @@ class System:
     partition: np.ndarray | None = None  # (N,) int8 alchemical partition (0 = environment)
     n_waters: int = 0
     n_beads: int = 0
+    chain_len: int = 0       # beads per chain (chains are stored first, chain by chain)
 
     @property
     def natoms(self) -> int:
@@ -315,7 +322,7 @@ def make_system(name: str, n_waters: int, n_chains: int, chain_len: int, box: fl
     return System(name=name, pos=pos, charge=charge.astype(np.float64), vdw_type=vdw_type,
                   lj_sigma=lj_sigma, lj_eps=lj_eps, excl_ptr=excl_ptr, excl_idx=excl_idx,
                   box=boxv, params=params or NbParams(), seed=seed,
-                  n_waters=n_waters, n_beads=n_beads)
+                  n_waters=n_waters, n_beads=n_beads, chain_len=chain_len if n_chains else 0)
 
 
 # ---------------------------------------------------------------------------
@@ -415,6 +422,33 @@ def nacl_crystal(nside: int = 8, spacing: float = 4.0, ewald_tol: float = 1e-8, 
                   vdw_type=np.zeros(n, np.int32), lj_sigma=np.array([1.0]), lj_eps=np.array([0.0]),
                   excl_ptr=np.zeros(n + 1, np.int64), excl_idx=np.zeros(0, np.int32), box=np.array([L, L, L]),
                   params=p)
+
+
+def chain_bonded_terms(s: System):
+    """Harmonic bonds (k, r0) and angles (k, theta0 = the built angle) of the
+    chains: (bonds (nb, 2), bond_k, bond_r0, angles (na, 3), angle_k,
+    angle_theta0)."""
+    L = s.chain_len
+    nch = s.n_beads // L if L else 0
+    b, a = [], []
+    for c in range(nch):
+        o = c * L
+        b += [(o + k, o + k + 1) for k in range(L - 1)]
+        a += [(o + k, o + k + 1, o + k + 2) for k in range(L - 2)]
+    bonds = np.array(b, np.int32).reshape(-1, 2)
+    angles = np.array(a, np.int32).reshape(-1, 3)
+    u = s.pos[angles[:, 0]] - s.pos[angles[:, 1]]
+    v = s.pos[angles[:, 2]] - s.pos[angles[:, 1]]
+    u -= s.box * np.round(u / s.box)
+    v -= s.box * np.round(v / s.box)
+    cs = (u * v).sum(1) / np.linalg.norm(u, axis=1) / np.linalg.norm(v, axis=1)
+    theta0 = np.arccos(np.clip(cs, -1.0, 1.0))
+    return (bonds, np.full(len(bonds), BEAD_BOND_K), np.full(len(bonds), BEAD_BOND_R0), angles,
+            np.full(len(angles), BEAD_ANGLE_K), theta0)
+
+
+def masses(s: System) -> np.ndarray:
+    return np.concatenate([np.full(s.n_beads, MASS_BEAD), np.tile([MASS_O, MASS_H, MASS_H], s.n_waters)])
 
 
 def ewald_coefficient(cutoff: float, tol: float) -> float:

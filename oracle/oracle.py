@@ -548,3 +548,38 @@ def settle_velocities(pos, vel, mO, mH):
             out[w, p] += lam[l] * r[l] / m[p]
             out[w, q] -= lam[l] * r[l] / m[q]
     return out
+
+
+# ---- harmonic bonds and angles (FP64) -----------------------------------------
+
+def bonded_fp64(pos, box, bonds, bond_k, bond_r0, angles, angle_k, angle_theta0):
+    """bondForce / angleForce (src/ComputeBondedCUDAKernel.cu:194-280,
+    :775-900) in CHARMM form, FP64: returns (E_bond, E_angle, F (n, 3))."""
+    pos = np.asarray(pos, np.float64)
+    box = np.asarray(box, np.float64)
+    F = np.zeros_like(pos)
+    d = pos[bonds[:, 0]] - pos[bonds[:, 1]]
+    d -= box * np.rint(d / box)
+    r = np.linalg.norm(d, axis=1)
+    db = r - bond_r0
+    Eb = float((bond_k * db * db).sum())
+    g = (-2.0 * bond_k * db / r)[:, None] * d
+    np.add.at(F, bonds[:, 0], g)
+    np.add.at(F, bonds[:, 1], -g)
+    u = pos[angles[:, 0]] - pos[angles[:, 1]]
+    v = pos[angles[:, 2]] - pos[angles[:, 1]]
+    u -= box * np.rint(u / box)
+    v -= box * np.rint(v / box)
+    ru = np.linalg.norm(u, axis=1)
+    rv = np.linalg.norm(v, axis=1)
+    cs = np.clip((u * v).sum(1) / (ru * rv), -1.0, 1.0)
+    th = np.arccos(cs)
+    sn = np.maximum(np.sqrt(1.0 - cs * cs), 1e-8)
+    dE = 2.0 * angle_k * (th - angle_theta0)
+    fi = (dE / sn)[:, None] * (v / (ru * rv)[:, None] - (cs / (ru * ru))[:, None] * u)
+    fk = (dE / sn)[:, None] * (u / (ru * rv)[:, None] - (cs / (rv * rv))[:, None] * v)
+    np.add.at(F, angles[:, 0], fi)
+    np.add.at(F, angles[:, 2], fk)
+    np.add.at(F, angles[:, 1], -(fi + fk))
+    Ea = float((angle_k * (th - angle_theta0) ** 2).sum())
+    return Eb, Ea, F
