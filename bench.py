@@ -62,14 +62,16 @@ def main():
     ap.add_argument("--no-overlap", action="store_true", help="run PME and nonbonded work on one stream")
     ap.add_argument("--wg-per-cu", type=int, default=0,
                     help="fix the pair kernel's persistent workgroups per CU instead of autotuning")
-    ap.add_argument("--workload", choices=("solvated96k", "solvated1m", "batch64", "fit23k", "nve9k", "nve96k"),
+    ap.add_argument("--workload", choices=("solvated96k", "solvated1m", "batch64", "fit23k", "nve9k", "nve96k",
+                                           "pme96k"),
                     default="solvated96k",
                     help="solvated96k (default): the 96k-atom step; solvated1m: the same step on 1.07M atoms; "
                          "batch64: 64 uneven alchemical systems in one launch against the same systems run one "
                          "at a time; fit23k: force-matching loss + gradient on 16 frames of the 23k box; nve9k: "
                          "rigid-water NVE (FP64 velocity Verlet + SETTLE) on the 9,000-atom box at liquid density, "
                          "energy drift, and the 3x-density box as specified; nve96k: NVE of the 96k box (chains with "
-                         "harmonic bonds/angles, rigid waters) after minimization")
+                         "harmonic bonds/angles, rigid waters) after minimization; pme96k: the PME kernels "
+                         "(spread, reciprocal solve, gather) alone on the 96k box's atoms, one after another")
     ap.add_argument("--nve-equil", type=int, default=2000, help="nve9k: velocity-rescaling steps before NVE")
     ap.add_argument("--force-table", action="store_true", help="NAMD's 1/r table mode instead of direct math")
     ap.add_argument("--electrostatics", choices=("full", "realspace"), default="full",
@@ -101,6 +103,9 @@ def main():
         return
     if args.workload == "nve96k":
         run_nve96k(args)
+        return
+    if args.workload == "pme96k":
+        run_pme(args)
         return
     dev = torch.device("cuda:0")
     # nonbonded on one stream, PME real space on another (they are independent
@@ -529,6 +534,70 @@ def run_nve96k(args):
         "nve_drift": {k2: (round(v, 6) if isinstance(v, float) else v) for k2, v in fit.items()},
         "nve_total_over_kinetic_rms": round(fit["total_rms"] / fit["kinetic_rms"], 5),
     }
+    print(json.dumps(out), flush=True)
+
+
+def run_pme(args):
+    """The PME kernels of one step, each timed alone (nothing concurrent):
+    fractional coordinates from the slots, spread, reciprocal solve (with
+    energy/virial), gather."""
+    import torch
+    from mi355nb import compute as mc
+    from mi355nb import systems
+
+    s = systems.solvated_96k()
+    p = s.params
+    st = torch.cuda.Stream()
+    torch.cuda.set_stream(st)
+    c = mc.ComputeNonbonded.from_system(s, stream=st.cuda_stream)
+    box = c.system_boxes()[0]
+    recip = np.diag(1.0 / box)
+    vol = float(np.prod(box))
+    beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+    K = p.pme_grid
+    at = c.pme_atoms(recip)
+    grid = torch.zeros(K[0] * K[1] * K[2], device="cuda")
+    f = torch.empty_like(at)
+    pr = mc.PmeRecip(K, p.pme_order)
+    ev = torch.zeros(7, dtype=torch.float64, device="cuda")
+    n = max(1, args.steps)
+
+    def timed(fn):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(n):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / n
+
+    ms = {"frac_coords": timed(lambda: c.pme_atoms(recip, out=at)),
+          "spread": timed(lambda: mc.spread_charge(at, grid, K, p.pme_order)),
+          "recip_solve": timed(lambda: pr.solve(grid, recip, vol, beta, out=ev)),
+          "gather": timed(lambda: mc.gather_force(at, grid, K, p.pme_order, recip=recip, out=f))}
+    total = sum(ms.values())
+    out = {
+        "metric": "PME kernel time on the 96k box (each kernel alone)",
+        "value": round(total, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": args.warmup,
+        "ms_per_step": round(total, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 energy/virial)",
+        "data": "synthetic",
+        "config": {"workload": "pme96k: 96,000 atoms in 101,376 slots, PME 108^3 order 4, fractional coordinates "
+                               "from the pair path's slots"},
+        "kernel_ms": {k2: round(v, 4) for k2, v in ms.items()},
+    }
+    pr.close()
+    c.close()
     print(json.dumps(out), flush=True)
 
 

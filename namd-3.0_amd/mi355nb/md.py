@@ -95,8 +95,8 @@ class NVE:
         """Reassign patches from the current positions (host) and refresh the
         slot maps and per-slot buffers."""
         torch = self.torch
+        self.stream.synchronize()          # queued device work is not host assignment time
         t0 = time.perf_counter()
-        self.stream.synchronize()
         host = np.mod(self.pos.cpu().numpy(), self.box)
         self.c.assign_patches(self.box, host)
         n_slots = self.c.info()["atomStorageSize"]
