@@ -72,13 +72,16 @@ def test_kick_drift_kinetic_energy(torch_cuda):
     dm, dim_ = _dev(torch, m), _dev(torch, 1.0 / m)
     dslot = _dev(torch, slot, torch.int32)
     df = [torch.from_numpy(a).cuda() for a in f]
+    fa = rng.normal(size=(n, 3))
+    dfa = _dev(torch, fa)
     L.check(L.lib().mi355nb_integrate_kick(n, 0.3, dim_.data_ptr(), dslot.data_ptr(), df[0].data_ptr(),
-                                           df[1].data_ptr(), df[2].data_ptr(), dv.data_ptr(), None), "kick")
+                                           df[1].data_ptr(), df[2].data_ptr(), dfa.data_ptr(), dv.data_ptr(), None),
+            "kick")
     L.check(L.lib().mi355nb_integrate_drift(n, 0.7, dv.data_ptr(), dx.data_ptr(), dref.data_ptr(), None), "drift")
     ke = torch.zeros(1, dtype=torch.float64, device="cuda")
     L.check(L.lib().mi355nb_kinetic_energy(n, dm.data_ptr(), dv.data_ptr(), ke.data_ptr(), None), "ke")
     torch.cuda.synchronize()
-    F = sum(a[slot, :3].astype(np.float64) for a in f)
+    F = sum(a[slot, :3].astype(np.float64) for a in f) + fa
     v1 = v + 0.3 * F / m[:, None]
     x1 = x + 0.7 * v1
     assert np.abs(dv.cpu().numpy() - v1).max() < 1e-12 * np.abs(v1).max()
