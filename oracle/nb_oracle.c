@@ -11,7 +11,8 @@
  *                          CudaComputeNonbondedInteractions.h:37-61) and the
  *                          C1-split Ewald real space (CudaComputeNonbondedInteractions.h:107-159;
  *                          fast + slow = erfc(beta r)/r).
- *  oracle_nb_fp64_alch     the same with alchemical partitions (FEP):
+ *  oracle_nb_fp64_alch     the same with alchemical partitions (FEP, and the
+ *                          TI lambda derivatives):
  *                          soft-core LJ of fep_vdw_forceandenergies
  *                          (ComputeNonbondedFEP.C:156-165, potential switching,
  *                          vdw_switch / vdw_forceandenergy of
@@ -112,6 +113,7 @@ typedef struct {
   double elec2Up, elec2Down, vdw2Up, vdw2Down;
   double shUp, shDown, sh2Up, sh2Down;
   int decouple;
+  double coeff;  /* alchVdwShiftCoeff: shift = coeff (1 - vdw lambda) */
 } AlchF;
 
 /* vdw_switch (ComputeNonbondedAlch.h:35-48): S and switchmul2 = -2 dS/d(r^2) */
@@ -173,6 +175,16 @@ static void pair_fp64_alch(const PairConst* c, double r2, double qq, double A, d
   const double slowE = qq * ((ec - 1.0) * rinv + 1.5 * c->rcinv - 0.5 * r2 * c->rc3inv);
   e[2] += le * slowE;
   e[5] += le2 * slowE;
+  /* TI (calcForceEnergyTI, src/CudaComputeNonbondedKernel.cu:358-526): per
+   * partition dE/d(lambda): vdw S (U + lv coeff F/2) (the shift's lambda
+   * dependence included), elec and slow unscaled; e[6..11] = vdw up, vdw
+   * down, elec up, elec down, slow up, slow down */
+  if (up || down) {
+    const int o = up ? 0 : 1;
+    e[6 + o] += S * (U + lv * al->coeff * 0.5 * F);
+    e[8 + o] += fastE;
+    e[10 + o] += slowE;
+  }
 }
 
 /* table-mode pair: columns (fast, -vdwB, vdwA, scor); e[0..2] = vdw, elec, slow */
@@ -291,7 +303,7 @@ static int nb_fp64_core(int n, const double* pos, const double* q, const int32_t
   nt = nthreads > 0 ? nthreads : omp_get_max_threads();
 #endif
   double* fb = (double*)calloc((size_t)nt * 6 * n, sizeof(double));
-  double* acc = (double*)calloc((size_t)nt * 24, sizeof(double));  /* 9 + 9 virial, up to 6 energies */
+  double* acc = (double*)calloc((size_t)nt * 30, sizeof(double));  /* 9 + 9 virial, up to 12 energies */
   int64_t* np = (int64_t*)calloc((size_t)nt, sizeof(int64_t));
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
@@ -302,7 +314,7 @@ static int nb_fp64_core(int n, const double* pos, const double* q, const int32_t
     tid = omp_get_thread_num();
 #endif
     double* F = fb + (size_t)tid * 6 * n;
-    double* S = acc + (size_t)tid * 24;
+    double* S = acc + (size_t)tid * 30;
     int nb[27];
     const int gz = cell % C.nc[2], gy = (cell / C.nc[2]) % C.nc[1], gx = cell / (C.nc[1] * C.nc[2]);
     const int m = cell_neighbours(&C, gx, gy, gz, nb);
@@ -341,7 +353,7 @@ static int nb_fp64_core(int n, const double* pos, const double* q, const int32_t
   }
   if (f) memset(f, 0, sizeof(double) * 3 * n);
   if (fs) memset(fs, 0, sizeof(double) * 3 * n);
-  double tot[24] = {0};
+  double tot[30] = {0};
   int64_t ntot = 0;
   for (int t = 0; t < nt; ++t) {
     const double* F = fb + (size_t)t * 6 * n;
@@ -349,7 +361,7 @@ static int nb_fp64_core(int n, const double* pos, const double* q, const int32_t
       for (int k = 0; k < 3 * n; ++k) f[k] += F[k];
     if (fs)
       for (int k = 0; k < 3 * n; ++k) fs[k] += F[3 * n + k];
-    for (int k = 0; k < 24; ++k) tot[k] += acc[(size_t)t * 24 + k];
+    for (int k = 0; k < 30; ++k) tot[k] += acc[(size_t)t * 30 + k];
     ntot += np[t];
   }
   if (virial) memcpy(virial, tot, 9 * sizeof(double));
@@ -397,8 +409,9 @@ int oracle_nb_fp64_alch(int n, const double* pos, const double* q, const int32_t
   al.elec2Up = lam[4]; al.elec2Down = lam[5]; al.vdw2Up = lam[6]; al.vdw2Down = lam[7];
   al.shUp = lam[8]; al.shDown = lam[9]; al.sh2Up = lam[10]; al.sh2Down = lam[11];
   al.decouple = decouple;
+  al.coeff = lam[12];
   return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, part, &al, NULL, f,
-                      fs, energies, 6, virial, virialSlow, npairs, nthreads);
+                      fs, energies, 12, virial, virialSlow, npairs, nthreads);
 }
 
 /* ---- force-matching adjoint ------------------------------------------------
@@ -1026,7 +1039,7 @@ int oracle_namd_cpu(int n, const double* pos, const double* q, const int32_t* ty
   nt = nthreads > 0 ? nthreads : omp_get_max_threads();
 #endif
   double* fb = (double*)calloc((size_t)nt * 6 * n, sizeof(double));
-  double* acc = (double*)calloc((size_t)nt * 24, sizeof(double));  /* 9 + 9 virial, up to 6 energies */
+  double* acc = (double*)calloc((size_t)nt * 30, sizeof(double));  /* 9 + 9 virial, up to 12 energies */
   int64_t* np = (int64_t*)calloc((size_t)nt, sizeof(int64_t));
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
@@ -1037,7 +1050,7 @@ int oracle_namd_cpu(int n, const double* pos, const double* q, const int32_t* ty
     tid = omp_get_thread_num();
 #endif
     double* F = fb + (size_t)tid * 6 * n;
-    double* S = acc + (size_t)tid * 24;
+    double* S = acc + (size_t)tid * 30;
     int nb[27];
     const int gz = cell % C.nc[2], gy = (cell / C.nc[2]) % C.nc[1], gx = cell / (C.nc[1] * C.nc[2]);
     const int m = cell_neighbours(&C, gx, gy, gz, nb);
@@ -1093,7 +1106,7 @@ int oracle_namd_cpu(int n, const double* pos, const double* q, const int32_t* ty
   }
   if (f) memset(f, 0, sizeof(double) * 3 * n);
   if (fs) memset(fs, 0, sizeof(double) * 3 * n);
-  double tot[24] = {0};
+  double tot[30] = {0};
   int64_t ntot = 0;
   for (int t = 0; t < nt; ++t) {
     const double* F = fb + (size_t)t * 6 * n;
@@ -1101,7 +1114,7 @@ int oracle_namd_cpu(int n, const double* pos, const double* q, const int32_t* ty
       for (int k = 0; k < 3 * n; ++k) f[k] += F[k];
     if (fs)
       for (int k = 0; k < 3 * n; ++k) fs[k] += F[3 * n + k];
-    for (int k = 0; k < 24; ++k) tot[k] += acc[(size_t)t * 24 + k];
+    for (int k = 0; k < 30; ++k) tot[k] += acc[(size_t)t * 30 + k];
     ntot += np[t];
   }
   if (virial) memcpy(virial, tot, 9 * sizeof(double));

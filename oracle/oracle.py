@@ -86,16 +86,20 @@ ALCH_KEYS = ("elecLambdaUp", "elecLambdaDown", "vdwLambdaUp", "vdwLambdaDown", "
 
 def nb_fp64_alch(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff, switchdist, beta, partition,
                  alch, nthreads=0):
-    """FP64 alchemical (FEP) oracle.  alch: mapping with ALCH_KEYS and
-    'alchDecouple' (an AlchData works).  Returns nb_fp64's dict plus the
-    lambda2 energies e_vdw_s, e_elec_s, e_slow_s."""
+    """FP64 alchemical oracle.  alch: mapping with ALCH_KEYS, 'alchDecouple'
+    and optionally 'alchVdwShiftCoeff' (an AlchData works).  Returns nb_fp64's
+    dict plus the lambda2 energies e_vdw_s, e_elec_s, e_slow_s and the TI
+    derivatives e_vdw_ti_1/2, e_elec_ti_1/2, e_slow_ti_1/2 (1: partition 1,
+    2: partition 2)."""
     n = len(q_scaled)
     ntypes = int(round(np.sqrt(np.asarray(A).size)))
     get = (lambda k: alch[k]) if isinstance(alch, dict) else (lambda k: getattr(alch, k))
-    lam = np.array([float(get(k)) for k in ALCH_KEYS], np.float64)
+    has = (lambda k: k in alch) if isinstance(alch, dict) else (lambda k: hasattr(alch, k))
+    lam = np.array([float(get(k)) for k in ALCH_KEYS] +
+                   [float(get("alchVdwShiftCoeff")) if has("alchVdwShiftCoeff") else 0.0], np.float64)
     f = np.zeros(3 * n)
     fs = np.zeros(3 * n)
-    en = np.zeros(6)
+    en = np.zeros(12)
     vir = np.zeros(9)
     virs = np.zeros(9)
     npairs = np.zeros(1, np.int64)
@@ -105,8 +109,9 @@ def nb_fp64_alch(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff,
                               switchdist, beta, _c(partition, np.int8), lam, int(bool(get("alchDecouple"))), f, fs,
                               en, vir, virs, npairs, int(nthreads))
     return dict(f=f.reshape(n, 3), fs=fs.reshape(n, 3), e_vdw=en[0], e_elec=en[1], e_slow=en[2], e_vdw_s=en[3],
-                e_elec_s=en[4], e_slow_s=en[5], virial=vir.reshape(3, 3), virial_slow=virs.reshape(3, 3),
-                npairs=int(npairs[0]))
+                e_elec_s=en[4], e_slow_s=en[5], e_vdw_ti_1=en[6], e_vdw_ti_2=en[7], e_elec_ti_1=en[8],
+                e_elec_ti_2=en[9], e_slow_ti_1=en[10], e_slow_ti_2=en[11], virial=vir.reshape(3, 3),
+                virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]))
 
 
 def nb_adjoint_fp64(pos, q_scaled, vdw_type, ntypes, excl_ptr, excl_idx, box, cutoff, switchdist, beta, g,

@@ -167,3 +167,39 @@ def test_softcore_finite_at_contact():
     U = A[0, 0] / x ** 6 - B[0, 0] / x ** 3
     assert math.isfinite(r["e_vdw"]) and abs(r["e_vdw"] - 0.5 * U) <= 1e-12 * abs(U)
     assert np.all(np.isfinite(r["f"]))
+
+
+def test_ti_sums_are_lambda_derivatives(box):
+    """The TI sums are the derivatives of the FEP energies with respect to
+    each partition's lambda, the soft-core shift moving with it
+    (shift = coeff (1 - lambda)): central differences of the oracle's own FEP
+    energies, with the vdw and elec lambdas varied separately."""
+    s, A, B, beta = box
+    part = np.zeros(s.natoms, np.int8)
+    part[:20] = 1
+    part[20:30] = 2
+    coeff = 5.0
+
+    def factors(vu, vd, eu, ed):
+        f = dict(elecLambdaUp=eu, elecLambdaDown=ed, vdwLambdaUp=vu, vdwLambdaDown=vd, elecLambda2Up=eu,
+                 elecLambda2Down=ed, vdwLambda2Up=vu, vdwLambda2Down=vd, vdwShiftUp=coeff * (1 - vu),
+                 vdwShiftDown=coeff * (1 - vd), vdwShift2Up=coeff * (1 - vu), vdwShift2Down=coeff * (1 - vd),
+                 alchDecouple=0, alchVdwShiftCoeff=coeff)
+        return f
+
+    lam = dict(vu=0.4, vd=0.7, eu=0.3, ed=0.9)
+    r = _alch(s, part, factors(**lam), A, B, beta)
+    h = 1e-5
+    checks = [("vu", "e_vdw", "e_vdw_ti_1"), ("vd", "e_vdw", "e_vdw_ti_2"), ("eu", "e_elec", "e_elec_ti_1"),
+              ("ed", "e_elec", "e_elec_ti_2"), ("eu", "e_slow", "e_slow_ti_1"), ("ed", "e_slow", "e_slow_ti_2")]
+    for key, energy, ti in checks:
+        lp, lm = dict(lam), dict(lam)
+        lp[key] += h
+        lm[key] -= h
+        g = (_alch(s, part, factors(**lp), A, B, beta)[energy] -
+             _alch(s, part, factors(**lm), A, B, beta)[energy]) / (2 * h)
+        assert abs(r[ti]) > 1e-3, ti
+        assert abs(g - r[ti]) <= 1e-6 * max(1.0, abs(r[ti])), (ti, g, r[ti])
+    # reference-only systems have no TI terms
+    r0 = _alch(s, np.zeros(s.natoms, np.int8), factors(**lam), A, B, beta)
+    assert all(r0[k] == 0.0 for k in ("e_vdw_ti_1", "e_vdw_ti_2", "e_elec_ti_1", "e_elec_ti_2"))
