@@ -35,6 +35,12 @@ extern "C" {
 #define MI355NB_ERR_STATE -2
 #define MI355NB_ERR_CAPACITY -3
 
+/* alchemical method of the pair kernel (SimParameters alchFepOn /
+ * alchThermIntOn, selected in CudaComputeNonbonded::launchWork,
+ * src/CudaComputeNonbonded.C:1397-1398) */
+#define MI355NB_ALCH_FEP 0
+#define MI355NB_ALCH_TI 1
+
 /* CudaNBConstants (src/CudaUtils.h:574-590), identical field order. */
 typedef struct {
   float lj_0, lj_1, lj_2, lj_3, lj_4, lj_5;
@@ -207,6 +213,15 @@ int mi355nb_alch_data_fill(const mi355nb_nb_params* p, double lambda, double lam
  * energy*_s (lambda2) fields.  partition == NULL switches it off. */
 int mi355nb_compute_set_alch(mi355nb_compute* c, const int8_t* partition, int numSystems,
                              const mi355nb_alch_data* perSystem);
+
+/* FEP (default) or TI sums on alchemical launches.  TI (doTI of
+ * CudaComputeNonbondedKernel::nonbondedForce, pair math calcForceEnergyTI,
+ * src/CudaComputeNonbondedKernel.cu:356-526) keeps the forces and energies
+ * at lambda and fills energy{Vdw,Elec,Slow}_ti_{1,2} with dU/dlambda of the
+ * appearing (1) and disappearing (2) partitions, the vdW term including the
+ * soft-core shift's lambda dependence (alchVdwShiftCoeff); the *_s fields are
+ * then zero. */
+int mi355nb_compute_set_alch_method(mi355nb_compute* c, int method);
 
 /* Per-step xyzq staging for the current assignment (openBoxesOnPe /
  * copyAtomsSubset, src/CudaComputeNonbonded.C:903-1052) and upload. */

@@ -63,6 +63,7 @@ struct mi355nb_compute {
 
   // alchemical partitions (optional)
   bool alchOn = false;
+  bool alchTi = false;   // TI sums instead of the FEP lambda2 sums
   std::vector<signed char> partByAtom;
   std::vector<AlchParams> alchHost;
 
@@ -600,12 +601,21 @@ int mi355nb_compute_set_alch(mi355nb_compute* c, const int8_t* partition, int nu
     a.vdwShiftUp = d.vdwShiftUp; a.vdwShiftDown = d.vdwShiftDown;
     a.vdwShift2Up = d.vdwShift2Up; a.vdwShift2Down = d.vdwShift2Down;
     a.decouple = d.alchDecouple ? 1 : 0;
+    a.shiftCoeff = d.alchVdwShiftCoeff;
   }
   c->partByAtom.assign(partition, partition + c->natoms);
   c->alchHost = ap;
   c->alchOn = true;
   MI355NB_HIP(hipSetDevice(c->device));
   if (c->haveLayout) return uploadPartitions(c);
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_set_alch_method(mi355nb_compute* c, int method) {
+  CHECK_ARG(c, "null compute");
+  CHECK_ARG(method == MI355NB_ALCH_FEP || method == MI355NB_ALCH_TI, "alchemical method must be FEP or TI");
+  if (c->alchTi != (method == MI355NB_ALCH_TI)) c->dropGraphs();
+  c->alchTi = method == MI355NB_ALCH_TI;
   return MI355NB_OK;
 }
 
@@ -650,6 +660,7 @@ static int launchImpl(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow
   a.wgPerCU = c->wgPerCU;
   a.alch = c->alchOn ? c->alch.ptr : nullptr;
   a.part = c->alchOn ? c->part.ptr : nullptr;
+  a.alchTi = (c->alchOn && c->alchTi) ? 1 : 0;
   a.forceTable = c->tableMode ? c->forceTable.ptr : nullptr;
   a.energyTable = c->tableMode ? c->energyTable.ptr : nullptr;
   a.tableSize = c->tableSize;

@@ -34,6 +34,7 @@ struct ForceArgsHost {
   int wgPerCU;            // persistent workgroups per CU (4 waves each); 0: one wave per list
   const AlchParams* alch; // per-system factors; null: no alchemical partitions
   const signed char* part;// per-slot partition (alchemical only)
+  int alchTi;             // alchemical: 1 = TI sums (dU/dlambda), 0 = FEP sums at lambda2
   const F4* forceTable;   // table mode: N float4 force entries; null: direct math
   const F4* energyTable;
   int tableSize;
@@ -60,7 +61,7 @@ hipError_t launchTransposeForces(int n, int stride, float* fsoa, F4* f, F4* fs, 
 hipError_t launchFill(size_t n, float* p, float v, hipStream_t s);
 hipError_t launchReduceTileListStat(int numLists, const int4* listStat, TileListStat* st, hipStream_t s);
 // Per-system reduction blocks: {system, kind (0 slots, 1 list records), begin, end}.
-constexpr int RED_VALUES = 24;   // doubles per partial block
+constexpr int RED_VALUES = 30;   // doubles per partial block
 void buildReduceBlocks(const int* sysSlotStart, const int* sysListStart, int numSystems,
                        std::vector<int4>& blocks, std::vector<int>& sysBlockStart);
 hipError_t launchReduceVirialEnergy(int numBlocks, int numSystems, const int4* blocks, const int* sysBlockStart,

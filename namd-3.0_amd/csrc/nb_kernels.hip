@@ -194,12 +194,19 @@ __device__ __forceinline__ void pairForce(const float r2, const float qq, const 
 // with (lv, sh) = (1, 0) for ref/dec and (0, 0) for (1,2), and every
 // electrostatic term (fast, slow, energies) is scaled by le.  Energies at
 // lambda2 (the FEP "_s" sums) use the lambda2 factors and shifts.
-template <bool doEnergy, bool doSlow, typename Acc>
+//
+// TI (calcForceEnergyTI, src/CudaComputeNonbondedKernel.cu:356-526) keeps
+// the forces and the energies at lambda and replaces the lambda2 sums by
+// dU/dlambda of each partition, up pairs into (eVdw2, eElec2, eSlow2), down
+// pairs into (eVdw3, eElec3, eSlow3):
+//    vdW   S (U(r^2 + sh) - lv shiftCoeff dU/dx(r^2 + sh))   (sh = shiftCoeff (1 - lv))
+//    elec  the unscaled fast and slow energies.
+template <bool doEnergy, bool doSlow, bool ti, typename Acc, typename AccTi>
 __device__ __forceinline__ void pairForceAlch(const float r2, const float qq, const float A, const float B,
                                               const NbConstants& c, const float beta3, const int pi,
                                               const int pj, const AlchParams& ap, float& f, float& fSlow,
                                               Acc& eVdw, Acc& eElec, Acc& eSlow, Acc& eVdw2, Acc& eElec2,
-                                              Acc& eSlow2) {
+                                              Acc& eSlow2, AccTi& eVdw3, AccTi& eElec3, AccTi& eSlow3) {
   const bool ref = (pi | pj) == 0;
   const bool cross = pi != 0 && pj != 0 && pi != pj;
   const bool alch = !ref && !cross;
@@ -230,7 +237,27 @@ __device__ __forceinline__ void pairForceAlch(const float r2, const float qq, co
     const float Q = 1.0f + u * (HQ1 + u * (HQ2 + u * (HQ3 + u * (HQ4 + u * (HQ5 + u * HQ6)))));
     fSlow = le * qq * (beta3 * (P * __builtin_amdgcn_rcpf(Q)) - c.e_0);
   }
-  if (doEnergy) {
+  if (doEnergy && ti) {
+    const float r = r2 * rinv;
+    const float x = r * c.e_2;
+    const float omx = 1.0f - x;
+    const float fastE = qq * (omx * omx * (1.0f + 0.5f * x) * rinv);
+    if (doSlow) slowE = qq * erfcf(r * c.ewald_0) * rinv - fastE;
+    eVdw.p += lv * (U * s);
+    eElec.p += le * fastE;
+    if (doSlow) eSlow.p += le * slowE;
+    // dU is 2 dU/dx at x = r^2 + sh
+    const float dudl = s * (U - 0.5f * lv * ap.shiftCoeff * dU);
+    const float tu = up ? 1.0f : 0.0f, td = down ? 1.0f : 0.0f;
+    eVdw2.p += tu * dudl;
+    eVdw3.p += td * dudl;
+    eElec2.p += tu * fastE;
+    eElec3.p += td * fastE;
+    if (doSlow) {
+      eSlow2.p += tu * slowE;
+      eSlow3.p += td * slowE;
+    }
+  } else if (doEnergy) {
     const float lv2 = up ? ap.vdwLambda2Up : (down ? ap.vdwLambda2Down : full);
     const float le2 = up ? ap.elecLambda2Up : (down ? ap.elecLambda2Down : full);
     const float sh2 = up ? ap.vdwShift2Up : (down ? ap.vdwShift2Down : 0.0f);
@@ -342,11 +369,13 @@ struct ForceArgs {
 constexpr int PAIR_DIRECT = 0;
 constexpr int PAIR_ALCH = 1;
 constexpr int PAIR_TABLE = 2;
+constexpr int PAIR_ALCH_TI = 3;
 
 template <bool doEnergy, bool doVirial, bool doSlow, int mode>
 __global__ void __launch_bounds__(64 * NB_WAVES)
 nonbondedForceKernel(const ForceArgs args) {
-  constexpr bool doAlch = mode == PAIR_ALCH;
+  constexpr bool doAlch = mode == PAIR_ALCH || mode == PAIR_ALCH_TI;
+  constexpr bool doTi = mode == PAIR_ALCH_TI;
   extern __shared__ float2 s_lj[];   // vdwWidth^2 (A, B)
   __shared__ F4 s_xyzq[NB_WAVES][TILE];
   __shared__ int s_type[NB_WAVES][TILE];     // vdw type | partition << 16 (alchemical)
@@ -386,7 +415,8 @@ nonbondedForceKernel(const ForceArgs args) {
     float ifx = 0.f, ify = 0.f, ifz = 0.f;
     float ifsx = 0.f, ifsy = 0.f, ifsz = 0.f;
     EnergyAcc<doEnergy> eVdw, eElec, eSlow;
-    EnergyAcc<doEnergy> eVdw2, eElec2, eSlow2;   // FEP sums at lambda2 (alchemical only)
+    EnergyAcc<doEnergy> eVdw2, eElec2, eSlow2;   // FEP sums at lambda2 / TI up sums (alchemical only)
+    EnergyAcc<doEnergy && doTi> eVdw3, eElec3, eSlow3;   // TI down sums
     // FP64 copies of the list's i-force for the virial's shift term
     EnergyAcc<doVirial> vfx, vfy, vfz, vfsx, vfsy, vfsz;
 
@@ -446,12 +476,16 @@ nonbondedForceKernel(const ForceArgs args) {
         mrot >>= 1;
         if (active) {
           float f, fs = 0.f;
-          if constexpr (mode == PAIR_ALCH) {
+          if constexpr (doAlch) {
             const int tw = st[j];
             const float2 ab = s_lj[vdwRow + (tw & 0xffff)];
             if (visitAlch) {
-              pairForceAlch<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, iPart, tw >> 16, ap, f,
-                                              fs, eVdw, eElec, eSlow, eVdw2, eElec2, eSlow2);
+              pairForceAlch<doEnergy, doSlow, doTi>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, iPart, tw >> 16,
+                                                    ap, f, fs, eVdw, eElec, eSlow, eVdw2, eElec2, eSlow2, eVdw3,
+                                                    eElec3, eSlow3);
+            } else if constexpr (doTi) {
+              // every pair here is (0,0): plain math, no dU/dlambda
+              pairForce<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, eVdw, eElec, eSlow);
             } else {
               // every pair here is (0,0): plain math; its lambda2 terms equal its lambda terms
               EnergyAcc<doEnergy> tv, te, ts;
@@ -488,6 +522,7 @@ nonbondedForceKernel(const ForceArgs args) {
       }
       eVdw.flush(); eElec.flush(); eSlow.flush();
       eVdw2.flush(); eElec2.flush(); eSlow2.flush();
+      eVdw3.flush(); eElec3.flush(); eSlow3.flush();
       vfx.flush(); vfy.flush(); vfz.flush();
       vfsx.flush(); vfsy.flush(); vfsz.flush();
       waveSync();
@@ -549,10 +584,21 @@ nonbondedForceKernel(const ForceArgs args) {
         b2 = waveSumD(eElec2.v);
         e2 = doSlow ? waveSumD(eSlow2.v) : 0.0;
       }
+      double a3 = 0.0, b3 = 0.0, e3 = 0.0;
+      if (doTi) {
+        a3 = waveSumD(eVdw3.v);
+        b3 = waveSumD(eElec3.v);
+        e3 = doSlow ? waveSumD(eSlow3.v) : 0.0;
+      }
       if (lane == 0) {
         TileListVirialEnergy& ve = args.virialEnergy[tl.listId];
         ve.energyVdw = a; ve.energyElec = b; ve.energySlow = e;
-        ve.energyVdw_s = a2; ve.energyElec_s = b2; ve.energySlow_s = e2;
+        if (doTi) {
+          ve.energyVdw_ti_1 = a2; ve.energyElec_ti_1 = b2; ve.energySlow_ti_1 = e2;
+          ve.energyVdw_ti_2 = a3; ve.energyElec_ti_2 = b3; ve.energySlow_ti_2 = e3;
+        } else {
+          ve.energyVdw_s = a2; ve.energyElec_s = b2; ve.energySlow_s = e2;
+        }
       }
     }
   }
@@ -754,10 +800,11 @@ reduceTileListStatKernel(int numLists, const int4* __restrict__ listStat, TileLi
 //   virial      = sum_slots  F_slot (x) x_slot          (reduceNonbondedVirialKernel)
 //               + sum_lists  F_list (x) shift_list       (reduceVirialEnergyKernel)
 //   component order a*3+b = F_a * x_b as in NAMD.
-// Partial record per block: 9 fast + 9 slow + 3 energies = 21 doubles.
+// Partial record per block: 9 fast + 9 slow + 3 energies + 3 FEP lambda2
+// energies + 6 TI sums = 30 doubles.
 // ---------------------------------------------------------------------------
 constexpr int RED_THREADS = 256;
-constexpr int RED_N = 24;
+constexpr int RED_N = 30;
 static_assert(RED_N == RED_VALUES, "partial block width");
 
 __device__ __forceinline__ double blockSumD(double v, double* sh) {
@@ -822,6 +869,12 @@ reducePartialsKernel(const int4* __restrict__ blocks, const F4* __restrict__ xyz
       v[21] = e.energyVdw_s;
       v[22] = e.energyElec_s;
       v[23] = doSlow ? e.energySlow_s : 0.0;
+      v[24] = e.energyVdw_ti_1;
+      v[25] = e.energyVdw_ti_2;
+      v[26] = e.energyElec_ti_1;
+      v[27] = e.energyElec_ti_2;
+      v[28] = doSlow ? e.energySlow_ti_1 : 0.0;
+      v[29] = doSlow ? e.energySlow_ti_2 : 0.0;
     }
   }
   for (int u = 0; u < RED_N; ++u) {
@@ -849,15 +902,16 @@ reduceFinalKernel(const int* __restrict__ sysBlockStart, const double* __restric
       else if (k == 20) out->energySlow = t;
       else if (k == 21) out->energyVdw_s = t;
       else if (k == 22) out->energyElec_s = t;
-      else out->energySlow_s = t;
+      else if (k == 23) out->energySlow_s = t;
+      else if (k == 24) out->energyVdw_ti_1 = t;
+      else if (k == 25) out->energyVdw_ti_2 = t;
+      else if (k == 26) out->energyElec_ti_1 = t;
+      else if (k == 27) out->energyElec_ti_2 = t;
+      else if (k == 28) out->energySlow_ti_1 = t;
+      else out->energySlow_ti_2 = t;
     }
   }
-  if (threadIdx.x == 0) {
-    out->energyVdw_ti_1 = out->energyVdw_ti_2 = 0.0;
-    out->energyElec_ti_1 = out->energyElec_ti_2 = 0.0;
-    out->energySlow_ti_1 = out->energySlow_ti_2 = 0.0;
-    out->energyGBIS = 0.0;
-  }
+  if (threadIdx.x == 0) out->energyGBIS = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -965,7 +1019,7 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
   const int wgCap = h.wgPerCU <= 0 ? wgNeeded : numCU * h.wgPerCU;
   dim3 g(std::min(wgNeeded, wgCap)), b(64 * NB_WAVES);
   const size_t shmem = sizeof(float2) * (size_t)h.vdwWidth * h.vdwWidth;
-  const int mode = h.alch ? PAIR_ALCH : (h.forceTable ? PAIR_TABLE : PAIR_DIRECT);
+  const int mode = h.alch ? (h.alchTi ? PAIR_ALCH_TI : PAIR_ALCH) : (h.forceTable ? PAIR_TABLE : PAIR_DIRECT);
   const int key = mode * 8 + (doEnergy ? 4 : 0) + (doVirial ? 2 : 0) + (doSlow ? 1 : 0);
   switch (key) {
 #define NB_CASE(M, E, V, S)                                                                  \
@@ -975,7 +1029,7 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
 #define NB_MODE(M)                                                                           \
   NB_CASE(M, 0, 0, 0) NB_CASE(M, 0, 0, 1) NB_CASE(M, 0, 1, 0) NB_CASE(M, 0, 1, 1)            \
   NB_CASE(M, 1, 0, 0) NB_CASE(M, 1, 0, 1) NB_CASE(M, 1, 1, 0) NB_CASE(M, 1, 1, 1)
-    NB_MODE(0) NB_MODE(1) NB_MODE(2)
+    NB_MODE(0) NB_MODE(1) NB_MODE(2) NB_MODE(3)
 #undef NB_MODE
 #undef NB_CASE
     default:

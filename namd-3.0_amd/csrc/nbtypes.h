@@ -114,6 +114,9 @@ struct TileListVirialEnergy {
   double forceSlowx, forceSlowy, forceSlowz;
   double energyVdw, energyElec, energySlow;
   double energyVdw_s, energyElec_s, energySlow_s;   // FEP: at lambda2
+  double energyVdw_ti_1, energyVdw_ti_2;             // TI: dU/dlambda per partition
+  double energyElec_ti_1, energyElec_ti_2;
+  double energySlow_ti_1, energySlow_ti_2;
 };
 
 // Per-system alchemical factors, the lambda-dependent part of AlchData
@@ -126,7 +129,8 @@ struct AlchParams {
   float elecLambda2Up, elecLambda2Down, vdwLambda2Up, vdwLambda2Down;
   float vdwShiftUp, vdwShiftDown, vdwShift2Up, vdwShift2Down;
   int decouple;
-  int pad[3];
+  float shiftCoeff;   // alchVdwShiftCoeff (TI: d shift / d lambda = -shiftCoeff)
+  int pad[2];
 };
 
 // VirialEnergy (src/CudaTileListKernel.h:96-117), same field order.

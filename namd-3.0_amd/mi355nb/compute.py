@@ -117,9 +117,14 @@ class ComputeNonbonded:
             self._h, starts.size - 1, starts, np.ascontiguousarray(boxes, np.float64).reshape(-1),
             np.ascontiguousarray(pos, np.float64).reshape(-1)), "assign_patches_batch")
 
-    def set_alch(self, partition, alch):
+    def set_alch(self, partition, alch, method="fep"):
         """partition: int8 per atom (0, 1, 2) or None to switch off; alch: one
-        lib.AlchData per system."""
+        lib.AlchData per system; method "fep" (energies at lambda2 in the *_s
+        fields) or "ti" (dU/dlambda per partition in the *_ti_1 / *_ti_2
+        fields)."""
+        if method not in ("fep", "ti"):
+            raise ValueError("method must be 'fep' or 'ti'")
+        L.check(L.lib().mi355nb_compute_set_alch_method(self._h, 1 if method == "ti" else 0), "set_alch_method")
         if partition is None:
             L.check(L.lib().mi355nb_compute_set_alch(self._h, None, 0, None), "set_alch")
             return
@@ -171,6 +176,8 @@ class ComputeNonbonded:
         return dict(
             e_vdw=ve.energyVdw, e_elec=ve.energyElec, e_slow=ve.energySlow,
             e_vdw_s=ve.energyVdw_s, e_elec_s=ve.energyElec_s, e_slow_s=ve.energySlow_s,
+            e_vdw_ti_1=ve.energyVdw_ti_1, e_vdw_ti_2=ve.energyVdw_ti_2, e_elec_ti_1=ve.energyElec_ti_1,
+            e_elec_ti_2=ve.energyElec_ti_2, e_slow_ti_1=ve.energySlow_ti_1, e_slow_ti_2=ve.energySlow_ti_2,
             virial=np.array(ve.virial[:]).reshape(3, 3), virial_slow=np.array(ve.virialSlow[:]).reshape(3, 3))
 
     @staticmethod

@@ -70,7 +70,7 @@ EXPORTED = [
     "mi355nb_compute_get_nonbonded_coef", "mi355nb_compute_get_effective_positions",
     "mi355nb_compute_get_tile_lists", "mi355nb_spread_charge", "mi355nb_gather_force", "mi355nb_self_energy",
     "mi355nb_pme_frac_coords", "mi355nb_compute_assign_patches_batch", "mi355nb_alch_data_fill",
-    "mi355nb_compute_set_alch", "mi355nb_compute_finish_batch", "mi355nb_compute_launch_graph",
+    "mi355nb_compute_set_alch", "mi355nb_compute_set_alch_method", "mi355nb_compute_finish_batch", "mi355nb_compute_launch_graph",
     "mi355nb_compute_update_parameters", "mi355nb_compute_force_matching_grad",
     "mi355nb_build_force_energy_tables", "mi355nb_compute_set_force_table", "mi355nb_compute_pme_atoms",
     "mi355nb_compute_get_system_boxes", "mi355nb_pme_b_moduli", "mi355nb_pme_recip_create",
@@ -128,6 +128,7 @@ def lib():
     L.mi355nb_alch_data_fill.argtypes = [C.POINTER(NbParams), C.c_double, C.c_double, C.c_double, C.c_double,
                                          C.c_double, C.c_int, C.POINTER(AlchData)]
     L.mi355nb_compute_set_alch.argtypes = [vp, vp, C.c_int, vp]
+    L.mi355nb_compute_set_alch_method.argtypes = [vp, C.c_int]
     L.mi355nb_compute_finish_batch.argtypes = [vp, vp, vp, vp, C.POINTER(TileListStat)]
     L.mi355nb_compute_launch_graph.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.mi355nb_compute_update_parameters.argtypes = [vp, vp, vp, vp]

@@ -95,6 +95,68 @@ def test_batch_matches_oracle_per_system(torch_cuda, members, alch):
     c.close()
 
 
+TI_KEYS = ("e_vdw_ti_1", "e_vdw_ti_2", "e_elec_ti_1", "e_elec_ti_2", "e_slow_ti_1", "e_slow_ti_2")
+
+
+def test_batch_ti_matches_oracle_per_system(torch_cuda, members):
+    """TI mode: forces and energies at lambda as in FEP mode, dU/dlambda of
+    each partition (soft-core shift coefficient 5) against the FP64 oracle,
+    system by system; the lambda2 sums are zero, and an FEP launch of the
+    same compute afterwards has zero TI sums."""
+    torch = torch_cuda
+    stream = torch.cuda.current_stream().cuda_stream
+    c = ComputeNonbonded.from_systems([m.system for m in members], stream=stream)
+    # partition 1 = the first chain, partition 2 = the second
+    parts = []
+    for m in members:
+        pk = m.partition.copy()
+        chain = m.system.chain_len
+        assert m.system.n_beads >= 2 * chain
+        pk[chain:2 * chain] = 2
+        parts.append(pk)
+    part = np.concatenate(parts)
+    alch = [_alch_for(m, c.params) for m in members]
+    c.set_alch(part, alch, method="ti")
+    print(f"[batch ti] {len(members)} systems, {c.natoms} atoms, launching", flush=True)
+    c.launch_work(do_energy=True, do_virial=True, do_slow=True, do_pairlist=True)
+    per, stat, f, fs = c.finish_batch()
+    pos = c.effective_positions()
+    boxes = c.system_boxes()
+    starts = c.sys_atom_start
+    for k, m in enumerate(members):
+        a, b = starts[k], starts[k + 1]
+        s = m.system
+        p = s.params
+        A, B = L.lj_mix(s.lj_sigma, s.lj_eps)
+        beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+        q = s.charge * math.sqrt(COULOMB)
+        o = orc.nb_fp64_alch(pos[a:b], q, s.vdw_type, A, B, s.excl_ptr, s.excl_idx, boxes[k], p.cutoff,
+                             p.switchdist, beta, parts[k], alch[k])
+        r = per[k]
+        ef, es = rel_rms(f[a:b], o["f"]), rel_rms(fs[a:b], o["fs"])
+        print(f"[ti sys {k} n={s.natoms} lam={m.lam:.3f}] force fast {ef:.2e} slow {es:.2e}", flush=True)
+        assert ef < FORCE_TOL and es < FORCE_TOL
+        elec_scale = abs(o["e_elec"] + o["e_slow"])
+        for key in ("e_vdw", "e_elec", "e_slow") + TI_KEYS:
+            # each TI sum against its own size (it is a sum over the partition's pairs only)
+            scale = abs(o[key]) if ("vdw" in key or "ti" in key) else elec_scale
+            err = abs(r[key] - o[key]) / max(scale, 1e-30)
+            print(f"    {key} gpu {r[key]:.10e} oracle {o[key]:.10e} rel {err:.2e}")
+            assert err < ENERGY_TOL, key
+        for key in ("e_vdw_s", "e_elec_s", "e_slow_s"):
+            assert r[key] == 0.0
+    # back to FEP on the same compute: TI sums vanish, lambda2 sums return
+    c.set_alch(part, alch, method="fep")
+    c.launch_work(do_energy=True, do_virial=True, do_slow=True, do_pairlist=False)
+    per2, _, f2, fs2 = c.finish_batch()
+    assert rel_rms(f2, f) < 1e-6 and rel_rms(fs2, fs) < 1e-6
+    for k in range(len(members)):
+        assert all(per2[k][key] == 0.0 for key in TI_KEYS)
+        assert abs(per2[k]["e_vdw"] - per[k]["e_vdw"]) <= 1e-9 * abs(per[k]["e_vdw"])
+        assert per2[k]["e_vdw_s"] != 0.0
+    c.close()
+
+
 def test_batch_of_one_equals_single_system(torch_cuda, members):
     torch = torch_cuda
     stream = torch.cuda.current_stream().cuda_stream
