@@ -52,6 +52,8 @@ struct mi355nb_compute {
   std::vector<uint32_t> exclBitsHost;
   std::vector<int64_t> exclPtrHost;   // the molecule's exclusion CSR (atom indices)
   std::vector<int32_t> exclIdxHost;
+  std::vector<int> exclSymPtr;        // the same pairs in both directions, deduplicated, per atom
+  std::vector<int> exclSymIdx;
   std::vector<float> vdwCoefHost;  // 2 * ntypes * ntypes
   bool haveMolecule = false;
   bool haveLayout = false;
@@ -310,6 +312,32 @@ int mi355nb_compute_set_molecule(mi355nb_compute* c, int natoms, const int32_t* 
   c->ntypes = ntypes;
   c->exclPtrHost.assign(excl_ptr, excl_ptr + natoms + 1);
   c->exclIdxHost.assign(excl_idx, excl_idx + excl_ptr[natoms]);
+  {
+    // symmetric closure of the listed pairs, one entry per (atom, partner)
+    std::vector<int> deg(natoms + 1, 0);
+    for (int a = 0; a < natoms; ++a)
+      for (int64_t k = excl_ptr[a]; k < excl_ptr[a + 1]; ++k) {
+        ++deg[a + 1];
+        ++deg[excl_idx[k] + 1];
+      }
+    for (int a = 0; a < natoms; ++a) deg[a + 1] += deg[a];
+    std::vector<int> fill(deg.begin(), deg.end() - 1), idx(deg[natoms]);
+    for (int a = 0; a < natoms; ++a)
+      for (int64_t k = excl_ptr[a]; k < excl_ptr[a + 1]; ++k) {
+        const int b = excl_idx[k];
+        idx[fill[a]++] = b;
+        idx[fill[b]++] = a;
+      }
+    c->exclSymPtr.assign(natoms + 1, 0);
+    c->exclSymIdx.clear();
+    c->exclSymIdx.reserve(idx.size());
+    for (int a = 0; a < natoms; ++a) {
+      std::sort(idx.begin() + deg[a], idx.begin() + deg[a + 1]);
+      for (int k = deg[a]; k < deg[a + 1]; ++k)
+        if (k == deg[a] || idx[k] != idx[k - 1]) c->exclSymIdx.push_back(idx[k]);
+      c->exclSymPtr[a + 1] = (int)c->exclSymIdx.size();
+    }
+  }
   c->vdwType.assign(vdw_type, vdw_type + natoms);
   c->qScaled.resize(natoms);
   const double qs = std::sqrt(COULOMB);  // nonbondedScaling = 1, dielectric = 1
@@ -446,24 +474,23 @@ static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomSt
                           hipMemcpyHostToDevice));
   }
   {
-    // excluded partners of every slot, both directions, in slot space (the
-    // molecule's CSR may list a pair once or twice)
-    std::vector<long long> keys;
-    keys.reserve(2 * c->exclIdxHost.size());
-    for (int a = 0; a < c->natoms; ++a)
-      for (int64_t k = c->exclPtrHost[a]; k < c->exclPtrHost[a + 1]; ++k) {
-        const long long b = c->exclIdxHost[k];
-        keys.push_back((long long)c->L.slotOfAtom[a] * n + c->L.slotOfAtom[b]);
-        keys.push_back((long long)c->L.slotOfAtom[b] * n + c->L.slotOfAtom[a]);
-      }
-    std::sort(keys.begin(), keys.end());
-    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-    std::vector<int> ptr(n + 1, 0), idx(keys.size());
-    for (size_t k = 0; k < keys.size(); ++k) {
-      ++ptr[(int)(keys[k] / n) + 1];
-      idx[k] = (int)(keys[k] % n);
+    // excluded partners of every slot (both directions, deduplicated, in
+    // ascending slot order): the atom-space closure renumbered by slot;
+    // padding slots have none
+    const std::vector<int>& sp = c->exclSymPtr;
+    const std::vector<int>& si = c->exclSymIdx;
+    std::vector<int> ptr(n + 1, 0), idx(si.size());
+    for (int s = 0; s < n; ++s) {
+      const int a = c->L.atomIndex[s];
+      ptr[s + 1] = ptr[s] + ((a >= 0 && c->L.slotOfAtom[a] == s) ? sp[a + 1] - sp[a] : 0);
     }
-    for (int k = 0; k < n; ++k) ptr[k + 1] += ptr[k];
+    for (int s = 0; s < n; ++s) {
+      if (ptr[s + 1] == ptr[s]) continue;
+      const int a = c->L.atomIndex[s];
+      int* out = idx.data() + ptr[s];
+      for (int k = sp[a]; k < sp[a + 1]; ++k) *out++ = c->L.slotOfAtom[si[k]];
+      std::sort(idx.data() + ptr[s], out);
+    }
     buildExclusionBlocks(c->L.sysSlotStart.data(), nsys, c->exclBlocksHost, c->exclSysStartHost);
     const size_t nb = c->exclBlocksHost.size();
     MI355NB_HIP(c->exclSlotPtr.reserve(n + 1, 1.0f));

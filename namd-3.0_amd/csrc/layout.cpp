@@ -5,10 +5,20 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <thread>
 
 namespace mi355nb {
 
-double quantize(double x, double q) { return std::nearbyint(x / q) * q; }
+// Round to nearest, ties to even (the default rounding of nearbyint), by the
+// 1.5 * 2^52 addition; exact for |y| < 2^51, library call beyond.
+static inline double roundEven(double y) {
+  const double magic = 6755399441055744.0;   // 1.5 * 2^52
+  if (!(std::fabs(y) < 2251799813685248.0)) return std::nearbyint(y);
+  const double t = y + magic;
+  return std::copysign(t - magic, y);         // nearbyint keeps the sign of a zero result
+}
+
+double quantize(double x, double q) { return roundEven(x / q) * q; }
 
 double ewaldCoefficient(double cutoff, double tol) {
   double b = 1.0;
@@ -180,28 +190,51 @@ bool buildExclusions(int natoms, const int64_t* exclPtr, const int* exclIdx,
   return true;
 }
 
-void orbPartition(int* order, const double* pos, int begin, int end) {
+namespace {
+// ORB on a patch's own contiguous copy of its coordinates: the same
+// comparisons and swaps as on an index array, without the scattered reads
+struct OrbItem {
+  double x[3];
+  int id;
+};
+
+void orbItems(OrbItem* it, int begin, int end) {
   if (!(begin / TILE < (end - 1) / TILE)) return;
-  int split = ((begin + end + TILE) / (TILE * 2)) * TILE;
+  const int split = ((begin + end + TILE) / (TILE * 2)) * TILE;
   double lo[3], hi[3];
-  for (int d = 0; d < 3; ++d) lo[d] = hi[d] = pos[3 * order[begin] + d];
+  for (int d = 0; d < 3; ++d) lo[d] = hi[d] = it[begin].x[d];
   for (int i = begin + 1; i < end; ++i)
     for (int d = 0; d < 3; ++d) {
-      double v = pos[3 * order[i] + d];
+      const double v = it[i].x[d];
       lo[d] = std::min(lo[d], v);
       hi[d] = std::max(hi[d], v);
     }
-  double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
-  int ax = (ex >= ey && ex >= ez) ? 0 : ((ey >= ex && ey >= ez) ? 1 : 2);
-  std::nth_element(order + begin, order + split, order + end, [&](int a, int b) {
-    double va = pos[3 * a + ax], vb = pos[3 * b + ax];
-    return va < vb || (va == vb && a < b);
+  const double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+  const int ax = (ex >= ey && ex >= ez) ? 0 : ((ey >= ex && ey >= ez) ? 1 : 2);
+  std::nth_element(it + begin, it + split, it + end, [ax](const OrbItem& a, const OrbItem& b) {
+    return a.x[ax] < b.x[ax] || (a.x[ax] == b.x[ax] && a.id < b.id);
   });
-  orbPartition(order, pos, begin, split);
-  orbPartition(order, pos, split, end);
+  orbItems(it, begin, split);
+  orbItems(it, split, end);
+}
+}  // namespace
+
+void orbPartition(int* order, const double* pos, int begin, int end) {
+  if (end <= begin) return;
+  std::vector<OrbItem> it(end - begin);
+  for (int i = begin; i < end; ++i) {
+    OrbItem& o = it[i - begin];
+    o.id = order[i];
+    for (int d = 0; d < 3; ++d) o.x[d] = pos[3 * (size_t)order[i] + d];
+  }
+  // split points depend on the absolute range, so recurse on a shifted base
+  OrbItem* base = it.data() - begin;
+  orbItems(base, begin, end);
+  for (int i = begin; i < end; ++i) order[i] = base[i].id;
 }
 
 static inline double wrap(double x, double L) {
+  if (x >= 0.0 && x < L) return x;   // the common case: already in the box
   double w = x - L * std::floor(x / L);
   if (w >= L) w -= L;
   return w;
@@ -225,6 +258,16 @@ bool PatchLayout::buildBatch(int nsys, const int* atomStart, const double* pos, 
   sysListStart.assign(nsys + 1, 0);
   sysBox.assign(3 * (size_t)nsys, 0.0);
   natoms = atomStart[nsys];
+  if (nsys == 1) {
+    // one system: its layout is the whole layout (buildOne numbers patches,
+    // lists and slots from zero, as the general path below would)
+    if (!buildOne(natoms, pos, boxes, patchDim, err)) return false;
+    for (int d = 0; d < 3; ++d) sysBox[d] = box[d];
+    patchSystem.assign(patches.size(), 0);
+    sysSlotStart[1] = atomStorageSize;
+    sysListStart[1] = (int)tileListSkeleton.size();
+    return true;
+  }
   centre.clear(); patches.clear(); computes.clear(); tileListSkeleton.clear(); patchSystem.clear();
   patchOfAtom.assign(natoms, 0);
   slotOfAtom.assign(natoms, -1);
@@ -304,38 +347,73 @@ bool PatchLayout::buildOne(int n, const double* pos, const double boxIn[3], doub
         for (int d = 0; d < 3; ++d) centre[3 * p + d] = quantize((g[d] + 0.5) * patchSize[d], POS_QUANTUM);
       }
 
-  // assign atoms
+  // assign atoms: patch of every atom, then a counting sort by patch
   patchOfAtom.assign(n, 0);
-  std::vector<std::vector<int>> members(np);
   std::vector<double> wpos(3 * (size_t)n);
+  std::vector<int> first(np + 1, 0);
   for (int i = 0; i < n; ++i) {
     int g[3];
     for (int d = 0; d < 3; ++d) {
       double w = wrap(pos[3 * i + d], box[d]);
       wpos[3 * i + d] = w;
-      g[d] = std::min(dims[d] - 1, std::max(0, (int)std::floor(w / patchSize[d])));
+      g[d] = std::min(dims[d] - 1, (int)(w / patchSize[d]));   // w >= 0: truncation is floor
     }
     int p = (g[0] * dims[1] + g[1]) * dims[2] + g[2];
     patchOfAtom[i] = p;
-    members[p].push_back(i);
+    ++first[p + 1];
+  }
+  for (int p = 0; p < np; ++p) first[p + 1] += first[p];
+  std::vector<int> members(n);
+  {
+    std::vector<int> fill(first.begin(), first.end() - 1);
+    for (int i = 0; i < n; ++i) members[fill[patchOfAtom[i]]++] = i;   // ascending atom order per patch
   }
 
   // per-patch ORB order and padded storage
   patches.resize(np);
   slotOfAtom.assign(n, -1);
   atomIndex.clear();
+  {
+    size_t total = 0;
+    for (int p = 0; p < np; ++p) total += (size_t)(first[p + 1] - first[p] + TILE - 1) / TILE * TILE;
+    atomIndex.reserve(total);
+  }
+  // ORB of every patch (independent: split over a few host threads for large systems)
+  auto orbRange = [&](int p0, int p1) {
+    std::vector<OrbItem> items;
+    for (int p = p0; p < p1; ++p) {
+      const int cnt = first[p + 1] - first[p];
+      int* m = members.data() + first[p];
+      if (cnt <= TILE) continue;
+      items.resize(cnt);
+      for (int k = 0; k < cnt; ++k) {
+        items[k].id = m[k];
+        for (int d = 0; d < 3; ++d) items[k].x[d] = wpos[3 * (size_t)m[k] + d];
+      }
+      orbItems(items.data(), 0, cnt);
+      for (int k = 0; k < cnt; ++k) m[k] = items[k].id;
+    }
+  };
+  const int nthr = n < 20000 ? 1 : (int)std::min<unsigned>(8u, std::max(1u, std::thread::hardware_concurrency()));
+  if (nthr <= 1 || np < 2 * nthr) {
+    orbRange(0, np);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthr; ++t) th.emplace_back(orbRange, (int)((long long)np * t / nthr), (int)((long long)np * (t + 1) / nthr));
+    for (std::thread& t : th) t.join();
+  }
   int slot = 0;
   for (int p = 0; p < np; ++p) {
-    std::vector<int>& m = members[p];
-    if (!m.empty()) orbPartition(m.data(), wpos.data(), 0, (int)m.size());
-    patches[p].numAtoms = (int)m.size();
-    patches[p].numFreeAtoms = (int)m.size();
+    const int cnt = first[p + 1] - first[p];
+    const int* m = members.data() + first[p];
+    patches[p].numAtoms = cnt;
+    patches[p].numFreeAtoms = cnt;
     patches[p].atomStart = slot;
     patches[p].patchID = p;
-    int padded = ((int)m.size() + TILE - 1) / TILE * TILE;
+    const int padded = (cnt + TILE - 1) / TILE * TILE;
     for (int k = 0; k < padded; ++k) {
-      int a = m.empty() ? -1 : m[std::min(k, (int)m.size() - 1)];
-      if (k < (int)m.size()) slotOfAtom[a] = slot + k;
+      const int a = cnt == 0 ? -1 : m[std::min(k, cnt - 1)];
+      if (k < cnt) slotOfAtom[a] = slot + k;
       atomIndex.push_back(a);
     }
     slot += padded;
@@ -345,6 +423,7 @@ bool PatchLayout::buildOne(int n, const double* pos, const double boxIn[3], doub
   // computes: self + 13 half-shell neighbours (the lexicographically positive
   // direction of every +-d pair), with the image vector T of the neighbour.
   computes.clear();
+  computes.reserve(14 * (size_t)np);
   for (int gx = 0; gx < dims[0]; ++gx)
     for (int gy = 0; gy < dims[1]; ++gy)
       for (int gz = 0; gz < dims[2]; ++gz) {
@@ -385,6 +464,11 @@ bool PatchLayout::buildOne(int n, const double* pos, const double boxIn[3], doub
 
   // tile-list skeleton and worst-case j-tile capacity
   tileListSkeleton.clear();
+  {
+    size_t lists = 0;
+    for (const ComputeRecord& c : computes) lists += (size_t)(patches[c.patchInd[0]].numAtoms + TILE - 1) / TILE;
+    tileListSkeleton.reserve(lists);
+  }
   jtileCapacity = 0;
   for (int ci = 0; ci < (int)computes.size(); ++ci) {
     const ComputeRecord& c = computes[ci];
@@ -432,7 +516,7 @@ void PatchLayout::packPositions(const double* pos) {
       for (int d = 0; d < 3; ++d) {
         // image of the atom closest to its patch centre
         double x = pos[3 * a + d] - centre[3 * p + d];
-        x -= box[d] * std::nearbyint(x / box[d]);
+        x -= box[d] * roundEven(x / box[d]);
         q3[d] = quantize(x, POS_QUANTUM);
       }
       xyzq[s].x = (float)q3[0];
