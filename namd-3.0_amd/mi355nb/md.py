@@ -194,6 +194,23 @@ class NVE:
                     vdw=r["e_vdw"], elec_real=r["e_elec"] + r["e_slow"], elec_recip=e_rec,
                     elec_self=self.self_energy, elec_excl=e_exc, bond=float(e_b[0]), angle=float(e_b[1]))
 
+    def components(self):
+        """Forces and energies of the current state split by term, atom order
+        (after a force evaluation with energies): nonbonded fast (LJ + fast
+        electrostatics), slow (real-space slow part + exclusion correction),
+        PME reciprocal and bonded (bonds + angles); positions as integrated."""
+        r = self.c.finish(forces=True)
+        e = self.energies()
+        self.stream.synchronize()
+        sa = self.slot_of_atom.long()
+        frec = self.frec[sa, :3].double().cpu().numpy()
+        fb = self.fb.cpu().numpy() if self.bonded is not None else np.zeros((self.n, 3))
+        return dict(pos=self.pos.cpu().numpy().copy(),
+                    force=dict(fast=r["f"], slow=r["fs"], recip=frec, bonded=fb),
+                    energy=dict(fast=r["e_vdw"] + r["e_elec"], slow=r["e_slow"] + e["elec_excl"],
+                                recip=e["elec_recip"], bonded=e["bond"] + e["angle"]),
+                    kinetic=e["kinetic"], total=e["total"])
+
     def minimize(self, nsteps, max_step=0.02, gamma=1e-3):
         """Capped steepest descent (waters kept rigid by SETTLE against the
         previous positions); patches reassigned every pairlist_every moves."""

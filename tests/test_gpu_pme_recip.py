@@ -25,8 +25,9 @@ from nbcheck import rel_rms
 pytestmark = pytest.mark.gpu
 
 COULOMB = 332.0636
-# 28 = 4*7, 44 = 4*11, 42 = 2*3*7: every radix of the FFT
-SHAPES = [(32, 32, 32), (108, 108, 108), (30, 28, 44), (20, 22, 42)]
+# 28 = 4*7, 44 = 4*11, 42 = 2*3*7: every radix of the FFT; (160, 150, 12):
+# long lines (160 = 2^5 * 5, 150 = 2 * 3 * 5^2) and a short z axis.
+SHAPES = [(32, 32, 32), (108, 108, 108), (30, 28, 44), (20, 22, 42), (160, 150, 12)]
 CELLS = {
     "ortho": np.diag([30.0, 32.0, 34.0]),
     "triclinic": np.array([[30.0, 0.0, 0.0], [4.0, 31.0, 0.0], [-3.0, 5.0, 33.0]]),
