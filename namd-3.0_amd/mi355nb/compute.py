@@ -13,7 +13,7 @@ import math
 import numpy as np
 
 from . import lib as L
-from .systems import System, ewald_coefficient
+from .systems import System, ewald_coefficient, lj_tables
 
 COULOMB = 332.0636
 
@@ -38,7 +38,7 @@ class ComputeNonbonded:
         beta = ewald_coefficient(p.cutoff, p.ewald_tol)
         c = cls(p.cutoff, p.switchdist, pairlistdist or p.pairlistdist, beta, slow_scale=1.0, margin=p.margin,
                 device=device, stream=stream)
-        A, B = L.lj_mix(s.lj_sigma, s.lj_eps)
+        A, B = lj_tables(s)
         c.set_molecule(s.vdw_type, s.charge, A, B, s.excl_ptr, s.excl_idx)
         c.assign_patches(s.box, s.pos)
         return c
@@ -54,12 +54,14 @@ class ComputeNonbonded:
             if (q.cutoff, q.switchdist, q.pairlistdist, q.ewald_tol, q.margin) != \
                     (p.cutoff, p.switchdist, p.pairlistdist, p.ewald_tol, p.margin):
                 raise ValueError("batched systems must share cutoff, switchdist, pairlistdist, ewald_tol and margin")
-            if not (np.array_equal(s.lj_sigma, s0.lj_sigma) and np.array_equal(s.lj_eps, s0.lj_eps)):
+            a0, b0 = lj_tables(s0)
+            a1, b1 = lj_tables(s)
+            if not (np.array_equal(a0, a1) and np.array_equal(b0, b1)):
                 raise ValueError("batched systems must share the LJ parameter table")
         beta = ewald_coefficient(p.cutoff, p.ewald_tol)
         c = cls(p.cutoff, p.switchdist, p.pairlistdist, beta, slow_scale=1.0, margin=p.margin, device=device,
                 stream=stream)
-        A, B = L.lj_mix(s0.lj_sigma, s0.lj_eps)
+        A, B = lj_tables(s0)
         starts = np.zeros(len(systems) + 1, np.int32)
         for k, s in enumerate(systems):
             starts[k + 1] = starts[k] + s.natoms

@@ -82,6 +82,8 @@ class System:
     n_waters: int = 0
     n_beads: int = 0
     chain_len: int = 0       # beads per chain (chains are stored first, chain by chain)
+    lj_AB: tuple | None = None   # ordered (A, B) ntypes x ntypes tables given directly (e.g. read
+                                 # from a dumpbench file) instead of mixing lj_sigma / lj_eps
 
     @property
     def natoms(self) -> int:
@@ -445,6 +447,15 @@ def chain_bonded_terms(s: System):
     theta0 = np.arccos(np.clip(cs, -1.0, 1.0))
     return (bonds, np.full(len(bonds), BEAD_BOND_K), np.full(len(bonds), BEAD_BOND_R0), angles,
             np.full(len(angles), BEAD_ANGLE_K), theta0)
+
+
+def lj_tables(s: System):
+    """Ordered LJ tables (A, B), ntypes x ntypes: the System's own when it
+    carries them, else Lorentz-Berthelot mixing of lj_sigma / lj_eps."""
+    if s.lj_AB is not None:
+        return np.asarray(s.lj_AB[0], np.float64), np.asarray(s.lj_AB[1], np.float64)
+    from . import lib as L
+    return L.lj_mix(s.lj_sigma, s.lj_eps)
 
 
 def masses(s: System) -> np.ndarray:

@@ -21,7 +21,7 @@ def rel_rms(a, b):
 def oracle_for(s: systems.System, pos_eff, box, nthreads=0):
     p = s.params
     beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
-    A, B = L.lj_mix(s.lj_sigma, s.lj_eps)
+    A, B = systems.lj_tables(s)
     q = s.charge * np.sqrt(COULOMB)
     return orc.nb_fp64(pos_eff, q, s.vdw_type, A, B, s.excl_ptr, s.excl_idx, np.asarray(box), p.cutoff,
                        p.switchdist, beta, nthreads=nthreads)
