@@ -475,8 +475,8 @@ static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomSt
   }
   {
     // excluded partners of every slot (both directions, deduplicated, in
-    // ascending slot order): the atom-space closure renumbered by slot;
-    // padding slots have none
+    // the molecule's partner order): the atom-space closure renumbered by
+    // slot; padding slots have none
     const std::vector<int>& sp = c->exclSymPtr;
     const std::vector<int>& si = c->exclSymIdx;
     std::vector<int> ptr(n + 1, 0), idx(si.size());
@@ -489,7 +489,6 @@ static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomSt
       const int a = c->L.atomIndex[s];
       int* out = idx.data() + ptr[s];
       for (int k = sp[a]; k < sp[a + 1]; ++k) *out++ = c->L.slotOfAtom[si[k]];
-      std::sort(idx.data() + ptr[s], out);
     }
     buildExclusionBlocks(c->L.sysSlotStart.data(), nsys, c->exclBlocksHost, c->exclSysStartHost);
     const size_t nb = c->exclBlocksHost.size();

@@ -502,9 +502,32 @@ bool PatchLayout::buildOne(int n, const double* pos, const double boxIn[3], doub
   return true;
 }
 
+namespace {
+// Runs body(begin, end) over [0, n) on up to 8 host threads when the work
+// (in atoms) is large enough to pay for them.
+template <typename F>
+void parallelRange(int n, long long work, F body) {
+  const int nthr = (work < 20000 || n < 16)
+                       ? 1
+                       : (int)std::min<unsigned>(8u, std::max(1u, std::thread::hardware_concurrency()));
+  if (nthr <= 1) {
+    body(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthr; ++t)
+    th.emplace_back(body, (int)((long long)n * t / nthr), (int)((long long)n * (t + 1) / nthr));
+  for (std::thread& t : th) t.join();
+}
+}  // namespace
+
 void PatchLayout::packPositions(const double* pos) {
   const int np = (int)patches.size();
-  for (int p = 0; p < np; ++p) {
+  parallelRange(np, atomStorageSize, [&](int p0, int p1) { packPatches(pos, p0, p1); });
+}
+
+void PatchLayout::packPatches(const double* pos, int p0, int p1) {
+  for (int p = p0; p < p1; ++p) {
     const PatchRecord& pr = patches[p];
     const double* box = &sysBox[3 * (size_t)patchSystem[p]];
     int padded = (pr.numAtoms + TILE - 1) / TILE * TILE;
@@ -530,18 +553,20 @@ void PatchLayout::packPositions(const double* pos) {
 
 void PatchLayout::pack(const double* pos, const double* chargeScaled, const int* vdwType,
                        const I2* exclByAtom) {
-  for (int s = 0; s < atomStorageSize; ++s) {
-    int a = atomIndex[s];
-    if (a < 0) {
-      xyzq[s] = F4{0, 0, 0, 0};
-      vdwTypes[s] = 0;
-      exclStartMaxdiff[s] = I2{0, 0};
-      continue;
+  parallelRange(atomStorageSize, atomStorageSize, [&](int s0, int s1) {
+    for (int s = s0; s < s1; ++s) {
+      int a = atomIndex[s];
+      if (a < 0) {
+        xyzq[s] = F4{0, 0, 0, 0};
+        vdwTypes[s] = 0;
+        exclStartMaxdiff[s] = I2{0, 0};
+        continue;
+      }
+      xyzq[s].w = (float)chargeScaled[a];
+      vdwTypes[s] = vdwType[a];
+      exclStartMaxdiff[s] = exclByAtom[a];
     }
-    xyzq[s].w = (float)chargeScaled[a];
-    vdwTypes[s] = vdwType[a];
-    exclStartMaxdiff[s] = exclByAtom[a];
-  }
+  });
   packPositions(pos);
 }
 

@@ -119,6 +119,11 @@ struct PatchLayout {
   void packPositions(const double* pos);
 
  private:
+  void packPatches(const double* pos, int p0, int p1);
+
+ public:
+
+ private:
   bool buildOne(int n, const double* pos, const double boxIn[3], double patchDim, std::string& err);
 };
 

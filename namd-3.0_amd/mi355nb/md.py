@@ -58,6 +58,7 @@ class NVE:
             self.e_bonded = torch.zeros(2, dtype=torch.float64, device=dev)
             self.pos = torch.tensor(np.mod(s.pos, self.box), dtype=torch.float64, device=dev).contiguous()
             self.pos_ref = torch.empty_like(self.pos)
+            self.pos_host = torch.empty(self.pos.shape, dtype=torch.float64, pin_memory=True)
             self.ev_recip = torch.zeros(7, dtype=torch.float64, device=dev)
             self.ev_excl = torch.zeros((1, 10), dtype=torch.float64, device=dev)
             self.ke = torch.zeros(1, dtype=torch.float64, device=dev)
@@ -97,8 +98,12 @@ class NVE:
         torch = self.torch
         self.stream.synchronize()          # queued device work is not host assignment time
         t0 = time.perf_counter()
-        host = np.mod(self.pos.cpu().numpy(), self.box)
-        self.c.assign_patches(self.box, host)
+        # positions to pinned host memory on the driver's stream (the layout
+        # wraps them into the box itself)
+        with torch.cuda.stream(self.stream):
+            self.pos_host.copy_(self.pos, non_blocking=True)
+        self.stream.synchronize()
+        self.c.assign_patches(self.box, self.pos_host.numpy())
         n_slots = self.c.info()["atomStorageSize"]
         with torch.cuda.stream(self.stream):
             self.slot_of_atom = torch.tensor(self.c.slot_of_atom(), dtype=torch.int32, device="cuda")
