@@ -76,7 +76,23 @@ static inline double min_image(double d, double L) { return d - L * nearbyint(d 
 /* ---- FP64 pair physics --------------------------------------------------- */
 typedef struct {
   double rc2, rs2, c1, rc, rcinv, rc3inv, beta;
+  /* vdW force switching (vdwForceSwitching, ComputeNonbondedUtil.C:523-533,
+   * :900-919; Steinbach & Brooks 1994): 0 = energy switching */
+  int forceSwitch;
+  double ka, kb, c6inv, c3inv, va, vb;
 } PairConst;
+
+static void set_force_switch(PairConst* c, double cutoff, double switchdist) {
+  const double s3 = switchdist * switchdist * switchdist, c3 = cutoff * cutoff * cutoff;
+  const double s6 = s3 * s3, c6 = c3 * c3;
+  c->forceSwitch = 1;
+  c->va = -1.0 / (s6 * c6);
+  c->vb = -1.0 / (s3 * c3);
+  c->ka = c6 / (c6 - s6);
+  c->kb = c3 / (c3 - s3);
+  c->c6inv = 1.0 / c6;
+  c->c3inv = 1.0 / c3;
+}
 
 static void pair_fp64(const PairConst* c, double r2, double qq, double A, double B, double* f, double* fs,
                       double* ev, double* ee, double* es) {
@@ -87,15 +103,31 @@ static void pair_fp64(const PairConst* c, double r2, double qq, double A, double
   /* LJ: E = A r^-12 - B r^-6, times S(r^2) above the switching distance */
   const double E = (A * rinv6 - B) * rinv6;
   const double dE_r = (-12.0 * A * rinv6 + 6.0 * B) * rinv6 * rinv2; /* (1/r) dE/dr */
-  double S = 1.0, dS_r = 0.0;
-  if (r2 > c->rs2) {
-    const double d = c->rc2 - r2;
-    S = d * d * (c->rc2 + 2.0 * r2 - 3.0 * c->rs2) * c->c1;
-    /* dS/dr / r = 2 dS/d(r^2) */
-    dS_r = 2.0 * c->c1 * (-2.0 * d * (c->rc2 + 2.0 * r2 - 3.0 * c->rs2) + 2.0 * d * d);
+  double fv;
+  if (c->forceSwitch) {
+    /* force switching: below the switching distance the plain terms plus a
+     * constant, above it k_a (r^-6 - rc^-6)^2 and k_b (r^-3 - rc^-3)^2 */
+    if (r2 > c->rs2) {
+      const double rinv3 = rinv * rinv2;
+      const double ta = rinv6 - c->c6inv, tb = rinv3 - c->c3inv;
+      *ev += A * c->ka * ta * ta - B * c->kb * tb * tb;
+      /* (1/r) dE/dr = 2 dE/d(r^2) */
+      fv = 2.0 * (-6.0 * A * c->ka * ta * rinv6 * rinv2 + 3.0 * B * c->kb * tb * rinv3 * rinv2);
+    } else {
+      *ev += E + A * c->va - B * c->vb;
+      fv = dE_r;
+    }
+  } else {
+    double S = 1.0, dS_r = 0.0;
+    if (r2 > c->rs2) {
+      const double d = c->rc2 - r2;
+      S = d * d * (c->rc2 + 2.0 * r2 - 3.0 * c->rs2) * c->c1;
+      /* dS/dr / r = 2 dS/d(r^2) */
+      dS_r = 2.0 * c->c1 * (-2.0 * d * (c->rc2 + 2.0 * r2 - 3.0 * c->rs2) + 2.0 * d * d);
+    }
+    fv = dE_r * S + E * dS_r;
+    *ev += E * S;
   }
-  const double fv = dE_r * S + E * dS_r;
-  *ev += E * S;
   /* fast: E = qq (1/r - 3/(2rc) + r^2/(2 rc^3)) */
   *f = fv + qq * (-rinv * rinv2 + c->rc3inv);
   *ee += qq * (rinv - 1.5 * c->rcinv + 0.5 * r2 * c->rc3inv);
@@ -286,8 +318,11 @@ static int nb_fp64_core(int n, const double* pos, const double* q, const int32_t
                         const double* A, const double* B, const int64_t* eptr, const int32_t* eidx,
                         const double L[3], double cutoff, double switchdist, double beta, const int8_t* part,
                         const AlchF* al, const Tables* tabs, double* f, double* fs, double* energies,
-                        int nenergies, double* virial, double* virialSlow, int64_t* npairs, int nthreads) {
+                        int nenergies, double* virial, double* virialSlow, int64_t* npairs, int nthreads,
+                        int forceSwitch) {
   PairConst c;
+  memset(&c, 0, sizeof(c));
+  if (forceSwitch) set_force_switch(&c, cutoff, switchdist);
   c.rc = cutoff;
   c.rc2 = cutoff * cutoff;
   c.rs2 = switchdist * switchdist;
@@ -380,7 +415,17 @@ int oracle_nb_fp64(int n, const double* pos, const double* q, const int32_t* typ
                    double switchdist, double beta, double* f, double* fs, double* energies, double* virial,
                    double* virialSlow, int64_t* npairs, int nthreads) {
   return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, NULL, NULL, NULL, f,
-                      fs, energies, 3, virial, virialSlow, npairs, nthreads);
+                      fs, energies, 3, virial, virialSlow, npairs, nthreads, 0);
+}
+
+/* The same with vdW force switching instead of energy switching. */
+int oracle_nb_fp64_vdw_force_switch(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+                                    const double* A, const double* B, const int64_t* eptr, const int32_t* eidx,
+                                    const double L[3], double cutoff, double switchdist, double beta, double* f,
+                                    double* fs, double* energies, double* virial, double* virialSlow,
+                                    int64_t* npairs, int nthreads) {
+  return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, NULL, NULL, NULL, f,
+                      fs, energies, 3, virial, virialSlow, npairs, nthreads, 1);
 }
 
 /* force4/energy4: N float4 table entries (columns fast, -vdwB, vdwA, scor) */
@@ -393,7 +438,7 @@ int oracle_nb_table(int n, const double* pos, const double* q, const int32_t* ty
   T.et = energy4;
   T.N = N;
   return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, 0.0, NULL, NULL, &T, f, fs,
-                      energies, 3, virial, virialSlow, npairs, nthreads);
+                      energies, 3, virial, virialSlow, npairs, nthreads, 0);
 }
 
 /* lam[12] = elecUp, elecDown, vdwUp, vdwDown, elec2Up, elec2Down, vdw2Up,
@@ -411,7 +456,7 @@ int oracle_nb_fp64_alch(int n, const double* pos, const double* q, const int32_t
   al.decouple = decouple;
   al.coeff = lam[12];
   return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, part, &al, NULL, f,
-                      fs, energies, 12, virial, virialSlow, npairs, nthreads);
+                      fs, energies, 12, virial, virialSlow, npairs, nthreads, 0);
 }
 
 /* ---- force-matching adjoint ------------------------------------------------

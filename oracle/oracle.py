@@ -31,6 +31,7 @@ def lib():
         L = _LIB
         L.oracle_nb_fp64.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, i64p, i32p, dp, C.c_double, C.c_double,
                                      C.c_double, dp, dp, dp, dp, dp, i64p, C.c_int]
+        L.oracle_nb_fp64_vdw_force_switch.argtypes = L.oracle_nb_fp64.argtypes
         L.oracle_pairs_fp32.argtypes = [C.c_int, dp, i64p, i32p, dp, C.c_float, C.c_float, i64p, u64p, u64p,
                                         C.c_void_p, C.c_int64]
         L.oracle_pair_hash.restype = C.c_uint64
@@ -62,8 +63,12 @@ def _c(a, dt):
     return np.ascontiguousarray(a, dtype=dt)
 
 
-def nb_fp64(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff, switchdist, beta, nthreads=0):
-    """FP64 analytic oracle. Returns dict(f, fs, e_vdw, e_elec, e_slow, virial, virial_slow, npairs)."""
+def nb_fp64(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff, switchdist, beta, nthreads=0,
+            vdw_force_switching=False):
+    """FP64 analytic oracle. Returns dict(f, fs, e_vdw, e_elec, e_slow, virial, virial_slow, npairs).
+    vdw_force_switching: NAMD's vdwForceSwitching (Steinbach-Brooks force
+    switch, ComputeNonbondedUtil.C:523-533, :900-919) instead of energy
+    switching."""
     n = len(q_scaled)
     ntypes = int(round(np.sqrt(np.asarray(A).size)))
     f = np.zeros(3 * n)
@@ -72,10 +77,11 @@ def nb_fp64(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff, swit
     vir = np.zeros(9)
     virs = np.zeros(9)
     npairs = np.zeros(1, np.int64)
-    lib().oracle_nb_fp64(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32),
-                         ntypes, _c(A, np.float64).reshape(-1), _c(B, np.float64).reshape(-1),
-                         _c(excl_ptr, np.int64), _c(excl_idx, np.int32), _c(box, np.float64), cutoff, switchdist,
-                         beta, f, fs, en, vir, virs, npairs, int(nthreads))
+    fn = lib().oracle_nb_fp64_vdw_force_switch if vdw_force_switching else lib().oracle_nb_fp64
+    fn(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32),
+       ntypes, _c(A, np.float64).reshape(-1), _c(B, np.float64).reshape(-1),
+       _c(excl_ptr, np.int64), _c(excl_idx, np.int32), _c(box, np.float64), cutoff, switchdist,
+       beta, f, fs, en, vir, virs, npairs, int(nthreads))
     return dict(f=f.reshape(n, 3), fs=fs.reshape(n, 3), e_vdw=en[0], e_elec=en[1], e_slow=en[2],
                 virial=vir.reshape(3, 3), virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]))
 
