@@ -318,6 +318,17 @@ int mi355nb_compute_exclusion_correction(mi355nb_compute* c, double* energyViria
  * instead of direct math.  Not combinable with alchemical partitions. */
 int mi355nb_compute_set_force_table(mi355nb_compute* c, int enable);
 
+/* vdW force switching (SimParameters::vdwForceSwitching; constants
+ * src/ComputeNonbondedUtil.C:523-533, switched form :900-919, which the
+ * reference's GPU kernel reaches through its tables): later launches use the
+ * CHARMM force-switched LJ in the direct-math kernel,
+ *   r <= switchdist: E = A (r^-12 - 1/(s^6 c^6)) - B (r^-6 - 1/(s^3 c^3)),
+ *   r >  switchdist: E = A ka (r^-6 - c^-6)^2 - B kb (r^-3 - c^-3)^2,
+ * ka = c^6/(c^6 - s^6), kb = c^3/(c^3 - s^3), instead of the energy switch.
+ * Electrostatics are unchanged.  Not combinable with table mode, alchemical
+ * partitions or the force-matching adjoint (MI355NB_ERR_STATE). */
+int mi355nb_compute_set_vdw_force_switching(mi355nb_compute* c, int enable);
+
 /* Launch shape of the pair kernel on non-pairlist steps: persistent
  * workgroups (4 waves each) per CU, 1..64, or 0 (default) for one wave per
  * tile list.  Chosen by mi355nb.autotune; no counterpart

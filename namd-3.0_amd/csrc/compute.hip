@@ -118,6 +118,8 @@ struct mi355nb_compute {
   DevBuf<int> patchSystemDev;
   // NAMD table mode (optional)
   bool tableMode = false;
+  bool vdwForceSwitch = false;   // force-switched LJ in the direct-math kernel
+  FsConstants fsw{};
   int tableSize = 4096;   // FORCE_ENERGY_TABLE_SIZE of the reference
   DevBuf<F4> forceTable, energyTable;
   // force-matching adjoint scratch
@@ -609,6 +611,7 @@ int mi355nb_compute_set_alch(mi355nb_compute* c, const int8_t* partition, int nu
   }
   CHECK_ARG(perSystem && numSystems >= 1, "bad alchemical arguments");
   CHECK_STATE(!c->tableMode, "table mode and alchemical partitions cannot be combined");
+  CHECK_STATE(!c->vdwForceSwitch, "vdW force switching and alchemical partitions cannot be combined");
   if (c->haveLayout) CHECK_ARG(numSystems == c->L.numSystems, "one alchemical record per system");
   for (int i = 0; i < c->natoms; ++i) CHECK_ARG(partition[i] >= 0 && partition[i] <= 2, "partition must be 0, 1 or 2");
   const float c2 = (float)(c->p.cutoff * c->p.cutoff), s2 = (float)(c->p.switchdist * c->p.switchdist);
@@ -690,6 +693,8 @@ static int launchImpl(mi355nb_compute* c, int doEnergy, int doVirial, int doSlow
   a.forceTable = c->tableMode ? c->forceTable.ptr : nullptr;
   a.energyTable = c->tableMode ? c->energyTable.ptr : nullptr;
   a.tableSize = c->tableSize;
+  a.vdwForceSwitch = c->vdwForceSwitch ? 1 : 0;
+  a.fsw = c->fsw;
 
   if (doPairlist) {
     const float cull = (float)(c->p.pairlistdist + 0.01);
@@ -823,6 +828,7 @@ int mi355nb_compute_force_matching_grad(mi355nb_compute* c, const double* target
   CHECK_ARG(c && targetForce, "null argument");
   CHECK_STATE(c->haveLayout && c->haveLists, "a pairlist launch must precede force_matching_grad");
   CHECK_STATE(!c->alchOn, "the force-matching adjoint does not support alchemical partitions");
+  CHECK_STATE(!c->vdwForceSwitch, "the force-matching adjoint does not support vdW force switching");
   if (c->ntypes > 24) {
     setLastError("force-matching adjoint: more than 24 LJ types (its LDS bins hold 24)");
     return MI355NB_ERR_CAPACITY;
@@ -988,6 +994,7 @@ int mi355nb_compute_pme_atoms(const mi355nb_compute* c, const double recip[9], c
 int mi355nb_compute_set_force_table(mi355nb_compute* c, int enable) {
   CHECK_ARG(c, "null compute");
   if (enable) CHECK_STATE(!c->alchOn, "table mode and alchemical partitions cannot be combined");
+  if (enable) CHECK_STATE(!c->vdwForceSwitch, "table mode and vdW force switching cannot be combined");
   MI355NB_HIP(hipSetDevice(c->device));
   c->dropGraphs();
   if (enable && !c->forceTable.ptr) {
@@ -1000,6 +1007,27 @@ int mi355nb_compute_set_force_table(mi355nb_compute* c, int enable) {
     MI355NB_HIP(hipMemcpy(c->energyTable.ptr, et.data(), et.size() * sizeof(float), hipMemcpyHostToDevice));
   }
   c->tableMode = enable != 0;
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_set_vdw_force_switching(mi355nb_compute* c, int enable) {
+  CHECK_ARG(c, "null compute");
+  if (enable) {
+    CHECK_STATE(!c->tableMode, "table mode and vdW force switching cannot be combined");
+    CHECK_STATE(!c->alchOn, "vdW force switching and alchemical partitions cannot be combined");
+    // constants of ComputeNonbondedUtil.C:523-533, in FP64 then rounded once
+    const double s2 = c->p.switchdist * c->p.switchdist, c2 = c->p.cutoff * c->p.cutoff;
+    const double s6 = s2 * s2 * s2, c6 = c2 * c2 * c2, s3 = s2 * c->p.switchdist, c3 = c2 * c->p.cutoff;
+    CHECK_ARG(c->p.switchdist < c->p.cutoff, "vdW force switching needs switchdist < cutoff");
+    c->fsw.ka = (float)(c6 / (c6 - s6));
+    c->fsw.kb = (float)(c3 / (c3 - s3));
+    c->fsw.c6inv = (float)(1.0 / c6);
+    c->fsw.c3inv = (float)(1.0 / c3);
+    c->fsw.va = (float)(-1.0 / (s6 * c6));
+    c->fsw.vb = (float)(-1.0 / (s3 * c3));
+  }
+  if (c->vdwForceSwitch != (enable != 0)) c->dropGraphs();
+  c->vdwForceSwitch = enable != 0;
   return MI355NB_OK;
 }
 

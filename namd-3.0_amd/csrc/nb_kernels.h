@@ -38,6 +38,8 @@ struct ForceArgsHost {
   const F4* forceTable;   // table mode: N float4 force entries; null: direct math
   const F4* energyTable;
   int tableSize;
+  int vdwForceSwitch;     // direct math: 1 = force-switched LJ (fsw), 0 = energy-switched
+  FsConstants fsw;
 };
 
 hipError_t launchBoundingBoxes(int numTiles, const F4* xyzq, BoundingBox* bbox, hipStream_t s);

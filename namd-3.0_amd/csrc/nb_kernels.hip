@@ -139,25 +139,39 @@ struct EnergyAcc<true> {
   }
 };
 
-template <bool doEnergy, bool doSlow, typename Acc>
+//
+// forceSwitch selects the force-switched LJ of FsConstants instead of the
+// energy switch (the electrostatics are the same).
+template <bool doEnergy, bool doSlow, bool forceSwitch = false, typename Acc>
 __device__ __forceinline__ void pairForce(const float r2, const float qq, const float A, const float B,
                                           const NbConstants& c, const float beta3, float& f, float& fSlow,
-                                          Acc& eVdw, Acc& eElec, Acc& eSlow) {
+                                          Acc& eVdw, Acc& eElec, Acc& eSlow, const FsConstants* fc = nullptr) {
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float rinv2 = rinv * rinv;
   const float rinv3 = rinv * rinv2;
   const float rinv6 = rinv3 * rinv3;
   const float rinv8 = rinv6 * rinv2;
 
-  // LJ with energy switching (CudaComputeNonbondedInteractions.h:37-61)
   const float ab_r6 = A * rinv6 - B;
   const float w = ab_r6 * rinv6;
   const float dw_r = (A * rinv6 + ab_r6) * -6.0f * rinv8;
   const bool sw = r2 > c.lj_5;
-  const float delta_r = c.lj_4 - r2;
-  const float s = sw ? delta_r * delta_r * (c.lj_0 + c.lj_1 * r2) : 1.0f;
-  const float ds_r = sw ? delta_r * (c.lj_3 + c.lj_2 * r2) : 0.0f;
-  const float f_vdw = w * ds_r + dw_r * s;
+  float f_vdw, e_vdw = 0.f;
+  if constexpr (forceSwitch) {
+    // force switching: beyond switchdist (1/r) dE/dr = (6 B kb tb r^-3 - 12 A ka ta r^-6) / r^2
+    const float ta = rinv6 - fc->c6inv;
+    const float tb = rinv3 - fc->c3inv;
+    const float Aka = A * fc->ka, Bkb = B * fc->kb;
+    f_vdw = sw ? rinv2 * (6.0f * Bkb * tb * rinv3 - 12.0f * Aka * ta * rinv6) : dw_r;
+    if (doEnergy) e_vdw = sw ? Aka * ta * ta - Bkb * tb * tb : w + (A * fc->va - B * fc->vb);
+  } else {
+    // LJ with energy switching (CudaComputeNonbondedInteractions.h:37-61)
+    const float delta_r = c.lj_4 - r2;
+    const float s = sw ? delta_r * delta_r * (c.lj_0 + c.lj_1 * r2) : 1.0f;
+    const float ds_r = sw ? delta_r * (c.lj_3 + c.lj_2 * r2) : 0.0f;
+    f_vdw = w * ds_r + dw_r * s;
+    if (doEnergy) e_vdw = w * s;
+  }
 
   // fast electrostatics (PMEFast_C1, :107-123)
   const float elec_fast = -1.0f * rinv3 + c.e_0;
@@ -173,7 +187,7 @@ __device__ __forceinline__ void pairForce(const float r2, const float qq, const 
     const float x = r * c.e_2;
     const float omx = 1.0f - x;
     const float fastE = omx * omx * (1.0f + 0.5f * x) * rinv;
-    eVdw.p += w * s;
+    eVdw.p += e_vdw;
     eElec.p += qq * fastE;
     if (doSlow) eSlow.p += qq * (erfcf(r * c.ewald_0) * rinv - fastE);
   }
@@ -363,6 +377,8 @@ struct ForceArgs {
   const F4* forceTable;
   const F4* energyTable;
   int tableSize;
+  // force-switched direct math
+  FsConstants fsw;
 };
 
 // pair-math variants of the kernel
@@ -370,6 +386,7 @@ constexpr int PAIR_DIRECT = 0;
 constexpr int PAIR_ALCH = 1;
 constexpr int PAIR_TABLE = 2;
 constexpr int PAIR_ALCH_TI = 3;
+constexpr int PAIR_DIRECT_FS = 4;   // direct math, force-switched LJ
 
 template <bool doEnergy, bool doVirial, bool doSlow, int mode>
 __global__ void __launch_bounds__(64 * NB_WAVES)
@@ -502,7 +519,8 @@ nonbondedForceKernel(const ForceArgs args) {
                                              args.tableSize, f, fs, eVdw, eElec, eSlow);
           } else {
             const float2 ab = s_lj[vdwRow + st[j]];
-            pairForce<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, eVdw, eElec, eSlow);
+            pairForce<doEnergy, doSlow, mode == PAIR_DIRECT_FS>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs,
+                                                                eVdw, eElec, eSlow, &args.fsw);
           }
           const float fx = dx * f, fy = dy * f, fz = dz * f;
           ifx += fx; ify += fy; ifz += fz;
@@ -993,6 +1011,7 @@ static ForceArgs toDevice(const ForceArgsHost& h) {
   a.forceTable = h.forceTable;
   a.energyTable = h.energyTable;
   a.tableSize = h.tableSize;
+  a.fsw = h.fsw;
   return a;
 }
 
@@ -1019,7 +1038,8 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
   const int wgCap = h.wgPerCU <= 0 ? wgNeeded : numCU * h.wgPerCU;
   dim3 g(std::min(wgNeeded, wgCap)), b(64 * NB_WAVES);
   const size_t shmem = sizeof(float2) * (size_t)h.vdwWidth * h.vdwWidth;
-  const int mode = h.alch ? (h.alchTi ? PAIR_ALCH_TI : PAIR_ALCH) : (h.forceTable ? PAIR_TABLE : PAIR_DIRECT);
+  const int mode = h.alch ? (h.alchTi ? PAIR_ALCH_TI : PAIR_ALCH)
+                          : (h.forceTable ? PAIR_TABLE : (h.vdwForceSwitch ? PAIR_DIRECT_FS : PAIR_DIRECT));
   const int key = mode * 8 + (doEnergy ? 4 : 0) + (doVirial ? 2 : 0) + (doSlow ? 1 : 0);
   switch (key) {
 #define NB_CASE(M, E, V, S)                                                                  \
@@ -1029,7 +1049,7 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
 #define NB_MODE(M)                                                                           \
   NB_CASE(M, 0, 0, 0) NB_CASE(M, 0, 0, 1) NB_CASE(M, 0, 1, 0) NB_CASE(M, 0, 1, 1)            \
   NB_CASE(M, 1, 0, 0) NB_CASE(M, 1, 0, 1) NB_CASE(M, 1, 1, 0) NB_CASE(M, 1, 1, 1)
-    NB_MODE(0) NB_MODE(1) NB_MODE(2) NB_MODE(3)
+    NB_MODE(0) NB_MODE(1) NB_MODE(2) NB_MODE(3) NB_MODE(4)
 #undef NB_MODE
 #undef NB_CASE
     default:

@@ -35,6 +35,18 @@ struct NbConstants {
   float slowScale;   // fullElectFrequency/nonbondedFrequency
 };
 
+// Force-switched LJ (vdwForceSwitching; src/ComputeNonbondedUtil.C:523-533
+// for the constants, :900-919 for the switched form).  With s = switchdist,
+// c = cutoff:
+//   r <= s:  E = A (r^-12 + va) - B (r^-6 + vb)               (plain force)
+//   r >  s:  E = A ka (r^-6 - c^-6)^2 - B kb (r^-3 - c^-3)^2
+// va = -1/(s^6 c^6), vb = -1/(s^3 c^3), ka = c^6/(c^6 - s^6), kb = c^3/(c^3 - s^3).
+struct FsConstants {
+  float ka, kb;
+  float c6inv, c3inv;
+  float va, vb;
+};
+
 // CudaPatchRecord (src/CudaTileListKernel.h:50-57).
 struct PatchRecord {
   int numAtoms;

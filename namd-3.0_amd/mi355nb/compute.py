@@ -256,6 +256,11 @@ class ComputeNonbonded:
         """NAMD's 1/r table mode (True) or direct math (False, the default)."""
         L.check(L.lib().mi355nb_compute_set_force_table(self._h, int(enable)), "set_force_table")
 
+    def set_vdw_force_switching(self, enable=True):
+        """NAMD's vdwForceSwitching (CHARMM force-switched LJ) in the direct-math kernel (True),
+        or the energy switch (False, the default)."""
+        L.check(L.lib().mi355nb_compute_set_vdw_force_switching(self._h, int(enable)), "set_vdw_force_switching")
+
     def set_launch_config(self, workgroups_per_cu):
         L.check(L.lib().mi355nb_compute_set_launch_config(self._h, int(workgroups_per_cu)), "set_launch_config")
 
