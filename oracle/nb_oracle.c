@@ -20,6 +20,11 @@
  *                          electrostatics (ComputeNonbondedBase2.h:367-372,
  *                          :497, :645-686), with the pair classes of
  *                          calcForceEnergyFEP (CudaComputeNonbondedKernel.cu:203-213).
+ *  oracle_nb_fp64_vdw_force_switch, oracle_nb_fp64_alch_vdw_force_switch
+ *                          the same two with vdW force switching
+ *                          (vdwForceSwitching: ComputeNonbondedUtil.C:523-533,
+ *                          :900-919; alchemical: ComputeNonbondedFEP.C:136-159,
+ *                          ComputeNonbondedTI.C:113-128).
  *  oracle_nb_table         table-mode pair evaluation: the pair loop of
  *                          oracle_nb_fp64 with forces and energies from given
  *                          1/r float4 tables, linearly interpolated in FP64
@@ -168,6 +173,29 @@ static void vdw_fe64(double A, double B, double x, double* U, double* F) {
   *F = 6.0 * i2 * (2.0 * (*U) + B * i6);
 }
 
+/* Force-switched soft-core LJ at x = r^2 + shift with the switching distance
+ * moved by the same shift, sd = switchdist^2 + shift, and the cutoff not
+ * (vdw_fswitch_forceandenergy / vdw_fswitch_shift, ComputeNonbondedAlch.h:56-104,
+ * as fep_vdw_forceandenergies and ti_vdw_force_energy_dUdl call them,
+ * ComputeNonbondedFEP.C:136-159, ComputeNonbondedTI.C:113-128).  inner: the
+ * pair is within the (unshifted) switching distance.  U is the energy (with
+ * the constant shift inside), F = -2 dU/dx. */
+static void vdw_fswitch64(const PairConst* c, int inner, double A, double B, double x, double sd, double* U,
+                          double* F) {
+  const double c6 = c->rc2 * c->rc2 * c->rc2, sd6 = sd * sd * sd;
+  if (inner) {
+    vdw_fe64(A, B, x, U, F);
+    *U += -A / (c6 * sd6) + B / sqrt(c6 * sd6);
+  } else {
+    const double i2 = 1.0 / x, i6 = i2 * i2 * i2, i3 = sqrt(i6);
+    const double ic6 = 1.0 / c6, ic3 = sqrt(ic6);
+    const double ka = A / (1.0 - sd6 * ic6), kb = B / (1.0 - sqrt(sd6 * ic6));
+    const double ta = i6 - ic6, tb = i3 - ic3;
+    *U = ka * ta * ta - kb * tb * tb;
+    *F = 6.0 * i2 * (2.0 * ka * ta * i6 - kb * tb * i3);
+  }
+}
+
 /* e[0..5] = vdw, elec, slow, vdw at lambda2, elec at lambda2, slow at lambda2 */
 static void pair_fp64_alch(const PairConst* c, double r2, double qq, double A, double B, int pi, int pj,
                            const AlchF* al, double* f, double* fs, double* e) {
@@ -188,9 +216,18 @@ static void pair_fp64_alch(const PairConst* c, double r2, double qq, double A, d
   const double sh = up ? al->shUp : down ? al->shDown : 0.0;
   const double sh2 = up ? al->sh2Up : down ? al->sh2Down : 0.0;
   double S, S2, U, F, U2, F2;
-  vdw_switch64(c, r2, &S, &S2);
-  vdw_fe64(A, B, r2 + sh, &U, &F);
-  vdw_fe64(A, B, r2 + sh2, &U2, &F2);
+  if (c->forceSwitch) {
+    /* force switching: no multiplicative switch, the switched form itself */
+    S = 1.0;
+    S2 = 0.0;
+    const int inner = r2 <= c->rs2;
+    vdw_fswitch64(c, inner, A, B, r2 + sh, c->rs2 + sh, &U, &F);
+    vdw_fswitch64(c, inner, A, B, r2 + sh2, c->rs2 + sh2, &U2, &F2);
+  } else {
+    vdw_switch64(c, r2, &S, &S2);
+    vdw_fe64(A, B, r2 + sh, &U, &F);
+    vdw_fe64(A, B, r2 + sh2, &U2, &F2);
+  }
   /* alch_vdw_force = lv (S F + S2 U) is -2 dE/d(r^2); here f = 2 dE/d(r^2) */
   const double fv = -lv * (S * F + S2 * U);
   e[0] += lv * S * U;
@@ -444,11 +481,11 @@ int oracle_nb_table(int n, const double* pos, const double* q, const int32_t* ty
 /* lam[12] = elecUp, elecDown, vdwUp, vdwDown, elec2Up, elec2Down, vdw2Up,
  * vdw2Down, shiftUp, shiftDown, shift2Up, shift2Down; energies[6] = vdw, elec,
  * slow and the same at lambda2. */
-int oracle_nb_fp64_alch(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
-                        const double* A, const double* B, const int64_t* eptr, const int32_t* eidx,
-                        const double L[3], double cutoff, double switchdist, double beta, const int8_t* part,
-                        const double* lam, int decouple, double* f, double* fs, double* energies, double* virial,
-                        double* virialSlow, int64_t* npairs, int nthreads) {
+static int nb_fp64_alch_core(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+                             const double* A, const double* B, const int64_t* eptr, const int32_t* eidx,
+                             const double L[3], double cutoff, double switchdist, double beta, const int8_t* part,
+                             const double* lam, int decouple, double* f, double* fs, double* energies,
+                             double* virial, double* virialSlow, int64_t* npairs, int nthreads, int forceSwitch) {
   AlchF al;
   al.elecUp = lam[0]; al.elecDown = lam[1]; al.vdwUp = lam[2]; al.vdwDown = lam[3];
   al.elec2Up = lam[4]; al.elec2Down = lam[5]; al.vdw2Up = lam[6]; al.vdw2Down = lam[7];
@@ -456,7 +493,28 @@ int oracle_nb_fp64_alch(int n, const double* pos, const double* q, const int32_t
   al.decouple = decouple;
   al.coeff = lam[12];
   return nb_fp64_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, part, &al, NULL, f,
-                      fs, energies, 12, virial, virialSlow, npairs, nthreads, 0);
+                      fs, energies, 12, virial, virialSlow, npairs, nthreads, forceSwitch);
+}
+
+int oracle_nb_fp64_alch(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+                        const double* A, const double* B, const int64_t* eptr, const int32_t* eidx,
+                        const double L[3], double cutoff, double switchdist, double beta, const int8_t* part,
+                        const double* lam, int decouple, double* f, double* fs, double* energies, double* virial,
+                        double* virialSlow, int64_t* npairs, int nthreads) {
+  return nb_fp64_alch_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, part, lam,
+                           decouple, f, fs, energies, virial, virialSlow, npairs, nthreads, 0);
+}
+
+/* The same with vdW force switching (alchVdwForceSwitching of the GPU kernel,
+ * vdwForceSwitching of the CPU FEP/TI pair code). */
+int oracle_nb_fp64_alch_vdw_force_switch(int n, const double* pos, const double* q, const int32_t* type,
+                                         int ntypes, const double* A, const double* B, const int64_t* eptr,
+                                         const int32_t* eidx, const double L[3], double cutoff, double switchdist,
+                                         double beta, const int8_t* part, const double* lam, int decouple,
+                                         double* f, double* fs, double* energies, double* virial,
+                                         double* virialSlow, int64_t* npairs, int nthreads) {
+  return nb_fp64_alch_core(n, pos, q, type, ntypes, A, B, eptr, eidx, L, cutoff, switchdist, beta, part, lam,
+                           decouple, f, fs, energies, virial, virialSlow, npairs, nthreads, 1);
 }
 
 /* ---- force-matching adjoint ------------------------------------------------

@@ -56,6 +56,7 @@ def lib():
         L.oracle_nb_fp64_alch.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, i64p, i32p, dp, C.c_double,
                                           C.c_double, C.c_double, i8p, dp, C.c_int, dp, dp, dp, dp, dp, i64p,
                                           C.c_int]
+        L.oracle_nb_fp64_alch_vdw_force_switch.argtypes = L.oracle_nb_fp64_alch.argtypes
     return _LIB
 
 
@@ -91,9 +92,10 @@ ALCH_KEYS = ("elecLambdaUp", "elecLambdaDown", "vdwLambdaUp", "vdwLambdaDown", "
 
 
 def nb_fp64_alch(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff, switchdist, beta, partition,
-                 alch, nthreads=0):
+                 alch, nthreads=0, vdw_force_switching=False):
     """FP64 alchemical oracle.  alch: mapping with ALCH_KEYS, 'alchDecouple'
-    and optionally 'alchVdwShiftCoeff' (an AlchData works).  Returns nb_fp64's
+    and optionally 'alchVdwShiftCoeff' (an AlchData works); vdw_force_switching
+    selects NAMD's force-switched soft-core LJ.  Returns nb_fp64's
     dict plus the lambda2 energies e_vdw_s, e_elec_s, e_slow_s and the TI
     derivatives e_vdw_ti_1/2, e_elec_ti_1/2, e_slow_ti_1/2 (1: partition 1,
     2: partition 2)."""
@@ -109,7 +111,8 @@ def nb_fp64_alch(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff,
     vir = np.zeros(9)
     virs = np.zeros(9)
     npairs = np.zeros(1, np.int64)
-    lib().oracle_nb_fp64_alch(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32),
+    fn = lib().oracle_nb_fp64_alch_vdw_force_switch if vdw_force_switching else lib().oracle_nb_fp64_alch
+    fn(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32),
                               ntypes, _c(A, np.float64).reshape(-1), _c(B, np.float64).reshape(-1),
                               _c(excl_ptr, np.int64), _c(excl_idx, np.int32), _c(box, np.float64), cutoff,
                               switchdist, beta, _c(partition, np.int8), lam, int(bool(get("alchDecouple"))), f, fs,
