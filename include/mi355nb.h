@@ -325,8 +325,13 @@ int mi355nb_compute_set_force_table(mi355nb_compute* c, int enable);
  *   r <= switchdist: E = A (r^-12 - 1/(s^6 c^6)) - B (r^-6 - 1/(s^3 c^3)),
  *   r >  switchdist: E = A ka (r^-6 - c^-6)^2 - B kb (r^-3 - c^-3)^2,
  * ka = c^6/(c^6 - s^6), kb = c^3/(c^3 - s^3), instead of the energy switch.
- * Electrostatics are unchanged.  Not combinable with table mode, alchemical
- * partitions or the force-matching adjoint (MI355NB_ERR_STATE). */
+ * Electrostatics are unchanged.  With alchemical partitions (doAlch +
+ * doAlchVdwForceSwitching, calcForceEnergyFEP/TI with vdwForceSwitch,
+ * src/CudaComputeNonbondedKernel.cu:219-304, :418-473) alchemical pairs take
+ * the CPU code's force-switched soft-core form (ComputeNonbondedFEP.C:136-159,
+ * ComputeNonbondedTI.C:113-128): x = r^2 + shift and the switching distance
+ * moved to switchdist^2 + shift.  Not combinable with table mode or the
+ * force-matching adjoint (MI355NB_ERR_STATE). */
 int mi355nb_compute_set_vdw_force_switching(mi355nb_compute* c, int enable);
 
 /* Launch shape of the pair kernel on non-pairlist steps: persistent

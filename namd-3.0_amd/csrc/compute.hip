@@ -611,7 +611,6 @@ int mi355nb_compute_set_alch(mi355nb_compute* c, const int8_t* partition, int nu
   }
   CHECK_ARG(perSystem && numSystems >= 1, "bad alchemical arguments");
   CHECK_STATE(!c->tableMode, "table mode and alchemical partitions cannot be combined");
-  CHECK_STATE(!c->vdwForceSwitch, "vdW force switching and alchemical partitions cannot be combined");
   if (c->haveLayout) CHECK_ARG(numSystems == c->L.numSystems, "one alchemical record per system");
   for (int i = 0; i < c->natoms; ++i) CHECK_ARG(partition[i] >= 0 && partition[i] <= 2, "partition must be 0, 1 or 2");
   const float c2 = (float)(c->p.cutoff * c->p.cutoff), s2 = (float)(c->p.switchdist * c->p.switchdist);
@@ -1014,7 +1013,6 @@ int mi355nb_compute_set_vdw_force_switching(mi355nb_compute* c, int enable) {
   CHECK_ARG(c, "null compute");
   if (enable) {
     CHECK_STATE(!c->tableMode, "table mode and vdW force switching cannot be combined");
-    CHECK_STATE(!c->alchOn, "vdW force switching and alchemical partitions cannot be combined");
     // constants of ComputeNonbondedUtil.C:523-533, in FP64 then rounded once
     const double s2 = c->p.switchdist * c->p.switchdist, c2 = c->p.cutoff * c->p.cutoff;
     const double s6 = s2 * s2 * s2, c6 = c2 * c2 * c2, s3 = s2 * c->p.switchdist, c3 = c2 * c->p.cutoff;

@@ -215,12 +215,46 @@ __device__ __forceinline__ void pairForce(const float r2, const float qq, const 
 // pairs into (eVdw3, eElec3, eSlow3):
 //    vdW   S (U(r^2 + sh) - lv shiftCoeff dU/dx(r^2 + sh))   (sh = shiftCoeff (1 - lv))
 //    elec  the unscaled fast and slow energies.
-template <bool doEnergy, bool doSlow, bool ti, typename Acc, typename AccTi>
+//
+// With forceSwitch the LJ is the force-switched soft-core form of the CPU
+// FEP/TI code (ComputeNonbondedFEP.C:136-159, ComputeNonbondedTI.C:113-128;
+// calcForceEnergyFEP/TI with vdwForceSwitch, CudaComputeNonbondedKernel.cu:219-304,
+// :418-473): S = 1, U the switched energy at x = r^2 + sh with the switching
+// distance moved to switchdist^2 + sh (fswitchSoftcore), the TI sum the same
+// S (U - lv shiftCoeff dU/dx).
+//
+// Force-switched LJ at x = r^2 + sh (inv = 1/x, i6 = inv^3) with switching
+// distance^2 sd = switchdist^2 + sh and the cutoff unshifted
+// (vdw_fswitch_forceandenergy / vdw_fswitch_shift, src/ComputeNonbondedAlch.h:56-104);
+// inner: the unshifted r^2 is within switchdist.  U includes the constant of
+// the inner branch; dU = 2 dU/dx.  sh = 0 gives the plain force-switched LJ.
+template <bool wantDU>
+__device__ __forceinline__ void fswitchSoftcore(const bool inner, const float A, const float B, const float inv,
+                                                const float i6, const float sd, const FsConstants& fc, float& U,
+                                                float& dU) {
+  const float sdh = sd * __builtin_sqrtf(sd);   // sd^(3/2)
+  const float sd3 = sdh * sdh;
+  if (inner) {
+    const float ai6 = A * i6;
+    U = i6 * (ai6 - B) + (B * fc.c3inv * __builtin_amdgcn_rcpf(sdh) - A * fc.c6inv * __builtin_amdgcn_rcpf(sd3));
+    if (wantDU) dU = (ai6 + (ai6 - B)) * i6 * -6.0f * inv;
+  } else {
+    const float ka = A * __builtin_amdgcn_rcpf(1.0f - sd3 * fc.c6inv);
+    const float kb = B * __builtin_amdgcn_rcpf(1.0f - sdh * fc.c3inv);
+    const float i3 = inv * __builtin_sqrtf(inv);
+    const float ta = i6 - fc.c6inv, tb = i3 - fc.c3inv;
+    U = ka * ta * ta - kb * tb * tb;
+    if (wantDU) dU = (6.0f * kb * tb * i3 - 12.0f * ka * ta * i6) * inv;
+  }
+}
+
+template <bool doEnergy, bool doSlow, bool ti, bool forceSwitch, typename Acc, typename AccTi>
 __device__ __forceinline__ void pairForceAlch(const float r2, const float qq, const float A, const float B,
                                               const NbConstants& c, const float beta3, const int pi,
                                               const int pj, const AlchParams& ap, float& f, float& fSlow,
                                               Acc& eVdw, Acc& eElec, Acc& eSlow, Acc& eVdw2, Acc& eElec2,
-                                              Acc& eSlow2, AccTi& eVdw3, AccTi& eElec3, AccTi& eSlow3) {
+                                              Acc& eSlow2, AccTi& eVdw3, AccTi& eElec3, AccTi& eSlow3,
+                                              const FsConstants& fc) {
   const bool ref = (pi | pj) == 0;
   const bool cross = pi != 0 && pj != 0 && pi != pj;
   const bool alch = !ref && !cross;
@@ -235,14 +269,21 @@ __device__ __forceinline__ void pairForceAlch(const float r2, const float qq, co
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float rinv3 = rinv * rinv * rinv;
   const bool sw = r2 > c.lj_5;
-  const float delta_r = c.lj_4 - r2;
-  const float s = sw ? delta_r * delta_r * (c.lj_0 + c.lj_1 * r2) : 1.0f;
-  const float ds_r = sw ? delta_r * (c.lj_3 + c.lj_2 * r2) : 0.0f;
   const float inv = __builtin_amdgcn_rcpf(r2 + sh);
   const float i6 = inv * inv * inv;
-  const float ai6 = A * i6;
-  const float U = i6 * (ai6 - B);
-  const float dU = (ai6 + (ai6 - B)) * i6 * -6.0f * inv;
+  float s, ds_r, U, dU;
+  if constexpr (forceSwitch) {
+    s = 1.0f;
+    ds_r = 0.0f;
+    fswitchSoftcore<true>(!sw, A, B, inv, i6, c.lj_5 + sh, fc, U, dU);
+  } else {
+    const float delta_r = c.lj_4 - r2;
+    s = sw ? delta_r * delta_r * (c.lj_0 + c.lj_1 * r2) : 1.0f;
+    ds_r = sw ? delta_r * (c.lj_3 + c.lj_2 * r2) : 0.0f;
+    const float ai6 = A * i6;
+    U = i6 * (ai6 - B);
+    dU = (ai6 + (ai6 - B)) * i6 * -6.0f * inv;
+  }
   f = le * qq * (c.e_0 - rinv3) + lv * (U * ds_r + dU * s);
   float slowE = 0.0f;
   if (doSlow) {
@@ -282,7 +323,11 @@ __device__ __forceinline__ void pairForceAlch(const float r2, const float qq, co
     if (doSlow) slowE = qq * erfcf(r * c.ewald_0) * rinv - fastE;
     const float inv2 = __builtin_amdgcn_rcpf(r2 + sh2);
     const float j6 = inv2 * inv2 * inv2;
-    const float U2 = j6 * (A * j6 - B);
+    float U2, unused;
+    if constexpr (forceSwitch)
+      fswitchSoftcore<false>(!sw, A, B, inv2, j6, c.lj_5 + sh2, fc, U2, unused);
+    else
+      U2 = j6 * (A * j6 - B);
     eVdw.p += lv * (U * s);
     eVdw2.p += lv2 * (U2 * s);
     eElec.p += le * fastE;
@@ -387,12 +432,15 @@ constexpr int PAIR_ALCH = 1;
 constexpr int PAIR_TABLE = 2;
 constexpr int PAIR_ALCH_TI = 3;
 constexpr int PAIR_DIRECT_FS = 4;   // direct math, force-switched LJ
+constexpr int PAIR_ALCH_FS = 5;     // FEP, force-switched LJ
+constexpr int PAIR_ALCH_TI_FS = 6;  // TI, force-switched LJ
 
 template <bool doEnergy, bool doVirial, bool doSlow, int mode>
 __global__ void __launch_bounds__(64 * NB_WAVES)
 nonbondedForceKernel(const ForceArgs args) {
-  constexpr bool doAlch = mode == PAIR_ALCH || mode == PAIR_ALCH_TI;
-  constexpr bool doTi = mode == PAIR_ALCH_TI;
+  constexpr bool doAlch = mode == PAIR_ALCH || mode == PAIR_ALCH_TI || mode == PAIR_ALCH_FS || mode == PAIR_ALCH_TI_FS;
+  constexpr bool doTi = mode == PAIR_ALCH_TI || mode == PAIR_ALCH_TI_FS;
+  constexpr bool fsw = mode == PAIR_DIRECT_FS || mode == PAIR_ALCH_FS || mode == PAIR_ALCH_TI_FS;
   extern __shared__ float2 s_lj[];   // vdwWidth^2 (A, B)
   __shared__ F4 s_xyzq[NB_WAVES][TILE];
   __shared__ int s_type[NB_WAVES][TILE];     // vdw type | partition << 16 (alchemical)
@@ -497,16 +545,18 @@ nonbondedForceKernel(const ForceArgs args) {
             const int tw = st[j];
             const float2 ab = s_lj[vdwRow + (tw & 0xffff)];
             if (visitAlch) {
-              pairForceAlch<doEnergy, doSlow, doTi>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, iPart, tw >> 16,
-                                                    ap, f, fs, eVdw, eElec, eSlow, eVdw2, eElec2, eSlow2, eVdw3,
-                                                    eElec3, eSlow3);
+              pairForceAlch<doEnergy, doSlow, doTi, fsw>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, iPart,
+                                                         tw >> 16, ap, f, fs, eVdw, eElec, eSlow, eVdw2, eElec2,
+                                                         eSlow2, eVdw3, eElec3, eSlow3, args.fsw);
             } else if constexpr (doTi) {
               // every pair here is (0,0): plain math, no dU/dlambda
-              pairForce<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, eVdw, eElec, eSlow);
+              pairForce<doEnergy, doSlow, fsw>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, eVdw, eElec,
+                                               eSlow, &args.fsw);
             } else {
               // every pair here is (0,0): plain math; its lambda2 terms equal its lambda terms
               EnergyAcc<doEnergy> tv, te, ts;
-              pairForce<doEnergy, doSlow>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, tv, te, ts);
+              pairForce<doEnergy, doSlow, fsw>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, tv, te, ts,
+                                               &args.fsw);
               if (doEnergy) {
                 eVdw.p += tv.p; eVdw2.p += tv.p;
                 eElec.p += te.p; eElec2.p += te.p;
@@ -519,8 +569,8 @@ nonbondedForceKernel(const ForceArgs args) {
                                              args.tableSize, f, fs, eVdw, eElec, eSlow);
           } else {
             const float2 ab = s_lj[vdwRow + st[j]];
-            pairForce<doEnergy, doSlow, mode == PAIR_DIRECT_FS>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs,
-                                                                eVdw, eElec, eSlow, &args.fsw);
+            pairForce<doEnergy, doSlow, fsw>(r2, xi.w * xj.w, ab.x, ab.y, args.c, beta3, f, fs, eVdw, eElec, eSlow,
+                                             &args.fsw);
           }
           const float fx = dx * f, fy = dy * f, fz = dz * f;
           ifx += fx; ify += fy; ifz += fz;
@@ -1038,8 +1088,9 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
   const int wgCap = h.wgPerCU <= 0 ? wgNeeded : numCU * h.wgPerCU;
   dim3 g(std::min(wgNeeded, wgCap)), b(64 * NB_WAVES);
   const size_t shmem = sizeof(float2) * (size_t)h.vdwWidth * h.vdwWidth;
-  const int mode = h.alch ? (h.alchTi ? PAIR_ALCH_TI : PAIR_ALCH)
-                          : (h.forceTable ? PAIR_TABLE : (h.vdwForceSwitch ? PAIR_DIRECT_FS : PAIR_DIRECT));
+  const bool fs = h.vdwForceSwitch != 0;
+  const int mode = h.alch ? (h.alchTi ? (fs ? PAIR_ALCH_TI_FS : PAIR_ALCH_TI) : (fs ? PAIR_ALCH_FS : PAIR_ALCH))
+                          : (h.forceTable ? PAIR_TABLE : (fs ? PAIR_DIRECT_FS : PAIR_DIRECT));
   const int key = mode * 8 + (doEnergy ? 4 : 0) + (doVirial ? 2 : 0) + (doSlow ? 1 : 0);
   switch (key) {
 #define NB_CASE(M, E, V, S)                                                                  \
@@ -1049,7 +1100,7 @@ hipError_t launchNonbondedForce(int maxLists, const ForceArgsHost& h, bool doEne
 #define NB_MODE(M)                                                                           \
   NB_CASE(M, 0, 0, 0) NB_CASE(M, 0, 0, 1) NB_CASE(M, 0, 1, 0) NB_CASE(M, 0, 1, 1)            \
   NB_CASE(M, 1, 0, 0) NB_CASE(M, 1, 0, 1) NB_CASE(M, 1, 1, 0) NB_CASE(M, 1, 1, 1)
-    NB_MODE(0) NB_MODE(1) NB_MODE(2) NB_MODE(3) NB_MODE(4)
+    NB_MODE(0) NB_MODE(1) NB_MODE(2) NB_MODE(3) NB_MODE(4) NB_MODE(5) NB_MODE(6)
 #undef NB_MODE
 #undef NB_CASE
     default:
