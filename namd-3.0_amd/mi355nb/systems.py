@@ -449,6 +449,30 @@ def chain_bonded_terms(s: System):
             np.full(len(angles), BEAD_ANGLE_K), theta0)
 
 
+def chain_pairs_14(s: System):
+    """The chains' 1-4 pairs (bead k, bead k + 3), which the full exclusion
+    lists (1-2, 1-3) leave as ordinary pairs, as an atom CSR (mod_ptr int64
+    (N+1), mod_idx int32) listing each pair once from its lower atom: NAMD's
+    modified exclusions under 'exclude scaled1-4'."""
+    L = s.chain_len
+    nch = s.n_beads // L if L else 0
+    cnt = np.zeros(s.natoms + 1, np.int64)
+    idx = []
+    for c in range(nch):
+        o = c * L
+        for k in range(L - 3):
+            cnt[o + k + 1] += 1
+            idx.append(o + k + 3)
+    return np.cumsum(cnt), np.array(idx, np.int32)
+
+
+def lj14_tables(s: System, scale_a: float = 0.5, scale_b: float = 0.7):
+    """1-4 LJ tables (A14, B14): the ordinary tables scaled (a stand-in for
+    CHARMM's separate 1-4 parameters, LJTable's second entry)."""
+    A, B = lj_tables(s)
+    return A * scale_a, B * scale_b
+
+
 def lj_tables(s: System):
     """Ordered LJ tables (A, B), ntypes x ntypes: the System's own when it
     carries them, else Lorentz-Berthelot mixing of lj_sigma / lj_eps."""

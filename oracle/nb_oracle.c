@@ -25,6 +25,9 @@
  *                          (vdwForceSwitching: ComputeNonbondedUtil.C:523-533,
  *                          :900-919; alchemical: ComputeNonbondedFEP.C:136-159,
  *                          ComputeNonbondedTI.C:113-128).
+ *  oracle_mod_excl_fp64    the modified (scaled 1-4) exclusion correction
+ *                          of modifiedExclusionForce
+ *                          (ComputeBondedCUDAKernel.cu:316-632) in FP64.
  *  oracle_nb_table         table-mode pair evaluation: the pair loop of
  *                          oracle_nb_fp64 with forces and energies from given
  *                          1/r float4 tables, linearly interpolated in FP64
@@ -1230,5 +1233,90 @@ int oracle_namd_cpu(int n, const double* pos, const double* q, const int32_t* ty
   free(four);
   cells_free(&C);
   namd_tables_free(&T);
+  return 0;
+}
+
+/* ---- modified (scaled 1-4) exclusion correction ----------------------------
+ * modifiedExclusionForce (src/ComputeBondedCUDAKernel.cu:316-632) with the
+ * C1 pair forms of cudaModExclForceMagCalc_VdwEnergySwitch_PMEC1
+ * (src/CudaComputeNonbondedInteractions.h:170-254).  The tile kernel treats a
+ * modified pair as an ordinary pair; this adds, for each listed pair with
+ * r^2 < rc^2 (minimum image):
+ *   vdW   the switched LJ of the pair with (A14 - A, B14 - B)
+ *         (exclusionVdwCoefTable, src/CudaNonbondedTables.C:119-131);
+ *   fast  qq' (1/r - 3/(2rc) + r^2/(2rc^3)),   qq' = -(1 - scale14) qq;
+ *   slow  qq' (3/(2rc) - r^2/(2rc^3)).
+ * pairs: atom CSR (mptr[n+1], midx), each pair listed once or from both ends
+ * (deduplicated here).  forceSwitch selects the force-switched vdW form.
+ * energies[3] = vdw, elec, slow; virial / virialSlow as oracle_nb_fp64. */
+int oracle_mod_excl_fp64(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+                         const double* A, const double* B, const double* A14, const double* B14,
+                         const int64_t* mptr, const int32_t* midx, const double L[3], double cutoff,
+                         double switchdist, double scale14, int forceSwitch, double* f, double* fs,
+                         double* energies, double* virial, double* virialSlow, int64_t* npairs) {
+  PairConst c;
+  memset(&c, 0, sizeof(c));
+  if (forceSwitch) set_force_switch(&c, cutoff, switchdist);
+  c.rc = cutoff;
+  c.rc2 = cutoff * cutoff;
+  c.rs2 = switchdist * switchdist;
+  c.c1 = (switchdist < cutoff) ? 1.0 / ((c.rc2 - c.rs2) * (c.rc2 - c.rs2) * (c.rc2 - c.rs2)) : 0.0;
+  c.rcinv = 1.0 / cutoff;
+  c.rc3inv = c.rcinv * c.rcinv * c.rcinv;
+  if (f) memset(f, 0, sizeof(double) * 3 * n);
+  if (fs) memset(fs, 0, sizeof(double) * 3 * n);
+  double e[3] = {0, 0, 0}, vir[9] = {0}, virs[9] = {0};
+  int64_t np = 0;
+  for (int i = 0; i < n; ++i) {
+    for (int64_t k = mptr[i]; k < mptr[i + 1]; ++k) {
+      const int j = midx[k];
+      if (j == i) continue;
+      /* each unordered pair once: skip (i, j) with j < i if (j, i) is listed too */
+      if (j < i) {
+        int dup = 0;
+        for (int64_t m = mptr[j]; m < mptr[j + 1]; ++m)
+          if (midx[m] == i) dup = 1;
+        if (dup) continue;
+      }
+      /* drop repeats of the same partner in i's own list */
+      int rep = 0;
+      for (int64_t m = mptr[i]; m < k; ++m)
+        if (midx[m] == j) rep = 1;
+      if (rep) continue;
+      double d[3];
+      for (int u = 0; u < 3; ++u) d[u] = min_image(pos[3 * j + u] - pos[3 * i + u], L[u]);
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (!(r2 < c.rc2)) continue;
+      const int ab = type[i] * ntypes + type[j];
+      const double qq = -(1.0 - scale14) * q[i] * q[j];
+      double ff, ffs, ev = 0.0, ee = 0.0, es = 0.0;
+      /* vdW with the difference coefficients; pair_fp64 with qq = 0 and beta = 0
+       * gives only the switched LJ (its slow term vanishes with qq) */
+      pair_fp64(&c, r2, 0.0, A14[ab] - A[ab], B14[ab] - B[ab], &ff, &ffs, &ev, &ee, &es);
+      const double r = sqrt(r2), rinv = 1.0 / r;
+      /* (1/r) dE/dr of the electrostatic parts */
+      ff += qq * (-rinv * rinv * rinv + c.rc3inv);
+      ee = qq * (rinv - 1.5 * c.rcinv + 0.5 * r2 * c.rc3inv);
+      ffs = -qq * c.rc3inv;
+      es = qq * (1.5 * c.rcinv - 0.5 * r2 * c.rc3inv);
+      e[0] += ev;
+      e[1] += ee;
+      e[2] += es;
+      for (int u = 0; u < 3; ++u) {
+        if (f) { f[3 * i + u] += ff * d[u]; f[3 * j + u] -= ff * d[u]; }
+        if (fs) { fs[3 * i + u] += ffs * d[u]; fs[3 * j + u] -= ffs * d[u]; }
+      }
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          vir[a * 3 + b] -= ff * d[a] * d[b];
+          virs[a * 3 + b] -= ffs * d[a] * d[b];
+        }
+      ++np;
+    }
+  }
+  if (energies) memcpy(energies, e, sizeof(e));
+  if (virial) memcpy(virial, vir, sizeof(vir));
+  if (virialSlow) memcpy(virialSlow, virs, sizeof(virs));
+  if (npairs) *npairs = np;
   return 0;
 }

@@ -1,0 +1,111 @@
+"""Pins of the oracle's modified (scaled 1-4) exclusion correction
+(modifiedExclusionForce, src/ComputeBondedCUDAKernel.cu:316-632, C1 forms of
+src/CudaComputeNonbondedInteractions.h:170-254), by what NAMD's scaled 1-4
+rule fixes: a listed pair keeps its ordinary tile-kernel interaction, and the
+correction turns it into (a) LJ with the 1-4 coefficients and (b) Coulomb
+scaled by scale14 minus the Ewald smooth part:
+  * scale14 = 1 with A14 = A, B14 = B: the correction is zero;
+  * one pair: ordinary pair (plain oracle) + correction has total
+    electrostatics s14 qq/r - qq erf(beta r)/r (closed form) and the LJ of the
+    plain oracle run with the 1-4 tables;
+  * its fast part is -(1 - s14) times the ordinary pair's fast part;
+  * forces are minus the gradient of the correction's energy (central
+    differences) on a box with its chains' 1-4 pairs listed;
+  * a pair listed from both ends counts once.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mi355nb import lib as L
+from mi355nb import systems
+from oracle import oracle as orc
+
+COULOMB = 332.0636
+
+
+def _pair_system(r, qi=0.6, qj=-0.45):
+    pos = np.array([[20.0, 20.0, 20.0], [20.0 + r * 0.6, 20.0 + r * 0.8, 20.0]])
+    q = np.array([qi, qj]) * math.sqrt(COULOMB)
+    t = np.array([0, 1], np.int32)
+    A = np.array([[4.0e5, 6.0e5], [6.0e5, 9.0e5]])
+    B = np.array([[5.0e2, 7.0e2], [7.0e2, 9.0e2]])
+    return pos, q, t, A, B
+
+
+def _mod(pos, q, t, A, B, A14, B14, s14, fs=False, both_ends=False):
+    ptr = np.array([0, 1, 2 if both_ends else 1], np.int64)
+    idx = np.array([1, 0] if both_ends else [1], np.int32)
+    return orc.mod_excl_fp64(pos, q, t, A, B, A14, B14, ptr, idx, np.array([60.0, 60.0, 60.0]), 12.0, 10.0, s14,
+                             vdw_force_switching=fs)
+
+
+@pytest.mark.parametrize("r", [1.6, 3.9, 10.7])
+def test_unit_scale_and_same_tables_give_zero(r):
+    pos, q, t, A, B = _pair_system(r)
+    o = _mod(pos, q, t, A, B, A, B, 1.0)
+    assert o["npairs"] == 1
+    assert o["e_vdw"] == 0.0 and o["e_elec"] == 0.0 and o["e_slow"] == 0.0
+    assert not o["f"].any() and not o["fs"].any()
+
+
+@pytest.mark.parametrize("fs", [False, True])
+@pytest.mark.parametrize("r", [1.6, 3.9, 10.7])
+def test_one_pair_closed_forms(r, fs):
+    pos, q, t, A, B = _pair_system(r)
+    A14, B14 = 0.5 * A, 0.7 * B
+    s14, beta = 0.8333, 0.32
+    box = np.array([60.0, 60.0, 60.0])
+    none_p, none_i = np.zeros(3, np.int64), np.zeros(0, np.int32)
+    plain = orc.nb_fp64(pos, q, t, A, B, none_p, none_i, box, 12.0, 10.0, beta, vdw_force_switching=fs)
+    plain14 = orc.nb_fp64(pos, q * 0, t, A14, B14, none_p, none_i, box, 12.0, 10.0, beta, vdw_force_switching=fs)
+    corr = _mod(pos, q, t, A, B, A14, B14, s14, fs=fs)
+    qq = q[0] * q[1]
+    total_elec = plain["e_elec"] + plain["e_slow"] + corr["e_elec"] + corr["e_slow"]
+    want = s14 * qq / r - qq * math.erf(beta * r) / r
+    assert abs(total_elec - want) <= 1e-12 * abs(qq / r)
+    assert abs(plain["e_vdw"] + corr["e_vdw"] - plain14["e_vdw"]) <= 1e-12 * max(1.0, abs(plain14["e_vdw"]))
+    assert abs(corr["e_elec"] + (1 - s14) * plain["e_elec"]) <= 1e-12 * abs(qq / r)
+    # forces: the LJ part turns the plain LJ force into the 1-4 one
+    plain_lj = orc.nb_fp64(pos, q * 0, t, A, B, none_p, none_i, box, 12.0, 10.0, beta, vdw_force_switching=fs)
+    elec_fast = plain["f"] - plain_lj["f"]
+    want_f = plain14["f"] - plain_lj["f"] - (1 - s14) * elec_fast
+    assert np.abs(corr["f"] - want_f).max() <= 1e-10 * np.abs(want_f).max()
+
+
+def test_pair_listed_from_both_ends_counts_once():
+    pos, q, t, A, B = _pair_system(3.0)
+    one = _mod(pos, q, t, A, B, 0.5 * A, 0.7 * B, 0.5)
+    two = _mod(pos, q, t, A, B, 0.5 * A, 0.7 * B, 0.5, both_ends=True)
+    assert two["npairs"] == 1
+    assert two["e_vdw"] == one["e_vdw"] and two["e_elec"] == one["e_elec"]
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_forces_are_energy_gradient(fs):
+    s = systems.small_box(n_waters=60, n_chains=3, chain_len=20, box=20.0, seed=5)
+    p = s.params
+    A, B = systems.lj_tables(s)
+    A14, B14 = systems.lj14_tables(s)
+    mptr, midx = systems.chain_pairs_14(s)
+    q = s.charge * math.sqrt(COULOMB)
+
+    def run(pos):
+        return orc.mod_excl_fp64(pos, q, s.vdw_type, A, B, A14, B14, mptr, midx, s.box, p.cutoff, p.switchdist,
+                                 0.8333, vdw_force_switching=fs)
+
+    o = run(s.pos)
+    assert o["npairs"] == 3 * 17
+    for a in (0, 5, 21, 47):
+        for d in range(3):
+            h = 1e-5
+            xp = s.pos.copy(); xp[a, d] += h
+            xm = s.pos.copy(); xm[a, d] -= h
+            ep, em = run(xp), run(xm)
+            g_fast = (ep["e_vdw"] + ep["e_elec"] - em["e_vdw"] - em["e_elec"]) / (2 * h)
+            g_slow = (ep["e_slow"] - em["e_slow"]) / (2 * h)
+            assert abs(o["f"][a, d] + g_fast) <= 1e-5 * max(1.0, abs(g_fast)), (a, d)
+            assert abs(o["fs"][a, d] + g_slow) <= 1e-5 * max(1.0, abs(g_slow)), (a, d)
+    # pair forces are central, so the virial is symmetric
+    assert np.allclose(o["virial"], o["virial"].T, rtol=0, atol=1e-9 * np.abs(o["virial"]).max())
