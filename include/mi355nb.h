@@ -312,6 +312,31 @@ int mi355nb_compute_get_slot_of_atom(const mi355nb_compute* c, int32_t* slotOfAt
  * from the nonbonded VirialEnergy. */
 int mi355nb_compute_exclusion_correction(mi355nb_compute* c, double* energyVirial);
 
+/* Modified (scaled 1-4) exclusions under 'exclude scaled1-4'.  NAMD's pair
+ * kernel skips only full exclusions (buildExclusions takes
+ * get_full_exclusions_for_atom, src/CudaComputeNonbonded.C:2221), so a
+ * modified pair goes through it as an ordinary pair, and the bonded kernel's
+ * modifiedExclusionForce (src/ComputeBondedCUDAKernel.cu:316-632, pair forms
+ * src/CudaComputeNonbondedInteractions.h:170-254) corrects it.
+ * set_modified_exclusions gives the pairs (atom CSR modPtr[natoms+1] /
+ * modIdx, each pair from one or both ends; none may be a full exclusion),
+ * scale14 (SimParameters::scale14) and the 1-4 LJ tables A14, B14
+ * (ntypes^2, LJTable's 1-4 entries); modPtr NULL switches it off.
+ * set_molecule clears it.  modified_exclusion_correction then adds, for each
+ * pair with r^2 < cutoff^2, evaluated in FP64:
+ *   vdW   the pair kernel's switched LJ (energy- or force-switched) with
+ *         (A14 - A, B14 - B) (exclusionVdwCoefTable,
+ *         src/CudaNonbondedTables.C:119-131), to the fast forces;
+ *   fast  qq' (1/r - 3/(2 rc) + r^2/(2 rc^3)), qq' = -(1 - scale14) qq;
+ *   slow  qq' (3/(2 rc) - r^2/(2 rc^3)), to the slow forces,
+ * to the forces of the last launch (which needs doSlow), enqueued on the
+ * compute's stream.  energyVirial: device doubles, 21 per system {E_vdw,
+ * E_elec, E_slow, virial[9], virialSlow[9]}, overwritten; NULL: forces only.
+ * Not with alchemical partitions (MI355NB_ERR_STATE). */
+int mi355nb_compute_set_modified_exclusions(mi355nb_compute* c, const int64_t* modPtr, const int32_t* modIdx,
+                                            double scale14, const double* A14, const double* B14);
+int mi355nb_compute_modified_exclusion_correction(mi355nb_compute* c, double* energyVirial);
+
 /* Table mode (doTable of CudaComputeNonbondedKernel::nonbondedForce,
  * src/CudaComputeNonbondedKernel.cu:99-161): later launches take LJ and
  * electrostatics from 4096-entry 1/r tables with linear interpolation

@@ -55,6 +55,12 @@ struct mi355nb_compute {
   std::vector<int> exclSymPtr;        // the same pairs in both directions, deduplicated, per atom
   std::vector<int> exclSymIdx;
   std::vector<float> vdwCoefHost;  // 2 * ntypes * ntypes
+  std::vector<double> ljAHost, ljBHost;  // the FP64 LJ tables as last given
+  // modified (scaled 1-4) exclusions (optional)
+  bool haveMod = false;
+  double scale14 = 1.0;
+  std::vector<double> lj14A, lj14B;
+  std::vector<int> modSymPtr, modSymIdx;  // both directions, deduplicated, per atom
   bool haveMolecule = false;
   bool haveLayout = false;
   bool haveLists = false;
@@ -115,6 +121,8 @@ struct mi355nb_compute {
   DevBuf<int4> exclBlocks;
   DevBuf<int> exclSlotPtr, exclSlotIdx, exclSysStart;
   DevBuf<double> exclPartials, sysBoxDev;
+  DevBuf<int> modSlotPtr, modSlotIdx;
+  DevBuf<double> modPartials, modDAB;
   DevBuf<int> patchSystemDev;
   // NAMD table mode (optional)
   bool tableMode = false;
@@ -157,6 +165,7 @@ struct mi355nb_compute {
     forceTable.release(); energyTable.release(); slotPatch.release(); patchCentre.release();
     exclSlotPtr.release(); exclSlotIdx.release(); exclBlocks.release(); exclSysStart.release();
     exclPartials.release(); sysBoxDev.release(); patchSystemDev.release();
+    modSlotPtr.release(); modSlotIdx.release(); modPartials.release(); modDAB.release();
     fmTarget.release(); fmLossPartials.release(); fmLoss.release(); fmGq.release(); fmBinPartials.release();
     fmBins.release(); fmG.release();
     xyzq.release(); force.release(); forceSlow.release();
@@ -290,6 +299,35 @@ int mi355nb_compute_destroy(mi355nb_compute* c) {
   return MI355NB_OK;
 }
 
+// Symmetric closure of a CSR pair list: every (atom, partner) in both
+// directions, sorted and deduplicated per atom.
+static void symmetricClosure(int natoms, const int64_t* ptrIn, const int32_t* idxIn, std::vector<int>& outPtr,
+                             std::vector<int>& outIdx) {
+  std::vector<int> deg(natoms + 1, 0);
+  for (int a = 0; a < natoms; ++a)
+    for (int64_t k = ptrIn[a]; k < ptrIn[a + 1]; ++k) {
+      ++deg[a + 1];
+      ++deg[idxIn[k] + 1];
+    }
+  for (int a = 0; a < natoms; ++a) deg[a + 1] += deg[a];
+  std::vector<int> fill(deg.begin(), deg.end() - 1), idx(deg[natoms]);
+  for (int a = 0; a < natoms; ++a)
+    for (int64_t k = ptrIn[a]; k < ptrIn[a + 1]; ++k) {
+      const int b = idxIn[k];
+      idx[fill[a]++] = b;
+      idx[fill[b]++] = a;
+    }
+  outPtr.assign(natoms + 1, 0);
+  outIdx.clear();
+  outIdx.reserve(idx.size());
+  for (int a = 0; a < natoms; ++a) {
+    std::sort(idx.begin() + deg[a], idx.begin() + deg[a + 1]);
+    for (int k = deg[a]; k < deg[a + 1]; ++k)
+      if (k == deg[a] || idx[k] != idx[k - 1]) outIdx.push_back(idx[k]);
+    outPtr[a + 1] = (int)outIdx.size();
+  }
+}
+
 int mi355nb_compute_set_molecule(mi355nb_compute* c, int natoms, const int32_t* vdw_type, const double* charge,
                                  int ntypes, const double* lj_A, const double* lj_B, const int64_t* excl_ptr,
                                  const int32_t* excl_idx) {
@@ -314,32 +352,10 @@ int mi355nb_compute_set_molecule(mi355nb_compute* c, int natoms, const int32_t* 
   c->ntypes = ntypes;
   c->exclPtrHost.assign(excl_ptr, excl_ptr + natoms + 1);
   c->exclIdxHost.assign(excl_idx, excl_idx + excl_ptr[natoms]);
-  {
-    // symmetric closure of the listed pairs, one entry per (atom, partner)
-    std::vector<int> deg(natoms + 1, 0);
-    for (int a = 0; a < natoms; ++a)
-      for (int64_t k = excl_ptr[a]; k < excl_ptr[a + 1]; ++k) {
-        ++deg[a + 1];
-        ++deg[excl_idx[k] + 1];
-      }
-    for (int a = 0; a < natoms; ++a) deg[a + 1] += deg[a];
-    std::vector<int> fill(deg.begin(), deg.end() - 1), idx(deg[natoms]);
-    for (int a = 0; a < natoms; ++a)
-      for (int64_t k = excl_ptr[a]; k < excl_ptr[a + 1]; ++k) {
-        const int b = excl_idx[k];
-        idx[fill[a]++] = b;
-        idx[fill[b]++] = a;
-      }
-    c->exclSymPtr.assign(natoms + 1, 0);
-    c->exclSymIdx.clear();
-    c->exclSymIdx.reserve(idx.size());
-    for (int a = 0; a < natoms; ++a) {
-      std::sort(idx.begin() + deg[a], idx.begin() + deg[a + 1]);
-      for (int k = deg[a]; k < deg[a + 1]; ++k)
-        if (k == deg[a] || idx[k] != idx[k - 1]) c->exclSymIdx.push_back(idx[k]);
-      c->exclSymPtr[a + 1] = (int)c->exclSymIdx.size();
-    }
-  }
+  symmetricClosure(natoms, excl_ptr, excl_idx, c->exclSymPtr, c->exclSymIdx);
+  c->ljAHost.assign(lj_A, lj_A + (size_t)ntypes * ntypes);
+  c->ljBHost.assign(lj_B, lj_B + (size_t)ntypes * ntypes);
+  c->haveMod = false;
   c->vdwType.assign(vdw_type, vdw_type + natoms);
   c->qScaled.resize(natoms);
   const double qs = std::sqrt(COULOMB);  // nonbondedScaling = 1, dielectric = 1
@@ -385,6 +401,53 @@ static int uploadPartitions(mi355nb_compute* c) {
   MI355NB_HIP(c->alch.reserve(c->alchHost.size()));
   MI355NB_HIP(hipMemcpy(c->alch.ptr, c->alchHost.data(), c->alchHost.size() * sizeof(AlchParams),
                         hipMemcpyHostToDevice));
+  return MI355NB_OK;
+}
+
+// An atom-space partner CSR renumbered by storage slot (in the partners'
+// atom order); padding slots have none.
+static void slotCsr(const mi355nb_compute* c, const std::vector<int>& sp, const std::vector<int>& si,
+                    std::vector<int>& ptr, std::vector<int>& idx) {
+  const int n = c->L.atomStorageSize;
+  ptr.assign(n + 1, 0);
+  idx.resize(si.size());
+  for (int s = 0; s < n; ++s) {
+    const int a = c->L.atomIndex[s];
+    ptr[s + 1] = ptr[s] + ((a >= 0 && c->L.slotOfAtom[a] == s) ? sp[a + 1] - sp[a] : 0);
+  }
+  for (int s = 0; s < n; ++s) {
+    if (ptr[s + 1] == ptr[s]) continue;
+    const int a = c->L.atomIndex[s];
+    int* out = idx.data() + ptr[s];
+    for (int k = sp[a]; k < sp[a + 1]; ++k) *out++ = c->L.slotOfAtom[si[k]];
+  }
+}
+
+// (A14 - A, B14 - B) per ordered type pair, FP64 (NAMD's exclusionVdwCoefTable).
+static int uploadModifiedCoef(mi355nb_compute* c) {
+  const size_t nt2 = (size_t)c->ntypes * c->ntypes;
+  std::vector<double> d(2 * nt2);
+  for (size_t k = 0; k < nt2; ++k) {
+    d[2 * k] = c->lj14A[k] - c->ljAHost[k];
+    d[2 * k + 1] = c->lj14B[k] - c->ljBHost[k];
+  }
+  MI355NB_HIP(c->modDAB.reserve(d.size(), 1.0f));
+  MI355NB_HIP(hipMemcpy(c->modDAB.ptr, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
+  return MI355NB_OK;
+}
+
+// Slot CSR of the modified partners for the current layout.
+static int uploadModifiedSlots(mi355nb_compute* c) {
+  const int n = c->L.atomStorageSize;
+  std::vector<int> ptr, idx;
+  slotCsr(c, c->modSymPtr, c->modSymIdx, ptr, idx);
+  const size_t nb = c->exclBlocksHost.size();
+  MI355NB_HIP(c->modSlotPtr.reserve(n + 1, 1.0f));
+  MI355NB_HIP(c->modSlotIdx.reserve(std::max<size_t>(1, idx.size()), 1.0f));
+  MI355NB_HIP(c->modPartials.reserve(std::max<size_t>(1, nb) * MOD_EXCL_VALUES_PER_SYSTEM, 1.0f));
+  MI355NB_HIP(hipMemcpy(c->modSlotPtr.ptr, ptr.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+  if (!idx.empty())
+    MI355NB_HIP(hipMemcpy(c->modSlotIdx.ptr, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
   return MI355NB_OK;
 }
 
@@ -479,19 +542,8 @@ static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomSt
     // excluded partners of every slot (both directions, deduplicated, in
     // the molecule's partner order): the atom-space closure renumbered by
     // slot; padding slots have none
-    const std::vector<int>& sp = c->exclSymPtr;
-    const std::vector<int>& si = c->exclSymIdx;
-    std::vector<int> ptr(n + 1, 0), idx(si.size());
-    for (int s = 0; s < n; ++s) {
-      const int a = c->L.atomIndex[s];
-      ptr[s + 1] = ptr[s] + ((a >= 0 && c->L.slotOfAtom[a] == s) ? sp[a + 1] - sp[a] : 0);
-    }
-    for (int s = 0; s < n; ++s) {
-      if (ptr[s + 1] == ptr[s]) continue;
-      const int a = c->L.atomIndex[s];
-      int* out = idx.data() + ptr[s];
-      for (int k = sp[a]; k < sp[a + 1]; ++k) *out++ = c->L.slotOfAtom[si[k]];
-    }
+    std::vector<int> ptr, idx;
+    slotCsr(c, c->exclSymPtr, c->exclSymIdx, ptr, idx);
     buildExclusionBlocks(c->L.sysSlotStart.data(), nsys, c->exclBlocksHost, c->exclSysStartHost);
     const size_t nb = c->exclBlocksHost.size();
     MI355NB_HIP(c->exclSlotPtr.reserve(n + 1, 1.0f));
@@ -513,7 +565,9 @@ static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomSt
                           hipMemcpyHostToDevice));
   }
   MI355NB_HIP(launchFill(c->fsoa.cap, c->fsoa.ptr, 0.0f, c->stream));
-  int rc = uploadSlots(c);
+  int rc = c->haveMod ? uploadModifiedSlots(c) : MI355NB_OK;
+  if (rc) return rc;
+  rc = uploadSlots(c);
   if (rc) return rc;
   MI355NB_HIP(hipStreamSynchronize(c->stream));
   if (c->alchOn) {
@@ -815,8 +869,15 @@ int mi355nb_compute_update_parameters(mi355nb_compute* c, const double* charge, 
       c->vdwCoefHost[2 * k] = (float)lj_A[k];
       c->vdwCoefHost[2 * k + 1] = (float)lj_B[k];
     }
+    c->ljAHost.assign(lj_A, lj_A + (size_t)nt * nt);
+    c->ljBHost.assign(lj_B, lj_B + (size_t)nt * nt);
     MI355NB_HIP(hipMemcpyAsync(c->vdwCoef.ptr, c->vdwCoefHost.data(), c->vdwCoefHost.size() * sizeof(float),
                                hipMemcpyHostToDevice, c->stream));
+    if (c->haveMod) {
+      MI355NB_HIP(hipStreamSynchronize(c->stream));
+      const int rc = uploadModifiedCoef(c);
+      if (rc) return rc;
+    }
   }
   MI355NB_HIP(hipStreamSynchronize(c->stream));
   return MI355NB_OK;
@@ -975,6 +1036,84 @@ int mi355nb_compute_exclusion_correction(mi355nb_compute* c, double* energyViria
                                         c->slotPatch.ptr,
                                         c->patchCentre.ptr, c->sysBoxDev.ptr, c->p.cutoff, c->p.ewaldcof,
                                         c->forceSlow.ptr, c->exclPartials.ptr, energyVirial, c->stream));
+  return MI355NB_OK;
+}
+
+int mi355nb_compute_set_modified_exclusions(mi355nb_compute* c, const int64_t* modPtr, const int32_t* modIdx,
+                                            double scale14, const double* A14, const double* B14) {
+  CHECK_ARG(c, "null compute");
+  CHECK_STATE(c->haveMolecule, "set_molecule must precede set_modified_exclusions");
+  if (!modPtr) {
+    c->haveMod = false;
+    return MI355NB_OK;
+  }
+  CHECK_ARG(A14 && B14 && std::isfinite(scale14), "bad modified-exclusion arguments");
+  const int n = c->natoms;
+  CHECK_ARG(modPtr[0] == 0, "modPtr[0] must be 0");
+  for (int a = 0; a < n; ++a) {
+    CHECK_ARG(modPtr[a + 1] >= modPtr[a], "modPtr must be nondecreasing");
+    for (int64_t k = modPtr[a]; k < modPtr[a + 1]; ++k)
+      CHECK_ARG(modIdx && modIdx[k] >= 0 && modIdx[k] < n && modIdx[k] != a, "modified partner out of range");
+  }
+  // a modified pair may not also be a full exclusion
+  for (int a = 0; a < n; ++a)
+    for (int64_t k = modPtr[a]; k < modPtr[a + 1]; ++k) {
+      const auto e0 = c->exclSymIdx.begin() + c->exclSymPtr[a], e1 = c->exclSymIdx.begin() + c->exclSymPtr[a + 1];
+      CHECK_ARG(!std::binary_search(e0, e1, (int)modIdx[k]), "a pair is both a full and a modified exclusion");
+    }
+  const size_t nt2 = (size_t)c->ntypes * c->ntypes;
+  for (size_t k = 0; k < nt2; ++k) CHECK_ARG(std::isfinite(A14[k]) && std::isfinite(B14[k]), "non-finite 1-4 LJ");
+  MI355NB_HIP(hipSetDevice(c->device));
+  MI355NB_HIP(hipStreamSynchronize(c->stream));
+  symmetricClosure(n, modPtr, modIdx, c->modSymPtr, c->modSymIdx);
+  c->scale14 = scale14;
+  c->lj14A.assign(A14, A14 + nt2);
+  c->lj14B.assign(B14, B14 + nt2);
+  c->haveMod = true;
+  int rc = uploadModifiedCoef(c);
+  if (rc) return rc;
+  if (c->haveLayout) rc = uploadModifiedSlots(c);
+  return rc;
+}
+
+int mi355nb_compute_modified_exclusion_correction(mi355nb_compute* c, double* energyVirial) {
+  CHECK_ARG(c, "null compute");
+  CHECK_STATE(c->haveMod, "set_modified_exclusions must precede the modified exclusion correction");
+  CHECK_STATE(c->haveLayout && c->lastSlow, "a launch with doSlow must precede the modified exclusion correction");
+  CHECK_STATE(!c->alchOn, "the modified exclusion correction does not support alchemical partitions");
+  MI355NB_HIP(hipSetDevice(c->device));
+  ModExclArgs a;
+  a.ptr = c->modSlotPtr.ptr;
+  a.slot = c->modSlotIdx.ptr;
+  a.blocks = c->exclBlocks.ptr;
+  a.xyzq = c->xyzq.ptr;
+  a.slotPatch = c->slotPatch.ptr;
+  a.centre = c->patchCentre.ptr;
+  a.sysBox = c->sysBoxDev.ptr;
+  a.vdwTypes = c->vdwTypes.ptr;
+  a.ntypes = c->ntypes;
+  a.dAB = c->modDAB.ptr;
+  const double rc = c->p.cutoff, rs = c->p.switchdist;
+  a.rc2 = rc * rc;
+  a.rs2 = rs * rs;
+  a.c1 = rs < rc ? 1.0 / ((a.rc2 - a.rs2) * (a.rc2 - a.rs2) * (a.rc2 - a.rs2)) : 0.0;
+  a.rcinv = 1.0 / rc;
+  a.rc3inv = a.rcinv * a.rcinv * a.rcinv;
+  a.oneMinusScale14 = 1.0 - c->scale14;
+  a.forceSwitch = c->vdwForceSwitch ? 1 : 0;
+  // force-switch constants (ComputeNonbondedUtil.C:523-533)
+  const double s6 = a.rs2 * a.rs2 * a.rs2, c6 = a.rc2 * a.rc2 * a.rc2, s3 = a.rs2 * rs, c3 = a.rc2 * rc;
+  a.ka = rs < rc ? c6 / (c6 - s6) : 0.0;
+  a.kb = rs < rc ? c3 / (c3 - s3) : 0.0;
+  a.c6inv = 1.0 / c6;
+  a.c3inv = 1.0 / c3;
+  a.va = -1.0 / (s6 * c6);
+  a.vb = -1.0 / (s3 * c3);
+  a.force = c->force.ptr;
+  a.forceSlow = c->forceSlow.ptr;
+  a.partials = c->modPartials.ptr;
+  MI355NB_HIP(launchModifiedExclusionCorrection((int)c->exclBlocksHost.size(), c->L.numSystems, a,
+                                                c->exclSysStart.ptr, energyVirial, c->stream));
   return MI355NB_OK;
 }
 

@@ -116,6 +116,147 @@ exclusionFinalKernel(const int* __restrict__ sysBlockStart, const double* __rest
   }
 }
 
+// Modified (scaled 1-4) exclusions (modifiedExclusionForce,
+// src/ComputeBondedCUDAKernel.cu:316-632; pair forms of
+// cudaModExclForceMagCalc_VdwEnergySwitch_PMEC1,
+// src/CudaComputeNonbondedInteractions.h:170-254).  The pair kernel treats a
+// modified pair as an ordinary pair; this adds for every listed pair with
+// r^2 < cutoff^2
+//   vdW   the switched LJ with (A14 - A, B14 - B) (NAMD's exclusionVdwCoefTable,
+//         src/CudaNonbondedTables.C:119-131), energy- or force-switched as the
+//         pair kernel is;
+//   fast  qq' (1/r - 3/(2rc) + r^2/(2rc^3)), slow qq' (3/(2rc) - r^2/(2rc^3)),
+//         qq' = -(1 - scale14) qq,
+// so that the pair ends with 1-4 LJ and scale14 qq/r - qq erf(beta r)/r.  FP64,
+// the work layout of exclusionCorrectionKernel: one thread per slot over its
+// partners, both ends evaluated, its own slot's force written without atomics.
+constexpr int MOD_VALUES = MOD_EXCL_VALUES_PER_SYSTEM;
+
+__global__ void __launch_bounds__(EXCL_THREADS) modifiedExclusionKernel(const ModExclArgs a, int doEnergyVirial) {
+  __shared__ double sh[EXCL_THREADS / 64];
+  const int4 bk = a.blocks[blockIdx.x];
+  const int k = bk.y + threadIdx.x;
+  double v[MOD_VALUES];
+#pragma unroll
+  for (int u = 0; u < MOD_VALUES; ++u) v[u] = 0.0;
+  if (k < bk.z) {
+    const int m0 = a.ptr[k], m1 = a.ptr[k + 1];
+    if (m0 < m1) {
+      const F4 p = a.xyzq[k];
+      const int pa = a.slotPatch[k];
+      const int ta = a.vdwTypes[k];
+      const double L[3] = {a.sysBox[3 * bk.x], a.sysBox[3 * bk.x + 1], a.sysBox[3 * bk.x + 2]};
+      const double xa[3] = {a.centre[3 * pa] + (double)p.x, a.centre[3 * pa + 1] + (double)p.y,
+                            a.centre[3 * pa + 2] + (double)p.z};
+      double f[3] = {0.0, 0.0, 0.0}, fs[3] = {0.0, 0.0, 0.0};
+      for (int m = m0; m < m1; ++m) {
+        const int j = a.slot[m];
+        const F4 b = a.xyzq[j];
+        const int pb = a.slotPatch[j];
+        double d[3] = {xa[0] - (a.centre[3 * pb] + (double)b.x), xa[1] - (a.centre[3 * pb + 1] + (double)b.y),
+                       xa[2] - (a.centre[3 * pb + 2] + (double)b.z)};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) d[c] -= L[c] * rint(d[c] / L[c]);
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (!(r2 < a.rc2 && r2 > 0.0)) continue;
+        const int tb = a.vdwTypes[j];
+        const double dA = a.dAB[2 * (ta * a.ntypes + tb)], dB = a.dAB[2 * (ta * a.ntypes + tb) + 1];
+        const double rinv2 = 1.0 / r2, rinv = sqrt(rinv2), rinv3 = rinv * rinv2, rinv6 = rinv3 * rinv3;
+        // vdW energy E and g = dE/d(r^2)
+        const double U = (dA * rinv6 - dB) * rinv6;
+        const double dU = (-6.0 * dA * rinv6 + 3.0 * dB) * rinv6 * rinv2;
+        double E, g;
+        if (a.forceSwitch) {
+          if (r2 > a.rs2) {
+            const double ta6 = rinv6 - a.c6inv, tb3 = rinv3 - a.c3inv;
+            E = dA * a.ka * ta6 * ta6 - dB * a.kb * tb3 * tb3;
+            g = (-6.0 * dA * a.ka * ta6 * rinv6 + 3.0 * dB * a.kb * tb3 * rinv3) * rinv2;
+          } else {
+            E = U + dA * a.va - dB * a.vb;
+            g = dU;
+          }
+        } else if (r2 > a.rs2) {
+          const double dd = a.rc2 - r2, w = a.rc2 + 2.0 * r2 - 3.0 * a.rs2;
+          const double S = dd * dd * w * a.c1;
+          const double dS = a.c1 * (2.0 * dd * dd - 2.0 * dd * w);
+          E = U * S;
+          g = dU * S + U * dS;
+        } else {
+          E = U;
+          g = dU;
+        }
+        const double qq = -a.oneMinusScale14 * (double)p.w * (double)b.w;
+        const double Ee = qq * (rinv - 1.5 * a.rcinv + 0.5 * r2 * a.rc3inv);
+        const double ge = qq * 0.5 * (a.rc3inv - rinv3);
+        const double Es = qq * (1.5 * a.rcinv - 0.5 * r2 * a.rc3inv);
+        const double gs = -0.5 * qq * a.rc3inv;
+        // force on this slot's atom: -dE/dx = -2 g d
+        const double fr = -2.0 * (g + ge), frs = -2.0 * gs;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          f[c] += fr * d[c];
+          fs[c] += frs * d[c];
+        }
+        v[0] += 0.5 * E;
+        v[1] += 0.5 * Ee;
+        v[2] += 0.5 * Es;
+#pragma unroll
+        for (int pp = 0; pp < 3; ++pp)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            v[3 + 3 * pp + q] += 0.5 * fr * d[pp] * d[q];
+            v[12 + 3 * pp + q] += 0.5 * frs * d[pp] * d[q];
+          }
+      }
+      F4 o = a.force[k];
+      o.x += (float)f[0];
+      o.y += (float)f[1];
+      o.z += (float)f[2];
+      a.force[k] = o;
+      F4 os = a.forceSlow[k];
+      os.x += (float)fs[0];
+      os.y += (float)fs[1];
+      os.z += (float)fs[2];
+      a.forceSlow[k] = os;
+    }
+  }
+  if (doEnergyVirial) {
+    for (int u = 0; u < MOD_VALUES; ++u) {
+      const double t = blockSum256(v[u], sh);
+      if (threadIdx.x == 0) a.partials[(size_t)blockIdx.x * MOD_VALUES + u] = t;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(EXCL_THREADS)
+modifiedExclusionFinalKernel(const int* __restrict__ sysBlockStart, const double* __restrict__ partials,
+                             double* __restrict__ out) {
+  __shared__ double sh[EXCL_THREADS / 64];
+  const int sys = blockIdx.x;
+  const int b0 = sysBlockStart[sys], b1 = sysBlockStart[sys + 1];
+  for (int u = 0; u < MOD_VALUES; ++u) {
+    double v = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += EXCL_THREADS) v += partials[(size_t)b * MOD_VALUES + u];
+    const double t = blockSum256(v, sh);
+    if (threadIdx.x == 0) out[(size_t)sys * MOD_VALUES + u] = t;
+  }
+}
+
+hipError_t launchModifiedExclusionCorrection(int numBlocks, int numSystems, const ModExclArgs& a,
+                                             const int* sysBlockStart, double* out, hipStream_t s) {
+  if (numBlocks > 0) {
+    hipLaunchKernelGGL(modifiedExclusionKernel, dim3(numBlocks), dim3(EXCL_THREADS), 0, s, a, out ? 1 : 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (out) {
+    hipLaunchKernelGGL(modifiedExclusionFinalKernel, dim3(numSystems), dim3(EXCL_THREADS), 0, s, sysBlockStart,
+                       a.partials, out);
+    return hipGetLastError();
+  }
+  return hipSuccess;
+}
+
 void buildExclusionBlocks(const int* sysSlotStart, int numSystems, std::vector<int4>& blocks,
                           std::vector<int>& sysBlockStart) {
   blocks.clear();

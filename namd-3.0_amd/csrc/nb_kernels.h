@@ -88,6 +88,33 @@ hipError_t launchExclusionCorrection(int numBlocks, int numSystems, const int* e
                                      const int* slotPatch, const double* centre, const double* sysBox, double cutoff,
                                      double beta, F4* forceSlow, double* partials, double* out, hipStream_t s);
 
+// Modified (scaled 1-4) exclusion correction in FP64 (exclusion.hip): the
+// same slot CSR / block layout for the modified partners; fast and slow
+// forces added to force / forceSlow; out: 21 doubles per system {E_vdw,
+// E_elec, E_slow, virial[9], virialSlow[9]} (null: forces only).
+constexpr int MOD_EXCL_VALUES_PER_SYSTEM = 21;
+struct ModExclArgs {
+  const int* ptr;             // slot CSR of modified partners (both directions)
+  const int* slot;
+  const int4* blocks;         // {system, begin slot, end slot, -}
+  const F4* xyzq;
+  const int* slotPatch;
+  const double* centre;
+  const double* sysBox;
+  const int* vdwTypes;        // per slot
+  int ntypes;
+  const double* dAB;          // per ordered type pair: A14 - A, B14 - B
+  double rc2, rs2, c1, rcinv, rc3inv;
+  double oneMinusScale14;
+  int forceSwitch;            // 1: force-switched vdW (ka, kb, c6inv, c3inv, va, vb)
+  double ka, kb, c6inv, c3inv, va, vb;
+  F4* force;
+  F4* forceSlow;
+  double* partials;
+};
+hipError_t launchModifiedExclusionCorrection(int numBlocks, int numSystems, const ModExclArgs& a,
+                                             const int* sysBlockStart, double* out, hipStream_t s);
+
 // FP64 positions (3 per atom) -> patch-centred slot coordinates (integrate.hip)
 hipError_t launchPackSlots(int nslots, const int* atomIndex, const int* slotPatch, const double* centre,
                            const int* patchSystem, const double* sysBox, double quantum, const double* pos, F4* xyzq,

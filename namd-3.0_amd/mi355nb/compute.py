@@ -252,6 +252,36 @@ class ComputeNonbonded:
                 "exclusion_correction")
         return ev
 
+    def set_modified_exclusions(self, mod_ptr, mod_idx, scale14, A14, B14):
+        """Modified (scaled 1-4) exclusions: atom CSR (mod_ptr int64 natoms+1,
+        mod_idx int32), scale14 and the 1-4 LJ tables; mod_ptr None switches
+        them off."""
+        if mod_ptr is None:
+            L.check(L.lib().mi355nb_compute_set_modified_exclusions(self._h, None, None, 1.0, None, None),
+                    "set_modified_exclusions")
+            return
+        mp = np.ascontiguousarray(mod_ptr, np.int64)
+        mi = np.ascontiguousarray(mod_idx, np.int32)
+        a = np.ascontiguousarray(A14, np.float64).reshape(-1)
+        b = np.ascontiguousarray(B14, np.float64).reshape(-1)
+        L.check(L.lib().mi355nb_compute_set_modified_exclusions(self._h, mp.ctypes.data, mi.ctypes.data if mi.size
+                                                                else None, float(scale14), a.ctypes.data,
+                                                                b.ctypes.data), "set_modified_exclusions")
+
+    def modified_exclusion_correction(self, energy_virial=True, out=None):
+        """Modified-exclusion correction (FP64) for the last launch with slow
+        forces: adds its fast and slow forces to those finish() returns;
+        returns a cuda float64 (numSystems, 21) tensor {E_vdw, E_elec, E_slow,
+        virial[9], virialSlow[9]} (out, if given) or None."""
+        import torch
+        ev = None
+        if energy_virial:
+            ev = out if out is not None else torch.empty((self.info()["numSystems"], 21), dtype=torch.float64,
+                                                         device="cuda")
+        L.check(L.lib().mi355nb_compute_modified_exclusion_correction(
+            self._h, ev.data_ptr() if ev is not None else None), "modified_exclusion_correction")
+        return ev
+
     def set_force_table(self, enable=True):
         """NAMD's 1/r table mode (True) or direct math (False, the default)."""
         L.check(L.lib().mi355nb_compute_set_force_table(self._h, int(enable)), "set_force_table")
