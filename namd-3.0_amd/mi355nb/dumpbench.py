@@ -24,9 +24,10 @@ Sections, in order:
                   trans = Lattice::index(i, j, k) = 9(k+1) + 3(j+1) + (i+1)
 
 What this build reads: an orthorhombic cell, exclude <= 1-3 or scaled1-4
-with scale14 = 1 and 1-4 LJ parameters equal to the normal ones (modified
-pairs are then ordinary pairs, as the path computes them), no fixed atoms,
-no LES or pair-interaction modes.  Anything else raises ValueError.
+(the modified pairs, scale14 and the 1-4 LJ table come back in info as
+mod_ptr / mod_idx / scale14 / A14 / B14, the arguments of
+ComputeNonbonded.set_modified_exclusions), no fixed atoms, no LES or
+pair-interaction modes.  Anything else raises ValueError.
 """
 from __future__ import annotations
 
@@ -77,14 +78,20 @@ def _patch_grid(s: S.System):
     return dims, pid
 
 
-def _exclusion_rows(s: S.System):
-    """Per atom: (min, max, flags) as NAMD's ExclusionCheck holds them."""
+def _exclusion_rows(s: S.System, mod_ptr=None, mod_idx=None):
+    """Per atom: (min, max, flags) as NAMD's ExclusionCheck holds them;
+    modified partners (atom CSR mod_ptr / mod_idx) flagged EXCHCK_MOD."""
     n = s.natoms
-    partners = [set() for _ in range(n)]
+    partners = [dict() for _ in range(n)]
     for a in range(n):
         for b in s.excl_idx[s.excl_ptr[a]:s.excl_ptr[a + 1]]:
-            partners[a].add(int(b))
-            partners[int(b)].add(a)
+            partners[a][int(b)] = EXCHCK_FULL
+            partners[int(b)][a] = EXCHCK_FULL
+    if mod_ptr is not None:
+        for a in range(n):
+            for b in mod_idx[mod_ptr[a]:mod_ptr[a + 1]]:
+                partners[a][int(b)] = EXCHCK_MOD
+                partners[int(b)][a] = EXCHCK_MOD
     rows = []
     for a in range(n):
         if not partners[a]:
@@ -93,21 +100,24 @@ def _exclusion_rows(s: S.System):
         lo, hi = min(partners[a]), max(partners[a])
         size = hi - lo + 1
         in_middle = 1 if lo < a < hi else 0
-        if size == len(partners[a]) + in_middle:
+        all_full = all(v == EXCHCK_FULL for v in partners[a].values())
+        if all_full and size == len(partners[a]) + in_middle:
             flags = [EXCHCK_FULL] * size          # the shared all-full array
         else:
-            flags = [EXCHCK_FULL if (lo + k) in partners[a] else 0 for k in range(size)]
+            flags = [partners[a].get(lo + k, 0) for k in range(size)]
         rows.append((lo, hi, flags))
     return rows
 
 
-def write(f, s: S.System, A=None, B=None, alch=None):
+def write(f, s: S.System, A=None, B=None, alch=None, modified=None):
     """Write System s in dumpbench format to a path or text file object.
     A, B: ordered ntypes x ntypes LJ tables (default: the System's own);
-    alch: optional dict alchLambda, alchLambda2, alchVdwShiftCoeff."""
+    alch: optional dict alchLambda, alchLambda2, alchVdwShiftCoeff;
+    modified: optional dict mod_ptr, mod_idx, scale14, A14, B14 (scaled 1-4
+    exclusions; written as exclude scaled1-4)."""
     if isinstance(f, (str, bytes)) or hasattr(f, "__fspath__"):
         with open(f, "w") as fh:
-            return write(fh, s, A, B, alch)
+            return write(fh, s, A, B, alch, modified)
     p = s.params
     if A is None or B is None:
         A, B = S.lj_tables(s)
@@ -117,11 +127,17 @@ def write(f, s: S.System, A=None, B=None, alch=None):
     box = np.asarray(s.box, np.float64)
     beta = S.ewald_coefficient(p.cutoff, p.ewald_tol)
     alch = alch or {}
+    if modified is not None:
+        A14 = np.asarray(modified["A14"], np.float64)
+        B14 = np.asarray(modified["B14"], np.float64)
+    else:
+        A14, B14 = A, B
     vals = {
         "cellBasisVector1": (box[0], 0.0, 0.0), "cellBasisVector2": (0.0, box[1], 0.0),
         "cellBasisVector3": (0.0, 0.0, box[2]), "cellOrigin": tuple(0.5 * box),
         "nonbondedFrequency": p.nonbonded_frequency, "fullElectFrequency": p.full_elect_frequency,
-        "dielectric": 1.0, "exclude": EXCLUDE_ONETHREE, "scale14": 1.0, "nonbondedScaling": 1.0,
+        "dielectric": 1.0, "exclude": EXCLUDE_SCALED14 if modified is not None else EXCLUDE_ONETHREE,
+        "scale14": float(modified["scale14"]) if modified is not None else 1.0, "nonbondedScaling": 1.0,
         "cutoff": p.cutoff, "switchingActive": 1, "switchingDist": p.switchdist, "pairlistDist": p.pairlistdist,
         "alchOn": 1 if s.partition is not None else 0, "alchLambda": alch.get("alchLambda", 0.0),
         "alchLambda2": alch.get("alchLambda2", 0.0), "alchOutFreq": 0, "alchEquilSteps": 0,
@@ -145,10 +161,11 @@ def write(f, s: S.System, A=None, B=None, alch=None):
     out.append("%d" % nt)
     for i in range(nt):
         for j in range(i, nt):
-            out.append("%s %s %s %s" % (_fmt_g(A[i, j]), _fmt_g(B[i, j]), _fmt_g(A[i, j]), _fmt_g(B[i, j])))
+            out.append("%s %s %s %s" % (_fmt_g(A[i, j]), _fmt_g(B[i, j]), _fmt_g(A14[i, j]), _fmt_g(B14[i, j])))
     out.append("LJTABLE_END")
-    rows = _exclusion_rows(s)
-    npairs = int(sum(1 for a in range(s.natoms) for b in s.excl_idx[s.excl_ptr[a]:s.excl_ptr[a + 1]] if a < b))
+    rows = _exclusion_rows(s, *((modified["mod_ptr"], modified["mod_idx"]) if modified is not None else ()))
+    # numCalcExclusions: the unique excluded pairs, full and modified
+    npairs = sum(sum(1 for k, fl in enumerate(fr) if fl and lo + k > a) for a, (lo, hi, fr) in enumerate(rows))
     out.append("MOLECULE_BEGIN")
     out.append("%d %d" % (s.natoms, npairs))
     for a, (lo, hi, flags) in enumerate(rows):
@@ -274,6 +291,8 @@ def read(f, name="dumpbench", pme_grid=None, pme_order=4):
     ptr = np.zeros(natoms + 1, np.int64)
     idx = []
     modified = 0
+    mptr = np.zeros(natoms + 1, np.int64)
+    midx = []
     for a in range(natoms):
         parts = [int(v) for v in nxt().split()]
         vdw[a], lo, hi = parts[0], parts[1], parts[2]
@@ -289,15 +308,15 @@ def read(f, name="dumpbench", pme_grid=None, pme_order=4):
                 row.append(b)
             elif fl == EXCHCK_MOD:
                 modified += 1
+                midx.append(b)
             else:
                 raise ValueError(f"dumpbench: atom {a}: unknown exclusion flag {fl}")
         idx.extend(sorted(row))
         ptr[a + 1] = len(idx)
+        mptr[a + 1] = len(midx)
     expect("MOLECULE_END")
-    if modified and not (exclude == EXCLUDE_SCALED14 and scale14 == 1.0 and np.array_equal(A14, A)
-                         and np.array_equal(B14, B)):
-        raise ValueError("dumpbench: modified (1-4) exclusions need scale14 = 1 and 1-4 LJ parameters equal to "
-                         "the normal ones")
+    if modified and exclude != EXCLUDE_SCALED14:
+        raise ValueError("dumpbench: modified (1-4) exclusions in a file whose exclude is not scaled1-4")
     if int(vdw.max(initial=0)) >= nt or int(vdw.min(initial=0)) < 0:
         raise ValueError("dumpbench: vdw type outside the LJ table")
 
@@ -357,5 +376,6 @@ def read(f, name="dumpbench", pme_grid=None, pme_order=4):
                  partition=part if prm.get("alchOn", 0) else None, n_waters=n_waters, n_beads=n_beads,
                  lj_AB=(A, B))
     info = dict(params=prm, A=A, B=B, A14=A14, B14=B14, patches=patches, computes=computes,
-                modified_pairs=modified // 2, hydrogen_group_size=hgs)
+                modified_pairs=modified // 2, hydrogen_group_size=hgs, mod_ptr=mptr,
+                mod_idx=np.asarray(midx, np.int32), scale14=scale14)
     return s, info

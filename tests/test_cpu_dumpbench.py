@@ -47,12 +47,16 @@ def test_rejects_what_the_path_does_not_compute():
     text = GOLDEN.read_text()
     with pytest.raises(ValueError, match="orthorhombic"):
         D.read(text.replace("cellBasisVector2 0.000000 20.000000", "cellBasisVector2 1.000000 20.000000"))
-    # a 1-4 (modified) exclusion needs scaled1-4 with scale 1 and equal 1-4 parameters
+    # a 1-4 (modified) exclusion flag needs exclude scaled1-4
     modified = text.replace("0 1 6 1 1 0 0 0 1", "0 1 6 1 1 2 0 0 1").replace("1 4 5 1 1", "1 0 5 2 0 0 0 1 1")
     with pytest.raises(ValueError, match="modified"):
         D.read(modified)
     ok, info = D.read(modified.replace("exclude 2", "exclude 4"))
     assert info["modified_pairs"] == 1 and (0, 3) not in _pairs(ok)
+    # the modified pair comes back from both ends, for set_modified_exclusions
+    mp, mi = info["mod_ptr"], info["mod_idx"]
+    got = {(a, int(b)) for a in range(ok.natoms) for b in mi[mp[a]:mp[a + 1]]}
+    assert got == {(0, 3), (3, 0)}
     with pytest.raises(ValueError, match="missing"):
         D.read(text.replace("4.000000 9.000000 5.000000 1.000000 6 1 0 0 0 0\n", "").replace("PATCH_BEGIN\n4\n",
                                                                                                "PATCH_BEGIN\n3\n"))
@@ -104,3 +108,39 @@ def test_oracle_consumes_the_file():
     # positions rounded to 1e-6 A and LJ tables to 6 significant digits
     assert err < 1e-4, err
     assert abs(out[1]["e_vdw"] - out[0]["e_vdw"]) <= 1e-5 * abs(out[0]["e_vdw"])
+
+
+def test_modified_exclusions_round_trip():
+    """Scaled 1-4 exclusions written as EXCHCK_MOD flags with exclude
+    scaled1-4, scale14 and the 1-4 LJ columns, read back as the pair lists
+    and tables the modified-exclusion correction takes; the oracle's
+    correction on the read data matches the original within the file's
+    rounding (positions 1e-6 A, tables 6 significant digits)."""
+    s = systems.small_box(n_waters=100, n_chains=2, chain_len=20, box=18.0, seed=4)
+    mp, mi = systems.chain_pairs_14(s)
+    A14, B14 = systems.lj14_tables(s)
+    mod = dict(mod_ptr=mp, mod_idx=mi, scale14=0.8333, A14=A14, B14=B14)
+    buf = io.StringIO()
+    D.write(buf, s, modified=mod)
+    text = buf.getvalue()
+    r, info = D.read(text)
+    assert info["params"]["exclude"] == D.EXCLUDE_SCALED14 and info["scale14"] == 0.8333
+    assert info["modified_pairs"] == mi.size and _pairs(r) == _pairs(s)
+    want = {(a, int(b)) for a in range(s.natoms) for b in mi[mp[a]:mp[a + 1]]}
+    want |= {(b, a) for a, b in want}
+    rp, ri = info["mod_ptr"], info["mod_idx"]
+    assert {(a, int(b)) for a in range(r.natoms) for b in ri[rp[a]:rp[a + 1]]} == want
+    assert np.allclose(info["A14"], A14, rtol=5e-6) and np.allclose(info["B14"], B14, rtol=5e-6)
+    buf2 = io.StringIO()
+    D.write(buf2, r, modified=dict(mod_ptr=rp, mod_idx=ri, scale14=info["scale14"], A14=info["A14"],
+                                   B14=info["B14"]))
+    assert buf2.getvalue() == text
+    p = s.params
+    out = []
+    for x, (pp, ii, a14, b14) in ((s, (mp, mi, A14, B14)), (r, (rp, ri, info["A14"], info["B14"]))):
+        A, B = systems.lj_tables(x)
+        out.append(orc.mod_excl_fp64(x.pos, x.charge * math.sqrt(332.0636), x.vdw_type, A, B, a14, b14, pp, ii,
+                                     x.box, p.cutoff, p.switchdist, 0.8333))
+    assert out[0]["npairs"] == out[1]["npairs"] == mi.size
+    for k in ("e_vdw", "e_elec", "e_slow"):
+        assert abs(out[1][k] - out[0][k]) <= 1e-4 * abs(out[0][k]), k
