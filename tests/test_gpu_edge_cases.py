@@ -97,3 +97,19 @@ def test_no_exclusions_charge_only(torch_cuda):
     T._check_pairs(s, c, r)
     assert r["e_vdw"] == 0.0
     c.close()
+
+
+def test_switching_off(torch_cuda):
+    """switching off (switchdist = cutoff): truncated LJ, no switch region
+    (ComputeNonbondedUtil.C:504-515 leaves the switch inactive); forces,
+    energies and virial against the oracle, and the pair sets bit-exact
+    (the LJ energy jumps at the cutoff, so the inclusive cutoff rule shows)."""
+    import dataclasses
+
+    s = systems.small_box(n_waters=300, n_chains=3, chain_len=20, box=22.0, seed=7)
+    s.params = dataclasses.replace(s.params, switchdist=s.params.cutoff)
+    print(f"[switching off] {s.natoms} atoms", flush=True)
+    c, r = gpu_compute(s)
+    T._check_against_oracle(s, c, r)
+    T._check_pairs(s, c, r)
+    c.close()
