@@ -158,7 +158,7 @@ def test_autograd_function_and_descent(problem, torch_cuda):
     loss.backward()
     l2, ds, de, dq = fm.loss_and_grad(base.lj_sigma, base.lj_eps, base.charge)
     # separate GPU evaluations: float atomics make the last bits differ
-    assert abs(float(loss) - l2) <= 1e-6 * l2
+    assert abs(float(loss.detach()) - l2) <= 1e-6 * l2
     assert _rel(sig.grad.numpy(), ds) < 1e-5 and _rel(eps.grad.numpy(), de) < 1e-5
     # a few normalised gradient steps on sigma and eps lower the loss
     losses = [float(loss)]
