@@ -1249,11 +1249,12 @@ int oracle_namd_cpu(int n, const double* pos, const double* q, const int32_t* ty
  * pairs: atom CSR (mptr[n+1], midx), each pair listed once or from both ends
  * (deduplicated here).  forceSwitch selects the force-switched vdW form.
  * energies[3] = vdw, elec, slow; virial / virialSlow as oracle_nb_fp64. */
-int oracle_mod_excl_fp64(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+static int mod_excl_core(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
                          const double* A, const double* B, const double* A14, const double* B14,
                          const int64_t* mptr, const int32_t* midx, const double L[3], double cutoff,
-                         double switchdist, double scale14, int forceSwitch, double* f, double* fs,
-                         double* energies, double* virial, double* virialSlow, int64_t* npairs) {
+                         double switchdist, double scale14, int forceSwitch, const int8_t* part, const AlchF* al,
+                         double* f, double* fs, double* energies, int nenergies, double* virial,
+                         double* virialSlow, int64_t* npairs) {
   PairConst c;
   memset(&c, 0, sizeof(c));
   if (forceSwitch) set_force_switch(&c, cutoff, switchdist);
@@ -1265,7 +1266,7 @@ int oracle_mod_excl_fp64(int n, const double* pos, const double* q, const int32_
   c.rc3inv = c.rcinv * c.rcinv * c.rcinv;
   if (f) memset(f, 0, sizeof(double) * 3 * n);
   if (fs) memset(fs, 0, sizeof(double) * 3 * n);
-  double e[3] = {0, 0, 0}, vir[9] = {0}, virs[9] = {0};
+  double e[12] = {0}, vir[9] = {0}, virs[9] = {0};
   int64_t np = 0;
   for (int i = 0; i < n; ++i) {
     for (int64_t k = mptr[i]; k < mptr[i + 1]; ++k) {
@@ -1289,16 +1290,61 @@ int oracle_mod_excl_fp64(int n, const double* pos, const double* q, const int32_
       if (!(r2 < c.rc2)) continue;
       const int ab = type[i] * ntypes + type[j];
       const double qq = -(1.0 - scale14) * q[i] * q[j];
+      const double dA = A14[ab] - A[ab], dB = B14[ab] - B[ab];
+      /* pair class as the bonded kernel's pswitch (ComputeBondedCUDA.C:97-99,
+       * :3126-3129): 0 plain, 1 up, 2 down, 99 (cross) nothing */
+      int ps = 0;
+      if (part) {
+        const int pi = part[i], pj = part[j];
+        const int ref = pi == 0 && pj == 0, cross = pi != 0 && pj != 0 && pi != pj;
+        const int dec = !ref && !cross && pi == pj && al->decouple;
+        ps = (ref || dec) ? 0 : cross ? 99 : (pi == 1 || pj == 1) ? 1 : 2;
+      }
+      if (ps == 99) continue;
       double ff, ffs, ev = 0.0, ee = 0.0, es = 0.0;
-      /* vdW with the difference coefficients; pair_fp64 with qq = 0 and beta = 0
-       * gives only the switched LJ (its slow term vanishes with qq) */
-      pair_fp64(&c, r2, 0.0, A14[ab] - A[ab], B14[ab] - B[ab], &ff, &ffs, &ev, &ee, &es);
       const double r = sqrt(r2), rinv = 1.0 / r;
-      /* (1/r) dE/dr of the electrostatic parts */
-      ff += qq * (-rinv * rinv * rinv + c.rc3inv);
-      ee = qq * (rinv - 1.5 * c.rcinv + 0.5 * r2 * c.rc3inv);
-      ffs = -qq * c.rc3inv;
-      es = qq * (1.5 * c.rcinv - 0.5 * r2 * c.rc3inv);
+      /* electrostatic parts: energies and (1/r) dE/dr */
+      const double eeP = qq * (rinv - 1.5 * c.rcinv + 0.5 * r2 * c.rc3inv);
+      const double feP = qq * (-rinv * rinv * rinv + c.rc3inv);
+      const double esP = qq * (1.5 * c.rcinv - 0.5 * r2 * c.rc3inv);
+      const double fsP = -qq * c.rc3inv;
+      if (ps == 0) {
+        /* vdW with the difference coefficients; pair_fp64 with qq = 0 and beta = 0
+         * gives only the switched LJ (its slow term vanishes with qq) */
+        pair_fp64(&c, r2, 0.0, dA, dB, &ff, &ffs, &ev, &ee, &es);
+        ff += feP;
+        ffs = fsP;
+        ee = eeP;
+        es = esP;
+        e[3] += ev;
+        e[4] += ee;
+        e[5] += es;
+      } else {
+        /* up / down (modifiedExclusionForce's alchemical route,
+         * ComputeBondedCUDAKernel.cu:379-475, :575-586): soft-core vdW with
+         * potential switching and the difference coefficients, electrostatic
+         * correction scaled by the pair's elec lambda */
+        const int up = ps == 1;
+        const double lv = up ? al->vdwUp : al->vdwDown, le = up ? al->elecUp : al->elecDown;
+        const double lv2 = up ? al->vdw2Up : al->vdw2Down, le2 = up ? al->elec2Up : al->elec2Down;
+        const double sh = up ? al->shUp : al->shDown, sh2 = up ? al->sh2Up : al->sh2Down;
+        double S, S2, U, F, U2, F2;
+        vdw_switch64(&c, r2, &S, &S2);
+        vdw_fe64(dA, dB, r2 + sh, &U, &F);
+        vdw_fe64(dA, dB, r2 + sh2, &U2, &F2);
+        ff = -lv * (S * F + S2 * U) + le * feP;
+        ffs = le * fsP;
+        ev = lv * S * U;
+        ee = le * eeP;
+        es = le * esP;
+        e[3] += lv2 * S * U2;
+        e[4] += le2 * eeP;
+        e[5] += le2 * esP;
+        const int o = up ? 0 : 1;
+        e[6 + o] += S * (U + lv * al->coeff * 0.5 * F);
+        e[8 + o] += eeP;
+        e[10 + o] += esP;
+      }
       e[0] += ev;
       e[1] += ee;
       e[2] += es;
@@ -1314,9 +1360,36 @@ int oracle_mod_excl_fp64(int n, const double* pos, const double* q, const int32_
       ++np;
     }
   }
-  if (energies) memcpy(energies, e, sizeof(e));
+  if (energies) memcpy(energies, e, nenergies * sizeof(double));
   if (virial) memcpy(virial, vir, sizeof(vir));
   if (virialSlow) memcpy(virialSlow, virs, sizeof(virs));
   if (npairs) *npairs = np;
   return 0;
+}
+
+int oracle_mod_excl_fp64(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+                         const double* A, const double* B, const double* A14, const double* B14,
+                         const int64_t* mptr, const int32_t* midx, const double L[3], double cutoff,
+                         double switchdist, double scale14, int forceSwitch, double* f, double* fs,
+                         double* energies, double* virial, double* virialSlow, int64_t* npairs) {
+  return mod_excl_core(n, pos, q, type, ntypes, A, B, A14, B14, mptr, midx, L, cutoff, switchdist, scale14,
+                       forceSwitch, NULL, NULL, f, fs, energies, 3, virial, virialSlow, npairs);
+}
+
+/* With alchemical partitions (lam[13] and energies[12] as oracle_nb_fp64_alch;
+ * cross-partition pairs are skipped and not counted). */
+int oracle_mod_excl_alch_fp64(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+                              const double* A, const double* B, const double* A14, const double* B14,
+                              const int64_t* mptr, const int32_t* midx, const double L[3], double cutoff,
+                              double switchdist, double scale14, const int8_t* part, const double* lam,
+                              int decouple, double* f, double* fs, double* energies, double* virial,
+                              double* virialSlow, int64_t* npairs) {
+  AlchF al;
+  al.elecUp = lam[0]; al.elecDown = lam[1]; al.vdwUp = lam[2]; al.vdwDown = lam[3];
+  al.elec2Up = lam[4]; al.elec2Down = lam[5]; al.vdw2Up = lam[6]; al.vdw2Down = lam[7];
+  al.shUp = lam[8]; al.shDown = lam[9]; al.sh2Up = lam[10]; al.sh2Down = lam[11];
+  al.decouple = decouple;
+  al.coeff = lam[12];
+  return mod_excl_core(n, pos, q, type, ntypes, A, B, A14, B14, mptr, midx, L, cutoff, switchdist, scale14, 0,
+                       part, &al, f, fs, energies, 12, virial, virialSlow, npairs);
 }

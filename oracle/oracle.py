@@ -59,6 +59,9 @@ def lib():
         L.oracle_nb_fp64_alch_vdw_force_switch.argtypes = L.oracle_nb_fp64_alch.argtypes
         L.oracle_mod_excl_fp64.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, dp, dp, i64p, i32p, dp,
                                            C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp, dp, i64p]
+        L.oracle_mod_excl_alch_fp64.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, dp, dp, i64p, i32p, dp,
+                                                C.c_double, C.c_double, C.c_double, i8p, dp, C.c_int, dp, dp, dp,
+                                                dp, dp, i64p]
     return _LIB
 
 
@@ -126,26 +129,40 @@ def nb_fp64_alch(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff,
 
 
 def mod_excl_fp64(pos, q_scaled, vdw_type, A, B, A14, B14, mod_ptr, mod_idx, box, cutoff, switchdist, scale14,
-                  vdw_force_switching=False):
+                  vdw_force_switching=False, partition=None, alch=None):
     """FP64 modified (scaled 1-4) exclusion correction over the listed pairs
     (atom CSR mod_ptr / mod_idx): returns f, fs (n x 3), e_vdw, e_elec, e_slow,
-    virial, virial_slow, npairs (pairs within the cutoff)."""
+    virial, virial_slow, npairs (pairs within the cutoff).  With partition and
+    alch (as nb_fp64_alch takes them) the alchemical correction, and also the
+    lambda2 sums e_*_s and the TI sums e_*_ti_1/2."""
     n = len(q_scaled)
     ntypes = int(round(np.sqrt(np.asarray(A).size)))
     f = np.zeros(3 * n)
     fs = np.zeros(3 * n)
-    en = np.zeros(3)
+    en = np.zeros(12)
     vir = np.zeros(9)
     virs = np.zeros(9)
     npairs = np.zeros(1, np.int64)
-    lib().oracle_mod_excl_fp64(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64),
-                               _c(vdw_type, np.int32), ntypes, _c(A, np.float64).reshape(-1),
-                               _c(B, np.float64).reshape(-1), _c(A14, np.float64).reshape(-1),
-                               _c(B14, np.float64).reshape(-1), _c(mod_ptr, np.int64), _c(mod_idx, np.int32),
-                               _c(box, np.float64), cutoff, switchdist, scale14, int(bool(vdw_force_switching)), f,
-                               fs, en, vir, virs, npairs)
+    args = (n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32), ntypes,
+            _c(A, np.float64).reshape(-1), _c(B, np.float64).reshape(-1), _c(A14, np.float64).reshape(-1),
+            _c(B14, np.float64).reshape(-1), _c(mod_ptr, np.int64), _c(mod_idx, np.int32), _c(box, np.float64),
+            cutoff, switchdist, scale14)
+    out = dict()
+    if partition is None:
+        lib().oracle_mod_excl_fp64(*args, int(bool(vdw_force_switching)), f, fs, en, vir, virs, npairs)
+    else:
+        if vdw_force_switching:
+            raise ValueError("the alchemical modified-exclusion correction uses potential switching only")
+        get = (lambda k: alch[k]) if isinstance(alch, dict) else (lambda k: getattr(alch, k))
+        has = (lambda k: k in alch) if isinstance(alch, dict) else (lambda k: hasattr(alch, k))
+        lam = np.array([float(get(k)) for k in ALCH_KEYS] +
+                       [float(get("alchVdwShiftCoeff")) if has("alchVdwShiftCoeff") else 0.0], np.float64)
+        lib().oracle_mod_excl_alch_fp64(*args, _c(partition, np.int8), lam, int(bool(get("alchDecouple"))), f, fs,
+                                        en, vir, virs, npairs)
+        out = dict(e_vdw_s=en[3], e_elec_s=en[4], e_slow_s=en[5], e_vdw_ti_1=en[6], e_vdw_ti_2=en[7],
+                   e_elec_ti_1=en[8], e_elec_ti_2=en[9], e_slow_ti_1=en[10], e_slow_ti_2=en[11])
     return dict(f=f.reshape(n, 3), fs=fs.reshape(n, 3), e_vdw=en[0], e_elec=en[1], e_slow=en[2],
-                virial=vir.reshape(3, 3), virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]))
+                virial=vir.reshape(3, 3), virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]), **out)
 
 
 def nb_adjoint_fp64(pos, q_scaled, vdw_type, ntypes, excl_ptr, excl_idx, box, cutoff, switchdist, beta, g,
