@@ -330,9 +330,17 @@ int mi355nb_compute_exclusion_correction(mi355nb_compute* c, double* energyViria
  *   fast  qq' (1/r - 3/(2 rc) + r^2/(2 rc^3)), qq' = -(1 - scale14) qq;
  *   slow  qq' (3/(2 rc) - r^2/(2 rc^3)), to the slow forces,
  * to the forces of the last launch (which needs doSlow), enqueued on the
- * compute's stream.  energyVirial: device doubles, 21 per system {E_vdw,
- * E_elec, E_slow, virial[9], virialSlow[9]}, overwritten; NULL: forces only.
- * Not with alchemical partitions (MI355NB_ERR_STATE). */
+ * compute's stream.  With alchemical partitions the pair classes are the
+ * bonded kernel's pswitch (ComputeBondedCUDA.C:97-99, :3126-3129): plain for
+ * environment and decoupled pairs, nothing for cross-partition pairs, and for
+ * up / down pairs the soft-core vdW lambda_vdw S U(r^2 + shift) with the
+ * difference coefficients (potential switching) and the electrostatic terms
+ * times lambda_elec (ComputeBondedCUDAKernel.cu:379-475, :575-586); not
+ * combinable with vdW force switching (MI355NB_ERR_STATE).  energyVirial:
+ * device doubles, 30 per system {E_vdw, E_elec, E_slow, virial[9],
+ * virialSlow[9], E_vdw_s, E_elec_s, E_slow_s (FEP, at lambda2),
+ * vdw_ti_1, vdw_ti_2, elec_ti_1, elec_ti_2, slow_ti_1, slow_ti_2 (TI)},
+ * overwritten; NULL: forces only. */
 int mi355nb_compute_set_modified_exclusions(mi355nb_compute* c, const int64_t* modPtr, const int32_t* modIdx,
                                             double scale14, const double* A14, const double* B14);
 int mi355nb_compute_modified_exclusion_correction(mi355nb_compute* c, double* energyVirial);

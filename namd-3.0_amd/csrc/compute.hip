@@ -1080,7 +1080,8 @@ int mi355nb_compute_modified_exclusion_correction(mi355nb_compute* c, double* en
   CHECK_ARG(c, "null compute");
   CHECK_STATE(c->haveMod, "set_modified_exclusions must precede the modified exclusion correction");
   CHECK_STATE(c->haveLayout && c->lastSlow, "a launch with doSlow must precede the modified exclusion correction");
-  CHECK_STATE(!c->alchOn, "the modified exclusion correction does not support alchemical partitions");
+  CHECK_STATE(!(c->alchOn && c->vdwForceSwitch),
+              "the alchemical modified exclusion correction uses potential switching (no vdW force switching)");
   MI355NB_HIP(hipSetDevice(c->device));
   ModExclArgs a;
   a.ptr = c->modSlotPtr.ptr;
@@ -1112,6 +1113,9 @@ int mi355nb_compute_modified_exclusion_correction(mi355nb_compute* c, double* en
   a.force = c->force.ptr;
   a.forceSlow = c->forceSlow.ptr;
   a.partials = c->modPartials.ptr;
+  a.part = c->alchOn ? c->part.ptr : nullptr;
+  a.alch = c->alchOn ? c->alch.ptr : nullptr;
+  a.alchTi = (c->alchOn && c->alchTi) ? 1 : 0;
   MI355NB_HIP(launchModifiedExclusionCorrection((int)c->exclBlocksHost.size(), c->L.numSystems, a,
                                                 c->exclSysStart.ptr, energyVirial, c->stream));
   return MI355NB_OK;

@@ -130,6 +130,15 @@ exclusionFinalKernel(const int* __restrict__ sysBlockStart, const double* __rest
 // so that the pair ends with 1-4 LJ and scale14 qq/r - qq erf(beta r)/r.  FP64,
 // the work layout of exclusionCorrectionKernel: one thread per slot over its
 // partners, both ends evaluated, its own slot's force written without atomics.
+//
+// With alchemical partitions the pair classes are the bonded kernel's pswitch
+// (src/ComputeBondedCUDA.C:97-99, :3126-3129): environment and decoupled
+// pairs take the plain correction, cross-partition pairs none, and up / down
+// pairs (src/ComputeBondedCUDAKernel.cu:379-475, :575-586) the soft-core vdW
+// lv S U(r^2 + sh) with the difference coefficients (potential switching)
+// and the electrostatic correction times the elec lambda; the FEP sums at
+// lambda2 or the TI sums (vdW S (U + lv shiftCoeff F/2), F = -2 dU/dx; elec
+// and slow unscaled) per partition, as the pair kernel's alchemical modes.
 constexpr int MOD_VALUES = MOD_EXCL_VALUES_PER_SYSTEM;
 
 __global__ void __launch_bounds__(EXCL_THREADS) modifiedExclusionKernel(const ModExclArgs a, int doEnergyVirial) {
@@ -186,10 +195,58 @@ __global__ void __launch_bounds__(EXCL_THREADS) modifiedExclusionKernel(const Mo
           g = dU;
         }
         const double qq = -a.oneMinusScale14 * (double)p.w * (double)b.w;
-        const double Ee = qq * (rinv - 1.5 * a.rcinv + 0.5 * r2 * a.rc3inv);
-        const double ge = qq * 0.5 * (a.rc3inv - rinv3);
-        const double Es = qq * (1.5 * a.rcinv - 0.5 * r2 * a.rc3inv);
-        const double gs = -0.5 * qq * a.rc3inv;
+        double Ee = qq * (rinv - 1.5 * a.rcinv + 0.5 * r2 * a.rc3inv);
+        double ge = qq * 0.5 * (a.rc3inv - rinv3);
+        double Es = qq * (1.5 * a.rcinv - 0.5 * r2 * a.rc3inv);
+        double gs = -0.5 * qq * a.rc3inv;
+        if (a.part) {
+          const int pi = a.part[k], pj = a.part[j];
+          const AlchParams& ap = a.alch[bk.x];
+          const bool ref = pi == 0 && pj == 0, cross = pi != 0 && pj != 0 && pi != pj;
+          const bool dec = !ref && !cross && pi == pj && ap.decouple != 0;
+          if (cross) continue;
+          if (ref || dec) {
+            if (!a.alchTi) {
+              v[21] += 0.5 * E;
+              v[22] += 0.5 * Ee;
+              v[23] += 0.5 * Es;
+            }
+          } else {
+            const bool up = pi == 1 || pj == 1;
+            const double lv = up ? ap.vdwLambdaUp : ap.vdwLambdaDown;
+            const double le = up ? ap.elecLambdaUp : ap.elecLambdaDown;
+            const double sh = up ? ap.vdwShiftUp : ap.vdwShiftDown;
+            double S = 1.0, S2 = 0.0;
+            if (r2 > a.rs2) {
+              const double cd = a.rc2 - r2, sd = r2 - a.rs2;
+              S = a.c1 * cd * cd * (cd + 3.0 * sd);
+              S2 = 12.0 * a.c1 * cd * sd;   // -2 dS/d(r^2)
+            }
+            const double i2 = 1.0 / (r2 + sh), i6 = i2 * i2 * i2;
+            const double Us = i6 * (dA * i6 - dB);
+            const double Fs = 6.0 * i2 * (2.0 * Us + dB * i6);   // -2 dU/dx
+            E = lv * S * Us;
+            g = -0.5 * lv * (S * Fs + S2 * Us);
+            if (a.alchTi) {
+              const int o = up ? 0 : 1;
+              v[24 + o] += 0.5 * S * (Us + 0.5 * lv * ap.shiftCoeff * Fs);
+              v[26 + o] += 0.5 * Ee;
+              v[28 + o] += 0.5 * Es;
+            } else {
+              const double lv2 = up ? ap.vdwLambda2Up : ap.vdwLambda2Down;
+              const double le2 = up ? ap.elecLambda2Up : ap.elecLambda2Down;
+              const double sh2 = up ? ap.vdwShift2Up : ap.vdwShift2Down;
+              const double j2 = 1.0 / (r2 + sh2), j6 = j2 * j2 * j2;
+              v[21] += 0.5 * lv2 * S * j6 * (dA * j6 - dB);
+              v[22] += 0.5 * le2 * Ee;
+              v[23] += 0.5 * le2 * Es;
+            }
+            Ee *= le;
+            ge *= le;
+            Es *= le;
+            gs *= le;
+          }
+        }
         // force on this slot's atom: -dE/dx = -2 g d
         const double fr = -2.0 * (g + ge), frs = -2.0 * gs;
 #pragma unroll

@@ -90,9 +90,12 @@ hipError_t launchExclusionCorrection(int numBlocks, int numSystems, const int* e
 
 // Modified (scaled 1-4) exclusion correction in FP64 (exclusion.hip): the
 // same slot CSR / block layout for the modified partners; fast and slow
-// forces added to force / forceSlow; out: 21 doubles per system {E_vdw,
-// E_elec, E_slow, virial[9], virialSlow[9]} (null: forces only).
-constexpr int MOD_EXCL_VALUES_PER_SYSTEM = 21;
+// forces added to force / forceSlow; out: 30 doubles per system {E_vdw,
+// E_elec, E_slow, virial[9], virialSlow[9], E_vdw_s, E_elec_s, E_slow_s (FEP,
+// at lambda2), vdW / elec / slow dU/dlambda of partitions 1 and 2 (TI):
+// vdw_ti_1, vdw_ti_2, elec_ti_1, elec_ti_2, slow_ti_1, slow_ti_2} (null:
+// forces only).
+constexpr int MOD_EXCL_VALUES_PER_SYSTEM = 30;
 struct ModExclArgs {
   const int* ptr;             // slot CSR of modified partners (both directions)
   const int* slot;
@@ -111,6 +114,10 @@ struct ModExclArgs {
   F4* force;
   F4* forceSlow;
   double* partials;
+  // alchemical partitions (null: none)
+  const signed char* part;    // per slot
+  const AlchParams* alch;     // per system
+  int alchTi;                 // 1: TI sums, 0: FEP sums at lambda2
 };
 hipError_t launchModifiedExclusionCorrection(int numBlocks, int numSystems, const ModExclArgs& a,
                                              const int* sysBlockStart, double* out, hipStream_t s);

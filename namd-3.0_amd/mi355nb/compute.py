@@ -271,12 +271,14 @@ class ComputeNonbonded:
     def modified_exclusion_correction(self, energy_virial=True, out=None):
         """Modified-exclusion correction (FP64) for the last launch with slow
         forces: adds its fast and slow forces to those finish() returns;
-        returns a cuda float64 (numSystems, 21) tensor {E_vdw, E_elec, E_slow,
-        virial[9], virialSlow[9]} (out, if given) or None."""
+        returns a cuda float64 (numSystems, 30) tensor {E_vdw, E_elec, E_slow,
+        virial[9], virialSlow[9], E_vdw_s, E_elec_s, E_slow_s, vdw_ti_1,
+        vdw_ti_2, elec_ti_1, elec_ti_2, slow_ti_1, slow_ti_2} (out, if given)
+        or None."""
         import torch
         ev = None
         if energy_virial:
-            ev = out if out is not None else torch.empty((self.info()["numSystems"], 21), dtype=torch.float64,
+            ev = out if out is not None else torch.empty((self.info()["numSystems"], 30), dtype=torch.float64,
                                                          device="cuda")
         L.check(L.lib().mi355nb_compute_modified_exclusion_correction(
             self._h, ev.data_ptr() if ev is not None else None), "modified_exclusion_correction")

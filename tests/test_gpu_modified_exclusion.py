@@ -132,3 +132,78 @@ def test_modified_exclusions_argument_and_state_errors(torch_cuda):
     with pytest.raises(L.Mi355nbError):
         c.modified_exclusion_correction()
     c.close()
+
+
+@pytest.mark.parametrize("method,decouple", [("fep", False), ("ti", False), ("fep", True)])
+def test_modified_exclusions_alchemical_batch(torch_cuda, method, decouple):
+    """Alchemical route (pswitch): a four-system batch whose first chain is
+    partition 1 and second chain partition 2, the chains' 1-4 pairs plus one
+    cross-partition pair per system listed as modified; the correction's
+    energies at lambda, the FEP sums at lambda2 or the TI sums, and virials
+    within 1e-9 of the alchemical oracle system by system (FP64 from the same
+    float32 inputs), and pair kernel + correction forces within 1e-5 of the
+    oracles' sum."""
+    torch = torch_cuda
+    members = systems.batch_systems(n_systems=4, seed=9, n_min=3000, n_max=8000)
+    c = ComputeNonbonded.from_systems([m.system for m in members], stream=torch.cuda.current_stream().cuda_stream)
+    ptrs, idxs, parts, per_sys, off = [np.zeros(1, np.int64)], [], [], [], 0
+    for m in members:
+        s = m.system
+        L_ = s.chain_len
+        pk = m.partition.copy()
+        pk[:L_] = 1
+        pk[L_:2 * L_] = 2
+        mp, mi = systems.chain_pairs_14(s)
+        # one cross-partition pair (first bead of chain 1 with first bead of chain 2)
+        lists = [list(mi[mp[a]:mp[a + 1]]) for a in range(s.natoms)]
+        lists[0].append(L_)
+        mp = np.concatenate([[0], np.cumsum([len(x) for x in lists])]).astype(np.int64)
+        mi = np.array([b for x in lists for b in x], np.int32)
+        per_sys.append((mp, mi, pk))
+        ptrs.append(mp[1:] + ptrs[-1][-1])
+        idxs.append(mi + off)
+        parts.append(pk)
+        off += s.natoms
+    A14, B14 = systems.lj14_tables(members[0].system)
+    alch = [L.alch_data(c.params, m.lam, min(1.0, m.lam + 0.05), elec_lambda_start=0.5, vdw_lambda_end=1.0,
+                        vdw_shift_coeff=5.0, decouple=decouple) for m in members]
+    c.set_alch(np.concatenate(parts), alch, method=method)
+    c.set_modified_exclusions(np.concatenate(ptrs), np.concatenate(idxs), S14, A14, B14)
+    print(f"[modified exclusions alch {method} decouple={decouple}] {c.natoms} atoms", flush=True)
+    c.launch_work(do_energy=False, do_virial=False, do_slow=True, do_pairlist=True)
+    r0 = c.finish_batch()
+    ev = c.modified_exclusion_correction()
+    r1 = c.finish_batch()
+    torch.cuda.synchronize()
+    ev = ev.cpu().numpy()
+    pos, boxes, st = c.effective_positions(), c.system_boxes(), c.sys_atom_start
+    keys = ("e_vdw", "e_elec", "e_slow") + (("e_vdw_s", "e_elec_s", "e_slow_s") if method == "fep" else ())
+    ti_keys = ("e_vdw_ti_1", "e_vdw_ti_2", "e_elec_ti_1", "e_elec_ti_2", "e_slow_ti_1", "e_slow_ti_2")
+    col = dict(e_vdw=0, e_elec=1, e_slow=2, e_vdw_s=21, e_elec_s=22, e_slow_s=23,
+               **{k: 24 + i for i, k in enumerate(ti_keys)})
+    for k, m in enumerate(members):
+        s = m.system
+        a, b = st[k], st[k + 1]
+        mp, mi, pk = per_sys[k]
+        p = s.params
+        A, B = systems.lj_tables(s)
+        o = orc.mod_excl_fp64(pos[a:b], _q32(s), s.vdw_type, A, B, A14, B14, mp, mi, boxes[k], p.cutoff,
+                              p.switchdist, S14, partition=pk, alch=alch[k])
+        scale = max(abs(o["e_vdw"]), abs(o["e_elec"]), abs(o["e_slow"]))
+        for key in keys + (ti_keys if method == "ti" else ()):
+            err = abs(ev[k, col[key]] - o[key])
+            print(f"    [sys {k}] {key} gpu {ev[k, col[key]]:.12e} oracle {o[key]:.12e}", flush=True)
+            assert err <= 1e-9 * max(abs(o[key]), 1e-3 * scale), key
+        if method == "ti":
+            assert not ev[k, 21:24].any()
+        else:
+            assert not ev[k, 24:30].any()
+        for off_, key in ((3, "virial"), (12, "virial_slow")):
+            assert np.abs(ev[k, off_:off_ + 9] - o[key].reshape(-1)).max() <= 1e-9 * np.abs(o[key]).max(), key
+        beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+        nb = orc.nb_fp64_alch(pos[a:b], s.charge * math.sqrt(COULOMB), s.vdw_type, A, B, s.excl_ptr, s.excl_idx,
+                              boxes[k], p.cutoff, p.switchdist, beta, pk, alch[k])
+        ef, es = rel_rms(r1[2][a:b], nb["f"] + o["f"]), rel_rms(r1[3][a:b], nb["fs"] + o["fs"])
+        print(f"    [sys {k}] totals fast {ef:.2e} slow {es:.2e}", flush=True)
+        assert ef < 1e-5 and es < 1e-5
+    c.close()
