@@ -187,6 +187,24 @@ int mi355nb_compute_set_molecule(mi355nb_compute* c, int natoms, const int32_t* 
  * computes, ORB-sorted padded storage, and uploads.  pos: natoms*3 doubles. */
 int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], const double* pos);
 
+/* The same for one system in a general periodic cell (the lata / latb / latc
+ * of CudaComputeNonbondedKernel::nonbondedForce and CudaTileListKernel::
+ * buildTileLists, src/CudaComputeNonbondedKernel.cu:2076-2080,
+ * src/CudaTileListKernel.cu:1022-1027; NAMD's Lattice).  lattice: rows a, b,
+ * c (9 doubles, right-handed), each rounded to a multiple of 2^-17 A.  The
+ * patch grid divides each lattice direction by floor(perpendicular width /
+ * (pairlistdist + margin)); atoms are wrapped and assigned in fractional
+ * coordinates; every image shift is an integer combination of the rounded
+ * lattice rows, so pair displacements stay exact and the cutoff rule
+ * unchanged.  Every perpendicular width must be at least 2 pairlistdist.
+ * Device position packing (update_positions_device) serves orthorhombic
+ * boxes only; update_positions serves both. */
+int mi355nb_compute_assign_patches_lattice(mi355nb_compute* c, const double lattice[9], const double* pos);
+
+/* Lattice rows a, b, c the kernels use, 9 doubles per system (orthorhombic
+ * systems: the diagonal of their box). */
+int mi355nb_compute_get_system_cells(const mi355nb_compute* c, double* cells);
+
 /* Batched systems (no counterpart in the reference, which runs one system
  * per process): numSystems independent periodic boxes whose atoms are
  * concatenated in the molecule given to set_molecule; system s owns atoms

@@ -120,7 +120,7 @@ struct mi355nb_compute {
   std::vector<int> exclSysStartHost;
   DevBuf<int4> exclBlocks;
   DevBuf<int> exclSlotPtr, exclSlotIdx, exclSysStart;
-  DevBuf<double> exclPartials, sysBoxDev;
+  DevBuf<double> exclPartials, sysBoxDev, sysCellDev;
   DevBuf<int> modSlotPtr, modSlotIdx;
   DevBuf<double> modPartials, modDAB;
   DevBuf<int> patchSystemDev;
@@ -164,7 +164,7 @@ struct mi355nb_compute {
     alch.release(); part.release(); redBlocks.release(); redSysStart.release();
     forceTable.release(); energyTable.release(); slotPatch.release(); patchCentre.release();
     exclSlotPtr.release(); exclSlotIdx.release(); exclBlocks.release(); exclSysStart.release();
-    exclPartials.release(); sysBoxDev.release(); patchSystemDev.release();
+    exclPartials.release(); sysBoxDev.release(); sysCellDev.release(); patchSystemDev.release();
     modSlotPtr.release(); modSlotIdx.release(); modPartials.release(); modDAB.release();
     fmTarget.release(); fmLossPartials.release(); fmLoss.release(); fmGq.release(); fmBinPartials.release();
     fmBins.release(); fmG.release();
@@ -452,14 +452,32 @@ static int uploadModifiedSlots(mi355nb_compute* c) {
 }
 
 static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomStart, const double* boxes,
-                             const double* pos) {
-  for (int s = 0; s < nsys; ++s)
-    for (int d = 0; d < 3; ++d)
-      CHECK_ARG(boxes[3 * s + d] >= 2.0 * c->p.pairlistdist, "box edge must be at least twice pairlistdist");
+                             const double* pos, const double* cell = nullptr) {
+  if (cell) {
+    // every perpendicular width at least twice pairlistdist (one image per pair)
+    const double* a = cell;
+    const double* b = cell + 3;
+    const double* cc = cell + 6;
+    const double cr[3][3] = {{b[1] * cc[2] - b[2] * cc[1], b[2] * cc[0] - b[0] * cc[2], b[0] * cc[1] - b[1] * cc[0]},
+                             {cc[1] * a[2] - cc[2] * a[1], cc[2] * a[0] - cc[0] * a[2], cc[0] * a[1] - cc[1] * a[0]},
+                             {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]}};
+    const double V = a[0] * cr[0][0] + a[1] * cr[0][1] + a[2] * cr[0][2];
+    CHECK_ARG(V > 0, "lattice vectors a, b, c must be right-handed with positive volume");
+    for (int j = 0; j < 3; ++j)
+      CHECK_ARG(V / std::sqrt(cr[j][0] * cr[j][0] + cr[j][1] * cr[j][1] + cr[j][2] * cr[j][2]) >=
+                    2.0 * c->p.pairlistdist,
+                "every perpendicular width of the cell must be at least twice pairlistdist");
+  } else {
+    for (int s = 0; s < nsys; ++s)
+      for (int d = 0; d < 3; ++d)
+        CHECK_ARG(boxes[3 * s + d] >= 2.0 * c->p.pairlistdist, "box edge must be at least twice pairlistdist");
+  }
   for (int i = 0; i < 3 * c->natoms; ++i) CHECK_ARG(std::isfinite(pos[i]), "non-finite position");
   std::string err;
   std::vector<int> start(atomStart, atomStart + nsys + 1);
-  if (!c->L.buildBatch(nsys, start.data(), pos, boxes, c->p.pairlistdist + c->p.margin, err)) {
+  const bool ok = cell ? c->L.buildCell(c->natoms, pos, cell, c->p.pairlistdist + c->p.margin, err)
+                       : c->L.buildBatch(nsys, start.data(), pos, boxes, c->p.pairlistdist + c->p.margin, err);
+  if (!ok) {
     setLastError(err);
     return MI355NB_ERR_ARG;
   }
@@ -560,6 +578,9 @@ static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomSt
                           hipMemcpyHostToDevice));
     MI355NB_HIP(hipMemcpy(c->sysBoxDev.ptr, c->L.sysBox.data(), 3 * (size_t)nsys * sizeof(double),
                           hipMemcpyHostToDevice));
+    MI355NB_HIP(c->sysCellDev.reserve(18 * (size_t)nsys, 1.0f));
+    MI355NB_HIP(hipMemcpy(c->sysCellDev.ptr, c->L.sysCell.data(), 18 * (size_t)nsys * sizeof(double),
+                          hipMemcpyHostToDevice));
     MI355NB_HIP(c->patchSystemDev.reserve(std::max<size_t>(1, c->L.patchSystem.size()), 1.0f));
     MI355NB_HIP(hipMemcpy(c->patchSystemDev.ptr, c->L.patchSystem.data(), c->L.patchSystem.size() * sizeof(int),
                           hipMemcpyHostToDevice));
@@ -589,6 +610,21 @@ int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], cons
   CHECK_STATE(c->haveMolecule, "set_molecule must precede assign_patches");
   const int32_t start[2] = {0, c->natoms};
   return assignPatchesImpl(c, 1, start, box, pos);
+}
+
+int mi355nb_compute_assign_patches_lattice(mi355nb_compute* c, const double lattice[9], const double* pos) {
+  CHECK_ARG(c && lattice && pos, "null argument");
+  CHECK_STATE(c->haveMolecule, "set_molecule must precede assign_patches_lattice");
+  const int32_t start[2] = {0, c->natoms};
+  return assignPatchesImpl(c, 1, start, nullptr, pos, lattice);
+}
+
+int mi355nb_compute_get_system_cells(const mi355nb_compute* c, double* cells) {
+  CHECK_ARG(c && cells, "null argument");
+  CHECK_STATE(c->haveLayout, "no layout yet");
+  for (int s = 0; s < c->L.numSystems; ++s)
+    for (int k = 0; k < 9; ++k) cells[9 * s + k] = c->L.sysCell[18 * (size_t)s + k];
+  return MI355NB_OK;
 }
 
 int mi355nb_compute_assign_patches_batch(mi355nb_compute* c, int numSystems, const int32_t* sysAtomStart,
@@ -1014,6 +1050,7 @@ int mi355nb_compute_finish_batch(mi355nb_compute* c, double* forceOut, double* f
 int mi355nb_compute_update_positions_device(mi355nb_compute* c, const double* pos) {
   CHECK_ARG(c && pos, "null argument");
   CHECK_STATE(c->haveLayout, "assign_patches must precede update_positions_device");
+  CHECK_STATE(!c->L.triclinic, "device position packing serves orthorhombic boxes (use update_positions)");
   MI355NB_HIP(hipSetDevice(c->device));
   MI355NB_HIP(launchPackSlots(c->L.atomStorageSize, c->atomIndex.ptr, c->slotPatch.ptr, c->patchCentre.ptr,
                               c->patchSystemDev.ptr, c->sysBoxDev.ptr, POS_QUANTUM, pos, c->xyzq.ptr, c->stream));
@@ -1034,7 +1071,7 @@ int mi355nb_compute_exclusion_correction(mi355nb_compute* c, double* energyViria
   MI355NB_HIP(launchExclusionCorrection((int)c->exclBlocksHost.size(), c->L.numSystems, c->exclSlotPtr.ptr,
                                         c->exclSlotIdx.ptr, c->exclBlocks.ptr, c->exclSysStart.ptr, c->xyzq.ptr,
                                         c->slotPatch.ptr,
-                                        c->patchCentre.ptr, c->sysBoxDev.ptr, c->p.cutoff, c->p.ewaldcof,
+                                        c->patchCentre.ptr, c->sysCellDev.ptr, c->p.cutoff, c->p.ewaldcof,
                                         c->forceSlow.ptr, c->exclPartials.ptr, energyVirial, c->stream));
   return MI355NB_OK;
 }
@@ -1090,7 +1127,7 @@ int mi355nb_compute_modified_exclusion_correction(mi355nb_compute* c, double* en
   a.xyzq = c->xyzq.ptr;
   a.slotPatch = c->slotPatch.ptr;
   a.centre = c->patchCentre.ptr;
-  a.sysBox = c->sysBoxDev.ptr;
+  a.sysCell = c->sysCellDev.ptr;
   a.vdwTypes = c->vdwTypes.ptr;
   a.ntypes = c->ntypes;
   a.dAB = c->modDAB.ptr;

@@ -28,6 +28,19 @@ namespace mi355nb {
 constexpr int EXCL_THREADS = 256;
 constexpr int EXCL_VALUES = 10;  // energy, virial[9]
 
+// Minimum image of an FP64 displacement in a periodic cell given as 18
+// doubles: lattice rows a, b, c, then reciprocal rows (fractional
+// coordinates rounded to the nearest integer; exact for displacements
+// shorter than half of every perpendicular width, and the subtracted image is
+// an integer combination of the quantized lattice rows, so exact in FP64).
+__device__ __forceinline__ void minImage(double d[3], const double* __restrict__ cell) {
+  double n[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) n[j] = rint(cell[9 + 3 * j] * d[0] + cell[9 + 3 * j + 1] * d[1] + cell[9 + 3 * j + 2] * d[2]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) d[c] -= n[0] * cell[c] + n[1] * cell[3 + c] + n[2] * cell[6 + c];
+}
+
 __device__ __forceinline__ double blockSum256(double v, double* sh) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -44,7 +57,7 @@ __global__ void __launch_bounds__(EXCL_THREADS)
 exclusionCorrectionKernel(const int* __restrict__ exclPtr, const int* __restrict__ exclSlot,
                           const int4* __restrict__ blocks, const F4* __restrict__ xyzq,
                           const int* __restrict__ slotPatch, const double* __restrict__ centre,
-                          const double* __restrict__ sysBox, double cutoff2, double beta, F4* __restrict__ forceSlow,
+                          const double* __restrict__ sysCell, double cutoff2, double beta, F4* __restrict__ forceSlow,
                           int doEnergyVirial, double* __restrict__ partials) {
   __shared__ double sh[EXCL_THREADS / 64];
   const int4 bk = blocks[blockIdx.x];   // {sys, begin slot, end slot, -}
@@ -57,7 +70,7 @@ exclusionCorrectionKernel(const int* __restrict__ exclPtr, const int* __restrict
     if (m0 < m1) {
       const F4 a = xyzq[k];
       const int pa = slotPatch[k];
-      const double L[3] = {sysBox[3 * bk.x], sysBox[3 * bk.x + 1], sysBox[3 * bk.x + 2]};
+      const double* cellS = sysCell + 18 * bk.x;
       const double xa[3] = {centre[3 * pa] + (double)a.x, centre[3 * pa + 1] + (double)a.y,
                             centre[3 * pa + 2] + (double)a.z};
       double f[3] = {0.0, 0.0, 0.0};
@@ -67,8 +80,7 @@ exclusionCorrectionKernel(const int* __restrict__ exclPtr, const int* __restrict
         const int pb = slotPatch[j];
         double d[3] = {xa[0] - (centre[3 * pb] + (double)b.x), xa[1] - (centre[3 * pb + 1] + (double)b.y),
                        xa[2] - (centre[3 * pb + 2] + (double)b.z)};
-#pragma unroll
-        for (int c = 0; c < 3; ++c) d[c] -= L[c] * rint(d[c] / L[c]);
+        minImage(d, cellS);
         const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         if (r2 < cutoff2 && r2 > 0.0) {
           const double qq = (double)a.w * (double)b.w;
@@ -154,7 +166,7 @@ __global__ void __launch_bounds__(EXCL_THREADS) modifiedExclusionKernel(const Mo
       const F4 p = a.xyzq[k];
       const int pa = a.slotPatch[k];
       const int ta = a.vdwTypes[k];
-      const double L[3] = {a.sysBox[3 * bk.x], a.sysBox[3 * bk.x + 1], a.sysBox[3 * bk.x + 2]};
+      const double* cellS = a.sysCell + 18 * bk.x;
       const double xa[3] = {a.centre[3 * pa] + (double)p.x, a.centre[3 * pa + 1] + (double)p.y,
                             a.centre[3 * pa + 2] + (double)p.z};
       double f[3] = {0.0, 0.0, 0.0}, fs[3] = {0.0, 0.0, 0.0};
@@ -164,8 +176,7 @@ __global__ void __launch_bounds__(EXCL_THREADS) modifiedExclusionKernel(const Mo
         const int pb = a.slotPatch[j];
         double d[3] = {xa[0] - (a.centre[3 * pb] + (double)b.x), xa[1] - (a.centre[3 * pb + 1] + (double)b.y),
                        xa[2] - (a.centre[3 * pb + 2] + (double)b.z)};
-#pragma unroll
-        for (int c = 0; c < 3; ++c) d[c] -= L[c] * rint(d[c] / L[c]);
+        minImage(d, cellS);
         const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         if (!(r2 < a.rc2 && r2 > 0.0)) continue;
         const int tb = a.vdwTypes[j];
@@ -328,11 +339,11 @@ void buildExclusionBlocks(const int* sysSlotStart, int numSystems, std::vector<i
 
 hipError_t launchExclusionCorrection(int numBlocks, int numSystems, const int* exclPtr, const int* exclSlot,
                                      const int4* blocks, const int* sysBlockStart, const F4* xyzq,
-                                     const int* slotPatch, const double* centre, const double* sysBox, double cutoff,
+                                     const int* slotPatch, const double* centre, const double* sysCell, double cutoff,
                                      double beta, F4* forceSlow, double* partials, double* out, hipStream_t s) {
   if (numBlocks > 0) {
     hipLaunchKernelGGL(exclusionCorrectionKernel, dim3(numBlocks), dim3(EXCL_THREADS), 0, s, exclPtr, exclSlot,
-                       blocks, xyzq, slotPatch, centre, sysBox, cutoff * cutoff, beta, forceSlow, out ? 1 : 0,
+                       blocks, xyzq, slotPatch, centre, sysCell, cutoff * cutoff, beta, forceSlow, out ? 1 : 0,
                        partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

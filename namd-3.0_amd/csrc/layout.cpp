@@ -246,6 +246,38 @@ bool PatchLayout::build(int n, const double* pos, const double boxIn[3], double 
   return buildBatch(1, start, pos, boxIn, patchDim, err);
 }
 
+// Per-system lattice and reciprocal rows for the device (orthorhombic:
+// diagonal; the single-system general cell: its own).
+void PatchLayout::fillSysCell() {
+  sysCell.assign(18 * (size_t)numSystems, 0.0);
+  for (int s = 0; s < numSystems; ++s) {
+    double* o = &sysCell[18 * (size_t)s];
+    if (triclinic) {
+      for (int k = 0; k < 9; ++k) { o[k] = cell[k]; o[9 + k] = recip[k]; }
+    } else {
+      for (int d = 0; d < 3; ++d) {
+        o[4 * d] = sysBox[3 * s + d];
+        o[9 + 4 * d] = 1.0 / sysBox[3 * s + d];
+      }
+    }
+  }
+}
+
+bool PatchLayout::buildCell(int n, const double* pos, const double cellIn[9], double patchDim, std::string& err) {
+  numSystems = 1;
+  sysAtomStart.assign({0, n});
+  sysSlotStart.assign(2, 0);
+  sysListStart.assign(2, 0);
+  natoms = n;
+  if (!buildOne(n, pos, nullptr, patchDim, err, cellIn)) return false;
+  sysBox.assign(box, box + 3);
+  patchSystem.assign(patches.size(), 0);
+  sysSlotStart[1] = atomStorageSize;
+  sysListStart[1] = (int)tileListSkeleton.size();
+  fillSysCell();
+  return true;
+}
+
 bool PatchLayout::buildBatch(int nsys, const int* atomStart, const double* pos, const double* boxes,
                              double patchDim, std::string& err) {
   if (nsys < 1) { err = "no systems"; return false; }
@@ -253,6 +285,7 @@ bool PatchLayout::buildBatch(int nsys, const int* atomStart, const double* pos, 
     if (atomStart[s + 1] <= atomStart[s]) { err = "every system needs at least one atom"; return false; }
   if (atomStart[0] != 0) { err = "system atom ranges must start at 0"; return false; }
   numSystems = nsys;
+  triclinic = false;
   sysAtomStart.assign(atomStart, atomStart + nsys + 1);
   sysSlotStart.assign(nsys + 1, 0);
   sysListStart.assign(nsys + 1, 0);
@@ -266,6 +299,7 @@ bool PatchLayout::buildBatch(int nsys, const int* atomStart, const double* pos, 
     patchSystem.assign(patches.size(), 0);
     sysSlotStart[1] = atomStorageSize;
     sysListStart[1] = (int)tileListSkeleton.size();
+    fillSysCell();
     return true;
   }
   centre.clear(); patches.clear(); computes.clear(); tileListSkeleton.clear(); patchSystem.clear();
@@ -324,18 +358,49 @@ bool PatchLayout::buildBatch(int nsys, const int* atomStart, const double* pos, 
   vdwTypes.assign(atomStorageSize, 0);
   exclStartMaxdiff.assign(atomStorageSize, I2{0, 0});
   effPos.assign(3 * (size_t)natoms, 0.0);
+  fillSysCell();
   return true;
 }
 
 bool PatchLayout::buildOne(int n, const double* pos, const double boxIn[3], double patchDim,
-                           std::string& err) {
+                           std::string& err, const double* cellIn) {
   natoms = n;
-  for (int d = 0; d < 3; ++d) {
-    box[d] = quantize(boxIn[d], POS_QUANTUM);
-    if (!(box[d] > 0)) { err = "box edge must be positive"; return false; }
-    dims[d] = std::max(1, (int)std::floor(box[d] / patchDim));
-    patchSize[d] = box[d] / dims[d];
-    if (patchSize[d] > 40.0) { err = "patch edge above 40 A (coordinates would leave the exact FP32 range)"; return false; }
+  triclinic = cellIn != nullptr;
+  if (triclinic) {
+    // lattice rows quantized; reciprocal rows r_a = (b x c)/V, ...;
+    // perpendicular widths 1/|r_j| set the patch counts
+    for (int k = 0; k < 9; ++k) {
+      if (!std::isfinite(cellIn[k])) { err = "non-finite lattice vector"; return false; }
+      cell[k] = quantize(cellIn[k], POS_QUANTUM);
+    }
+    const double* a = cell;
+    const double* b = cell + 3;
+    const double* c = cell + 6;
+    const double bxc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+    const double cxa[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+    const double axb[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double V = a[0] * bxc[0] + a[1] * bxc[1] + a[2] * bxc[2];
+    if (!(V > 0)) { err = "lattice vectors a, b, c must be right-handed with positive volume"; return false; }
+    for (int d = 0; d < 3; ++d) { recip[d] = bxc[d] / V; recip[3 + d] = cxa[d] / V; recip[6 + d] = axb[d] / V; }
+    double halfDiag = 0.0;
+    for (int j = 0; j < 3; ++j) {
+      const double* r = recip + 3 * j;
+      const double w = 1.0 / std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      dims[j] = std::max(1, (int)std::floor(w / patchDim));
+      const double* v = cell + 3 * j;
+      box[j] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      patchSize[j] = box[j] / dims[j];
+      halfDiag += 0.5 * patchSize[j];
+    }
+    if (halfDiag > 60.0) { err = "patch diagonal above 120 A (coordinates would leave the exact FP32 range)"; return false; }
+  } else {
+    for (int d = 0; d < 3; ++d) {
+      box[d] = quantize(boxIn[d], POS_QUANTUM);
+      if (!(box[d] > 0)) { err = "box edge must be positive"; return false; }
+      dims[d] = std::max(1, (int)std::floor(box[d] / patchDim));
+      patchSize[d] = box[d] / dims[d];
+      if (patchSize[d] > 40.0) { err = "patch edge above 40 A (coordinates would leave the exact FP32 range)"; return false; }
+    }
   }
   const int np = dims[0] * dims[1] * dims[2];
   centre.assign(3 * (size_t)np, 0.0);
@@ -344,7 +409,15 @@ bool PatchLayout::buildOne(int n, const double* pos, const double boxIn[3], doub
       for (int gz = 0; gz < dims[2]; ++gz) {
         int p = (gx * dims[1] + gy) * dims[2] + gz;
         int g[3] = {gx, gy, gz};
-        for (int d = 0; d < 3; ++d) centre[3 * p + d] = quantize((g[d] + 0.5) * patchSize[d], POS_QUANTUM);
+        if (triclinic) {
+          for (int d = 0; d < 3; ++d) {
+            double x = 0.0;
+            for (int j = 0; j < 3; ++j) x += (g[j] + 0.5) / dims[j] * cell[3 * j + d];
+            centre[3 * p + d] = quantize(x, POS_QUANTUM);
+          }
+        } else {
+          for (int d = 0; d < 3; ++d) centre[3 * p + d] = quantize((g[d] + 0.5) * patchSize[d], POS_QUANTUM);
+        }
       }
 
   // assign atoms: patch of every atom, then a counting sort by patch
@@ -353,6 +426,19 @@ bool PatchLayout::buildOne(int n, const double* pos, const double boxIn[3], doub
   std::vector<int> first(np + 1, 0);
   for (int i = 0; i < n; ++i) {
     int g[3];
+    if (triclinic) {
+      // fractional coordinates wrapped into [0, 1), the patch from them
+      double sw[3];
+      for (int j = 0; j < 3; ++j) {
+        const double* r = recip + 3 * j;
+        double sj = r[0] * pos[3 * i] + r[1] * pos[3 * i + 1] + r[2] * pos[3 * i + 2];
+        sj -= std::floor(sj);
+        if (sj >= 1.0) sj -= 1.0;
+        sw[j] = sj;
+        g[j] = std::min(dims[j] - 1, (int)(sj * dims[j]));
+      }
+      for (int d = 0; d < 3; ++d) wpos[3 * i + d] = sw[0] * cell[d] + sw[1] * cell[3 + d] + sw[2] * cell[6 + d];
+    } else
     for (int d = 0; d < 3; ++d) {
       double w = wrap(pos[3 * i + d], box[d]);
       wpos[3 * i + d] = w;
@@ -436,11 +522,15 @@ bool PatchLayout::buildOne(int n, const double* pos, const double boxIn[3], doub
               int g1[3] = {gx + dx, gy + dy, gz + dz};
               int g2[3];
               double T[3];
+              int qv[3];
               for (int d = 0; d < 3; ++d) {
                 int q = (int)std::floor((double)g1[d] / dims[d]);
                 g2[d] = g1[d] - q * dims[d];
+                qv[d] = q;
                 T[d] = q * box[d];
               }
+              if (triclinic)   // the image is an integer combination of the lattice rows
+                for (int d = 0; d < 3; ++d) T[d] = qv[0] * cell[d] + qv[1] * cell[3 + d] + qv[2] * cell[6 + d];
               int pb = (g2[0] * dims[1] + g2[1]) * dims[2] + g2[2];
               ComputeRecord c;
               c.patchInd[0] = pa;
@@ -536,6 +626,15 @@ void PatchLayout::packPatches(const double* pos, int p0, int p1) {
       int a = atomIndex[s];
       if (a < 0) continue;
       double q3[3];
+      if (triclinic) {
+        // the image nearest the patch centre in fractional coordinates
+        double x[3], nv[3];
+        for (int d = 0; d < 3; ++d) x[d] = pos[3 * a + d] - centre[3 * p + d];
+        for (int j = 0; j < 3; ++j)
+          nv[j] = roundEven(recip[3 * j] * x[0] + recip[3 * j + 1] * x[1] + recip[3 * j + 2] * x[2]);
+        for (int d = 0; d < 3; ++d)
+          q3[d] = quantize(x[d] - (nv[0] * cell[d] + nv[1] * cell[3 + d] + nv[2] * cell[6 + d]), POS_QUANTUM);
+      } else
       for (int d = 0; d < 3; ++d) {
         // image of the atom closest to its patch centre
         double x = pos[3 * a + d] - centre[3 * p + d];

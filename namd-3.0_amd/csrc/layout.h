@@ -81,6 +81,12 @@ struct PatchLayout {
   std::vector<int> sysSlotStart;    // numSystems + 1
   std::vector<int> sysListStart;    // numSystems + 1
   std::vector<double> sysBox;       // 3 per system, quantized
+  std::vector<double> sysCell;      // 18 per system: lattice rows a, b, c, then reciprocal rows
+  // general (triclinic) cell of a single system: rows a, b, c (multiples of
+  // POS_QUANTUM) and reciprocal rows (s_j = recip_j . x)
+  bool triclinic = false;
+  double cell[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double recip[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<int> patchSystem;     // per patch
   std::vector<double> centre;       // 3 per patch, multiples of POS_QUANTUM
 
@@ -105,6 +111,13 @@ struct PatchLayout {
   // computes (self + 13 half-shell neighbours) and the tile-list skeleton.
   bool build(int n, const double* pos, const double boxIn[3], double patchDim, std::string& err);
 
+  // One system in a general periodic cell (NAMD's lata / latb / latc):
+  // cellIn rows a, b, c, right-handed.  The patch grid divides each lattice
+  // direction by floor(perpendicular width / patchDim); atoms are wrapped and
+  // assigned in fractional coordinates, and every image shift is an integer
+  // combination of the quantized lattice vectors (exact in FP32).
+  bool buildCell(int n, const double* pos, const double cellIn[9], double patchDim, std::string& err);
+
   // The same for numSystems systems; boxes holds 3 edge lengths per system.
   bool buildBatch(int numSystems, const int* atomStart, const double* pos, const double* boxes,
                   double patchDim, std::string& err);
@@ -124,7 +137,9 @@ struct PatchLayout {
  public:
 
  private:
-  bool buildOne(int n, const double* pos, const double boxIn[3], double patchDim, std::string& err);
+  bool buildOne(int n, const double* pos, const double boxIn[3], double patchDim, std::string& err,
+                const double* cellIn = nullptr);
+  void fillSysCell();
 };
 
 // ORB partition of one patch into TILE-sized spatially compact groups

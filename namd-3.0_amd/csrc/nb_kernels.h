@@ -85,7 +85,7 @@ void buildExclusionBlocks(const int* sysSlotStart, int numSystems, std::vector<i
                           std::vector<int>& sysBlockStart);
 hipError_t launchExclusionCorrection(int numBlocks, int numSystems, const int* exclPtr, const int* exclSlot,
                                      const int4* blocks, const int* sysBlockStart, const F4* xyzq,
-                                     const int* slotPatch, const double* centre, const double* sysBox, double cutoff,
+                                     const int* slotPatch, const double* centre, const double* sysCell, double cutoff,
                                      double beta, F4* forceSlow, double* partials, double* out, hipStream_t s);
 
 // Modified (scaled 1-4) exclusion correction in FP64 (exclusion.hip): the
@@ -103,7 +103,7 @@ struct ModExclArgs {
   const F4* xyzq;
   const int* slotPatch;
   const double* centre;
-  const double* sysBox;
+  const double* sysCell;      // 18 per system: lattice rows, reciprocal rows
   const int* vdwTypes;        // per slot
   int ntypes;
   const double* dAB;          // per ordered type pair: A14 - A, B14 - B

@@ -40,7 +40,10 @@ class ComputeNonbonded:
                 device=device, stream=stream)
         A, B = lj_tables(s)
         c.set_molecule(s.vdw_type, s.charge, A, B, s.excl_ptr, s.excl_idx)
-        c.assign_patches(s.box, s.pos)
+        if s.cell is not None:
+            c.assign_patches_lattice(s.cell, s.pos)
+        else:
+            c.assign_patches(s.box, s.pos)
         return c
 
     @classmethod
@@ -111,6 +114,20 @@ class ComputeNonbonded:
                                                        np.ascontiguousarray(pos, np.float64).reshape(-1)),
                 "assign_patches")
         self.sys_atom_start = np.array([0, self.natoms], np.int32)
+
+    def assign_patches_lattice(self, lattice, pos):
+        """One system in a general periodic cell: lattice rows a, b, c (3 x 3)."""
+        lat = np.ascontiguousarray(lattice, np.float64).reshape(-1)
+        L.check(L.lib().mi355nb_compute_assign_patches_lattice(self._h, lat.ctypes.data, np.ascontiguousarray(
+            pos, np.float64).reshape(-1).ctypes.data), "assign_patches_lattice")
+        self.sys_atom_start = np.array([0, self.natoms], np.int32)
+
+    def system_cells(self):
+        """(numSystems, 3, 3) lattice rows the kernels use (rounded to 2^-17 A)."""
+        n = self.info()["numSystems"]
+        b = np.zeros(9 * n)
+        L.check(L.lib().mi355nb_compute_get_system_cells(self._h, b.ctypes.data), "get_system_cells")
+        return b.reshape(n, 3, 3)
 
     def assign_patches_batch(self, sys_atom_start, boxes, pos):
         starts = np.ascontiguousarray(sys_atom_start, np.int32)

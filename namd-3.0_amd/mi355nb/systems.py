@@ -84,6 +84,8 @@ class System:
     chain_len: int = 0       # beads per chain (chains are stored first, chain by chain)
     lj_AB: tuple | None = None   # ordered (A, B) ntypes x ntypes tables given directly (e.g. read
                                  # from a dumpbench file) instead of mixing lj_sigma / lj_eps
+    cell: np.ndarray | None = None   # (3,3) lattice rows a, b, c of a general periodic cell
+                                     # (None: the orthorhombic box)
 
     @property
     def natoms(self) -> int:
@@ -447,6 +449,17 @@ def chain_bonded_terms(s: System):
     theta0 = np.arccos(np.clip(cs, -1.0, 1.0))
     return (bonds, np.full(len(bonds), BEAD_BOND_K), np.full(len(bonds), BEAD_BOND_R0), angles,
             np.full(len(angles), BEAD_ANGLE_K), theta0)
+
+
+def sheared(s: System, cell) -> System:
+    """The same molecules in a general periodic cell: fractional coordinates
+    of the orthorhombic box carried onto lattice rows a, b, c (cell, 3 x 3),
+    x = s_a a + s_b b + s_c c.  Bond geometry follows the shear; the
+    exclusion lists are unchanged."""
+    import dataclasses
+    cell = np.asarray(cell, np.float64).reshape(3, 3)
+    frac = s.pos / s.box
+    return dataclasses.replace(s, name=s.name + "-triclinic", pos=frac @ cell, cell=cell)
 
 
 def chain_pairs_14(s: System):

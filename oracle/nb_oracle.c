@@ -25,6 +25,8 @@
  *                          (vdwForceSwitching: ComputeNonbondedUtil.C:523-533,
  *                          :900-919; alchemical: ComputeNonbondedFEP.C:136-159,
  *                          ComputeNonbondedTI.C:113-128).
+ *  oracle_nb_fp64_cell     oracle_nb_fp64 in a general (triclinic) cell, by
+ *                          brute force over pairs and images.
  *  oracle_mod_excl_fp64    the modified (scaled 1-4) exclusion correction
  *                          of modifiedExclusionForce
  *                          (ComputeBondedCUDAKernel.cu:316-632) in FP64.
@@ -1392,4 +1394,106 @@ int oracle_mod_excl_alch_fp64(int n, const double* pos, const double* q, const i
   al.coeff = lam[12];
   return mod_excl_core(n, pos, q, type, ntypes, A, B, A14, B14, mptr, midx, L, cutoff, switchdist, scale14, 0,
                        part, &al, f, fs, energies, 12, virial, virialSlow, npairs);
+}
+
+/* ---- general (triclinic) cell ----------------------------------------------
+ * oracle_nb_fp64 in a general periodic cell (lattice rows a, b, c, NAMD's
+ * Lattice), by brute force: every pair i < j, each of the 27 images around
+ * the fractional minimum image that lies within the cutoff (inclusive), the
+ * pair physics of pair_fp64.  Shares no cell-list or wrapping code with the
+ * layout it checks. */
+int oracle_nb_fp64_cell(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+                        const double* A, const double* B, const int64_t* eptr, const int32_t* eidx,
+                        const double cell[9], double cutoff, double switchdist, double beta, double* f, double* fs,
+                        double* energies, double* virial, double* virialSlow, int64_t* npairs, int nthreads) {
+  PairConst c;
+  memset(&c, 0, sizeof(c));
+  c.rc = cutoff;
+  c.rc2 = cutoff * cutoff;
+  c.rs2 = switchdist * switchdist;
+  c.c1 = (switchdist < cutoff) ? 1.0 / ((c.rc2 - c.rs2) * (c.rc2 - c.rs2) * (c.rc2 - c.rs2)) : 0.0;
+  c.rcinv = 1.0 / cutoff;
+  c.rc3inv = c.rcinv * c.rcinv * c.rcinv;
+  c.beta = beta;
+  const double* a = cell;
+  const double* b = cell + 3;
+  const double* cc = cell + 6;
+  const double bxc[3] = {b[1] * cc[2] - b[2] * cc[1], b[2] * cc[0] - b[0] * cc[2], b[0] * cc[1] - b[1] * cc[0]};
+  const double cxa[3] = {cc[1] * a[2] - cc[2] * a[1], cc[2] * a[0] - cc[0] * a[2], cc[0] * a[1] - cc[1] * a[0]};
+  const double axb[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const double V = a[0] * bxc[0] + a[1] * bxc[1] + a[2] * bxc[2];
+  double R[9];
+  for (int d = 0; d < 3; ++d) { R[d] = bxc[d] / V; R[3 + d] = cxa[d] / V; R[6 + d] = axb[d] / V; }
+  int nt = 1;
+#ifdef _OPENMP
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+  double* fb = (double*)calloc((size_t)nt * 6 * n, sizeof(double));
+  double* acc = (double*)calloc((size_t)nt * 21, sizeof(double));
+  int64_t* np = (int64_t*)calloc((size_t)nt, sizeof(int64_t));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nt)
+#endif
+  for (int i = 0; i < n; ++i) {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double* F = fb + (size_t)tid * 6 * n;
+    double* S = acc + (size_t)tid * 21;
+    for (int j = i + 1; j < n; ++j) {
+      double d0[3];
+      for (int u = 0; u < 3; ++u) d0[u] = pos[3 * j + u] - pos[3 * i + u];
+      double s[3];
+      for (int k = 0; k < 3; ++k) s[k] = nearbyint(R[3 * k] * d0[0] + R[3 * k + 1] * d0[1] + R[3 * k + 2] * d0[2]);
+      for (int u = 0; u < 3; ++u) d0[u] -= s[0] * a[u] + s[1] * b[u] + s[2] * cc[u];
+      int excl = -1;
+      for (int ia = -1; ia <= 1; ++ia)
+        for (int ib = -1; ib <= 1; ++ib)
+          for (int ic = -1; ic <= 1; ++ic) {
+            double d[3];
+            for (int u = 0; u < 3; ++u) d[u] = d0[u] + ia * a[u] + ib * b[u] + ic * cc[u];
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (!(r2 <= c.rc2)) continue;
+            if (excl < 0) excl = is_excluded(eptr, eidx, i, j);
+            if (excl) continue;
+            double ff, ffs;
+            const int ab = type[i] * ntypes + type[j];
+            pair_fp64(&c, r2, q[i] * q[j], A[ab], B[ab], &ff, &ffs, &S[18], &S[19], &S[20]);
+            for (int u = 0; u < 3; ++u) {
+              F[3 * i + u] += ff * d[u];
+              F[3 * j + u] -= ff * d[u];
+              F[3 * n + 3 * i + u] += ffs * d[u];
+              F[3 * n + 3 * j + u] -= ffs * d[u];
+            }
+            for (int x = 0; x < 3; ++x)
+              for (int y = 0; y < 3; ++y) {
+                S[x * 3 + y] -= ff * d[x] * d[y];
+                S[9 + x * 3 + y] -= ffs * d[x] * d[y];
+              }
+            np[tid] += 1;
+          }
+    }
+  }
+  if (f) memset(f, 0, sizeof(double) * 3 * n);
+  if (fs) memset(fs, 0, sizeof(double) * 3 * n);
+  double tot[21] = {0};
+  int64_t ntot = 0;
+  for (int t = 0; t < nt; ++t) {
+    const double* F = fb + (size_t)t * 6 * n;
+    if (f)
+      for (int k = 0; k < 3 * n; ++k) f[k] += F[k];
+    if (fs)
+      for (int k = 0; k < 3 * n; ++k) fs[k] += F[3 * n + k];
+    for (int k = 0; k < 21; ++k) tot[k] += acc[(size_t)t * 21 + k];
+    ntot += np[t];
+  }
+  if (virial) memcpy(virial, tot, 9 * sizeof(double));
+  if (virialSlow) memcpy(virialSlow, tot + 9, 9 * sizeof(double));
+  if (energies) memcpy(energies, tot + 18, 3 * sizeof(double));
+  if (npairs) *npairs = ntot;
+  free(fb);
+  free(acc);
+  free(np);
+  return 0;
 }

@@ -59,6 +59,8 @@ def lib():
         L.oracle_nb_fp64_alch_vdw_force_switch.argtypes = L.oracle_nb_fp64_alch.argtypes
         L.oracle_mod_excl_fp64.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, dp, dp, i64p, i32p, dp,
                                            C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp, dp, i64p]
+        L.oracle_nb_fp64_cell.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, i64p, i32p, dp, C.c_double,
+                                          C.c_double, C.c_double, dp, dp, dp, dp, dp, i64p, C.c_int]
         L.oracle_mod_excl_alch_fp64.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, dp, dp, i64p, i32p, dp,
                                                 C.c_double, C.c_double, C.c_double, i8p, dp, C.c_int, dp, dp, dp,
                                                 dp, dp, i64p]
@@ -126,6 +128,25 @@ def nb_fp64_alch(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff,
                 e_elec_s=en[4], e_slow_s=en[5], e_vdw_ti_1=en[6], e_vdw_ti_2=en[7], e_elec_ti_1=en[8],
                 e_elec_ti_2=en[9], e_slow_ti_1=en[10], e_slow_ti_2=en[11], virial=vir.reshape(3, 3),
                 virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]))
+
+
+def nb_fp64_cell(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, cell, cutoff, switchdist, beta, nthreads=0):
+    """nb_fp64 in a general periodic cell (cell: 3 x 3 lattice rows a, b, c),
+    by brute force over all pairs and the 27 images around the minimum image."""
+    n = len(q_scaled)
+    ntypes = int(round(np.sqrt(np.asarray(A).size)))
+    f = np.zeros(3 * n)
+    fs = np.zeros(3 * n)
+    en = np.zeros(3)
+    vir = np.zeros(9)
+    virs = np.zeros(9)
+    npairs = np.zeros(1, np.int64)
+    lib().oracle_nb_fp64_cell(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32),
+                              ntypes, _c(A, np.float64).reshape(-1), _c(B, np.float64).reshape(-1),
+                              _c(excl_ptr, np.int64), _c(excl_idx, np.int32), _c(cell, np.float64).reshape(-1),
+                              cutoff, switchdist, beta, f, fs, en, vir, virs, npairs, int(nthreads))
+    return dict(f=f.reshape(n, 3), fs=fs.reshape(n, 3), e_vdw=en[0], e_elec=en[1], e_slow=en[2],
+                virial=vir.reshape(3, 3), virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]))
 
 
 def mod_excl_fp64(pos, q_scaled, vdw_type, A, B, A14, B14, mod_ptr, mod_idx, box, cutoff, switchdist, scale14,
