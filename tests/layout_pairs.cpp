@@ -4,6 +4,9 @@
 // arithmetic (FP32 patch-centred coordinates plus FP32 shift, exact r^2) and
 // the exclusion bit table, and writes the pairs and the effective positions.
 // usage: layout_pairs in.bin rc patchDim pairs.out effpos.out
+// A negative system count in in.bin means one system in a general cell: the
+// 3 box doubles are then 9 (lattice rows a, b, c) and PatchLayout::buildCell
+// lays it out.
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -16,11 +19,14 @@ int main(int argc, char** argv) {
   FILE* f = fopen(argv[1], "rb");
   int nsys = 0;
   if (fread(&nsys, 4, 1, f) != 1) return 2;
+  const bool cellMode = nsys < 0;
+  if (cellMode) nsys = 1;
   std::vector<int> start(nsys + 1);
   if (fread(start.data(), 4, nsys + 1, f) != (size_t)(nsys + 1)) return 2;
   int n = start[nsys];
-  std::vector<double> boxes(3 * nsys);
-  if (fread(boxes.data(), 8, 3 * nsys, f) != (size_t)(3 * nsys)) return 2;
+  const int nbox = cellMode ? 9 : 3 * nsys;
+  std::vector<double> boxes(nbox);
+  if (fread(boxes.data(), 8, nbox, f) != (size_t)nbox) return 2;
   std::vector<double> pos(3 * n);
   if (fread(pos.data(), 8, 3 * n, f) != (size_t)(3 * n)) return 2;
   std::vector<long long> ptr(n + 1);
@@ -33,7 +39,9 @@ int main(int argc, char** argv) {
   std::vector<I2> sm; std::vector<uint32_t> bits; int nl = 0; std::string err;
   if (!buildExclusions(n, eptr.data(), idx.data(), sm, bits, nl, err)) { printf("excl err %s\n", err.c_str()); return 1; }
   PatchLayout L;
-  if (!L.buildBatch(nsys, start.data(), pos.data(), boxes.data(), pl, err)) { printf("layout err %s\n", err.c_str()); return 1; }
+  const bool ok = cellMode ? L.buildCell(n, pos.data(), boxes.data(), pl, err)
+                           : L.buildBatch(nsys, start.data(), pos.data(), boxes.data(), pl, err);
+  if (!ok) { printf("layout err %s\n", err.c_str()); return 1; }
   std::vector<double> q(n, 0.0); std::vector<int> t(n, 0);
   L.pack(pos.data(), q.data(), t.data(), sm.data());
   FILE* o = fopen(argv[4], "wb");

@@ -69,3 +69,48 @@ def test_batched_layout_covers_exact_pair_sets(helper):
         mine = set(x for x in gs if a <= x[0] < b)
         assert all(a <= x[1] < b for x in mine), "cross-system pair"
         assert mine == ref, (k, len(mine - ref), len(ref - mine))
+
+
+@pytest.mark.parametrize("cell", [
+    [[34.0, 0.0, 0.0], [8.0, 33.0, 0.0], [-6.0, 7.0, 34.0]],          # skewed, 2 patches per direction
+    [[29.3, 0.0, 0.0], [14.1, 27.7, 0.0], [13.9, 9.2, 28.1]],         # strongly skewed, 1-2 patches
+    [[45.1, 1.3, -0.7], [-3.2, 41.9, 2.2], [5.5, -4.1, 47.3]],        # general orientation, 3 patches
+])
+def test_general_cell_layout_covers_exact_pair_sets(helper, cell):
+    """One system in a general cell: the skeleton covers exactly the pairs
+    within the cutoff of a brute-force enumeration over the 27 images around
+    the fractional minimum image (same rounded lattice, effective positions)."""
+    d, exe = helper
+    cell = np.asarray(cell)
+    base = systems.small_box(n_waters=500, n_chains=2, chain_len=20, box=30.0, seed=8)
+    s = systems.sheared(base, cell)
+    with open(d / "cell.bin", "wb") as f:
+        np.array([-1], np.int32).tofile(f)
+        np.array([0, s.natoms], np.int32).tofile(f)
+        cell.astype(np.float64).tofile(f)
+        s.pos.astype(np.float64).tofile(f)
+        np.asarray(s.excl_ptr, np.int64).tofile(f)
+        np.asarray(s.excl_idx, np.int32).tofile(f)
+    out = subprocess.run([exe, str(d / "cell.bin"), "12.0", "13.5", str(d / "cpairs.bin"), str(d / "ceff.bin")],
+                         check=True, capture_output=True, text=True)
+    got = np.fromfile(d / "cpairs.bin", np.int32).reshape(-1, 2)
+    eff = np.fromfile(d / "ceff.bin", np.float64).reshape(-1, 3)
+    gs = set(map(tuple, got.tolist()))
+    assert len(gs) == len(got), "a pair is covered twice"
+    lat = np.round(cell * 131072.0) / 131072.0
+    inv = np.linalg.inv(lat)
+    imgs = np.array([[i, j, k] for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)], np.float64) @ lat
+    excl = set()
+    for i in range(s.natoms):
+        for j in s.excl_idx[s.excl_ptr[i]:s.excl_ptr[i + 1]]:
+            excl.add((min(i, int(j)), max(i, int(j))))
+    ref = set()
+    for i in range(s.natoms - 1):
+        dd = eff[i + 1:] - eff[i]
+        dd = dd - np.rint(dd @ inv) @ lat
+        r2 = (((dd[:, None, :] + imgs[None, :, :]) ** 2).sum(2) <= 144.0).any(1)
+        for j in np.nonzero(r2)[0] + i + 1:
+            if (i, int(j)) not in excl:
+                ref.add((i, int(j)))
+    print(out.stdout.strip())
+    assert gs == ref, (len(gs - ref), len(ref - gs))
