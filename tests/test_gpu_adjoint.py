@@ -161,7 +161,7 @@ def test_autograd_function_and_descent(problem, torch_cuda):
     assert abs(float(loss.detach()) - l2) <= 1e-6 * l2
     assert _rel(sig.grad.numpy(), ds) < 1e-5 and _rel(eps.grad.numpy(), de) < 1e-5
     # a few normalised gradient steps on sigma and eps lower the loss
-    losses = [float(loss)]
+    losses = [float(loss.detach())]
     opt = torch.optim.Adam([sig, eps], lr=2e-3)
     for _ in range(5):
         opt.zero_grad()
