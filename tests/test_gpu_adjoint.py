@@ -168,6 +168,6 @@ def test_autograd_function_and_descent(problem, torch_cuda):
         l = FML.apply(sig, eps, q, fm)
         l.backward()
         opt.step()
-        losses.append(float(FML.apply(sig, eps, q, fm)))
+        losses.append(float(FML.apply(sig, eps, q, fm).detach()))
     print(f"[fit] losses {['%.4e' % v for v in losses]}")
     assert losses[-1] < losses[0]
