@@ -23,7 +23,9 @@ Sections, in order:
   COMPUTEPAIR     number of pair computes, then `pid1 trans1 pid2 trans2`,
                   trans = Lattice::index(i, j, k) = 9(k+1) + 3(j+1) + (i+1)
 
-What this build reads: an orthorhombic cell, exclude <= 1-3 or scaled1-4
+What this build reads: any right-handed cell (a general one comes back as
+System.cell, which ComputeNonbonded lays out with assign_patches_lattice),
+exclude <= 1-3 or scaled1-4
 (the modified pairs, scale14 and the 1-4 LJ table come back in info as
 mod_ptr / mod_idx / scale14 / A14 / B14, the arguments of
 ComputeNonbonded.set_modified_exclusions), no fixed atoms, no LES or
@@ -68,8 +70,18 @@ def _fmt_g(x: float) -> str:
 
 def _patch_grid(s: S.System):
     """NAMD-style patch grid of the build's layout: floor(box / (pairlistdist
-    + margin)) patches per edge, atoms binned by their wrapped position."""
+    + margin)) patches per edge (a general cell: per lattice direction, from
+    its perpendicular width), atoms binned by their wrapped position."""
     p = s.params
+    if s.cell is not None:
+        cell = np.asarray(s.cell, np.float64)
+        inv = np.linalg.inv(cell)
+        widths = 1.0 / np.linalg.norm(inv, axis=0)
+        dims = np.maximum(1, np.floor(widths / (p.pairlistdist + p.margin)).astype(int))
+        frac = np.mod(s.pos @ inv, 1.0)
+        g = np.minimum(dims - 1, np.floor(frac * dims).astype(int))
+        pid = (g[:, 0] * dims[1] + g[:, 1]) * dims[2] + g[:, 2]
+        return dims, pid
     box = np.asarray(s.box, np.float64)
     dims = np.maximum(1, np.floor(box / (p.pairlistdist + p.margin)).astype(int))
     w = np.mod(s.pos, box)
@@ -132,9 +144,10 @@ def write(f, s: S.System, A=None, B=None, alch=None, modified=None):
         B14 = np.asarray(modified["B14"], np.float64)
     else:
         A14, B14 = A, B
+    cellm = np.asarray(s.cell, np.float64) if s.cell is not None else np.diag(box)
     vals = {
-        "cellBasisVector1": (box[0], 0.0, 0.0), "cellBasisVector2": (0.0, box[1], 0.0),
-        "cellBasisVector3": (0.0, 0.0, box[2]), "cellOrigin": tuple(0.5 * box),
+        "cellBasisVector1": tuple(cellm[0]), "cellBasisVector2": tuple(cellm[1]),
+        "cellBasisVector3": tuple(cellm[2]), "cellOrigin": tuple(0.5 * cellm.sum(0)),
         "nonbondedFrequency": p.nonbonded_frequency, "fullElectFrequency": p.full_elect_frequency,
         "dielectric": 1.0, "exclude": EXCLUDE_SCALED14 if modified is not None else EXCLUDE_ONETHREE,
         "scale14": float(modified["scale14"]) if modified is not None else 1.0, "nonbondedScaling": 1.0,
@@ -260,8 +273,9 @@ def read(f, name="dumpbench", pme_grid=None, pme_order=4):
         if key not in prm:
             raise ValueError(f"dumpbench: parameter {key} missing")
     basis = np.array([prm["cellBasisVector1"], prm["cellBasisVector2"], prm["cellBasisVector3"]], np.float64)
-    if np.abs(basis - np.diag(np.diag(basis))).max() > 0:
-        raise ValueError("dumpbench: only orthorhombic cells are supported")
+    if not np.linalg.det(basis) > 0:
+        raise ValueError("dumpbench: the cell basis vectors must be right-handed with positive volume")
+    general = np.abs(basis - np.diag(np.diag(basis))).max() > 0
     for key in ("fixedAtomsOn", "lesOn", "pairInteractionOn", "FMAOn", "fullDirectOn", "mollyOn"):
         if prm.get(key, 0):
             raise ValueError(f"dumpbench: {key} is not supported")
@@ -363,7 +377,7 @@ def read(f, name="dumpbench", pme_grid=None, pme_order=4):
     rest = hgs[n_single:]
     waters = rest.size % 3 == 0 and np.array_equal(rest, np.tile([3, 0, 0], rest.size // 3))
     n_beads, n_waters = (n_single, rest.size // 3) if waters else (0, 0)
-    box = np.diag(basis).copy()
+    box = np.linalg.norm(basis, axis=1) if general else np.diag(basis).copy()
     cutoff = float(prm["cutoff"])
     switchdist = float(prm.get("switchingDist", cutoff)) if prm.get("switchingActive", 1) else cutoff
     grid = tuple(pme_grid) if pme_grid is not None else tuple(_smooth_at_least(int(math.ceil(L))) for L in box)
@@ -374,7 +388,7 @@ def read(f, name="dumpbench", pme_grid=None, pme_order=4):
     s = S.System(name=name, pos=pos, charge=charge, vdw_type=vdw, lj_sigma=np.zeros(nt), lj_eps=np.zeros(nt),
                  excl_ptr=ptr, excl_idx=np.asarray(idx, np.int32), box=box, params=params,
                  partition=part if prm.get("alchOn", 0) else None, n_waters=n_waters, n_beads=n_beads,
-                 lj_AB=(A, B))
+                 lj_AB=(A, B), cell=basis if general else None)
     info = dict(params=prm, A=A, B=B, A14=A14, B14=B14, patches=patches, computes=computes,
                 modified_pairs=modified // 2, hydrogen_group_size=hgs, mod_ptr=mptr,
                 mod_idx=np.asarray(midx, np.int32), scale14=scale14)

@@ -45,8 +45,11 @@ def test_golden_file_parses():
 
 def test_rejects_what_the_path_does_not_compute():
     text = GOLDEN.read_text()
-    with pytest.raises(ValueError, match="orthorhombic"):
-        D.read(text.replace("cellBasisVector2 0.000000 20.000000", "cellBasisVector2 1.000000 20.000000"))
+    with pytest.raises(ValueError, match="right-handed"):
+        D.read(text.replace("cellBasisVector2 0.000000 20.000000", "cellBasisVector2 0.000000 -20.000000"))
+    # a general cell is read as System.cell
+    sk, _ = D.read(text.replace("cellBasisVector2 0.000000 20.000000", "cellBasisVector2 1.000000 20.000000"))
+    assert sk.cell is not None and sk.cell[1, 0] == 1.0
     # a 1-4 (modified) exclusion flag needs exclude scaled1-4
     modified = text.replace("0 1 6 1 1 0 0 0 1", "0 1 6 1 1 2 0 0 1").replace("1 4 5 1 1", "1 0 5 2 0 0 0 1 1")
     with pytest.raises(ValueError, match="modified"):
@@ -142,5 +145,33 @@ def test_modified_exclusions_round_trip():
         out.append(orc.mod_excl_fp64(x.pos, x.charge * math.sqrt(332.0636), x.vdw_type, A, B, a14, b14, pp, ii,
                                      x.box, p.cutoff, p.switchdist, 0.8333))
     assert out[0]["npairs"] == out[1]["npairs"] == mi.size
+    for k in ("e_vdw", "e_elec", "e_slow"):
+        assert abs(out[1][k] - out[0][k]) <= 1e-4 * abs(out[0][k]), k
+
+
+def test_general_cell_round_trip():
+    """A system in a general cell written and read back: the lattice rows,
+    the patch list (binned in fractional coordinates) and a byte-identical
+    rewrite; the general-cell oracle gives the same energies on the read data
+    within the file's rounding."""
+    base = systems.small_box(n_waters=300, n_chains=2, chain_len=20, box=30.0, seed=6)
+    s = systems.sheared(base, [[30.0, 0.0, 0.0], [6.0, 29.0, 0.0], [-4.0, 5.0, 30.5]])
+    buf = io.StringIO()
+    D.write(buf, s)
+    text = buf.getvalue()
+    r, info = D.read(text)
+    assert r.cell is not None and np.allclose(r.cell, s.cell, atol=1e-6)
+    assert sorted(np.concatenate(info["patches"]).tolist()) == list(range(s.natoms))
+    buf2 = io.StringIO()
+    D.write(buf2, r)
+    assert buf2.getvalue() == text
+    p = s.params
+    beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+    out = []
+    for x in (s, r):
+        A, B = systems.lj_tables(x)
+        out.append(orc.nb_fp64_cell(x.pos, x.charge * math.sqrt(332.0636), x.vdw_type, A, B, x.excl_ptr,
+                                    x.excl_idx, x.cell, p.cutoff, p.switchdist, beta))
+    assert out[0]["npairs"] == out[1]["npairs"]
     for k in ("e_vdw", "e_elec", "e_slow"):
         assert abs(out[1][k] - out[0][k]) <= 1e-4 * abs(out[0][k]), k
