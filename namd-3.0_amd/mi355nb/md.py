@@ -30,12 +30,17 @@ BOLTZMANN = 0.001987191    # kcal/mol/K (NAMD src/common.h)
 
 
 class NVE:
-    """NVE of a System of chains (stored first) and rigid waters (O, H1, H2)."""
+    """NVE of a System of chains (stored first) and rigid waters (O, H1, H2)
+    in an orthorhombic box (its wrapping, PME reciprocal rows and device
+    position packing are orthorhombic; general cells run through
+    ComputeNonbonded directly)."""
 
     def __init__(self, s: S.System, dt_fs=2.0, temperature=300.0, seed=0, pairlist_every=20, stream=None):
         import torch
         if s.n_beads and not s.chain_len:
             raise ValueError("chain beads without chain_len: bonded terms unknown")
+        if s.cell is not None:
+            raise ValueError("the NVE driver runs orthorhombic boxes; this System has a general cell")
         self.torch = torch
         self.s = s
         self.n = s.natoms
