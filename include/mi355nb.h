@@ -197,8 +197,7 @@ int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], cons
  * coordinates; every image shift is an integer combination of the rounded
  * lattice rows, so pair displacements stay exact and the cutoff rule
  * unchanged.  Every perpendicular width must be at least 2 pairlistdist.
- * Device position packing (update_positions_device) serves orthorhombic
- * boxes only; update_positions serves both. */
+ * update_positions and update_positions_device serve general cells too. */
 int mi355nb_compute_assign_patches_lattice(mi355nb_compute* c, const double lattice[9], const double* pos);
 
 /* Lattice rows a, b, c the kernels use, 9 doubles per system (orthorhombic

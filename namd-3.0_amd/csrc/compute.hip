@@ -1050,10 +1050,10 @@ int mi355nb_compute_finish_batch(mi355nb_compute* c, double* forceOut, double* f
 int mi355nb_compute_update_positions_device(mi355nb_compute* c, const double* pos) {
   CHECK_ARG(c && pos, "null argument");
   CHECK_STATE(c->haveLayout, "assign_patches must precede update_positions_device");
-  CHECK_STATE(!c->L.triclinic, "device position packing serves orthorhombic boxes (use update_positions)");
   MI355NB_HIP(hipSetDevice(c->device));
   MI355NB_HIP(launchPackSlots(c->L.atomStorageSize, c->atomIndex.ptr, c->slotPatch.ptr, c->patchCentre.ptr,
-                              c->patchSystemDev.ptr, c->sysBoxDev.ptr, POS_QUANTUM, pos, c->xyzq.ptr, c->stream));
+                              c->patchSystemDev.ptr, c->sysBoxDev.ptr, c->sysCellDev.ptr, c->L.triclinic ? 1 : 0,
+                              POS_QUANTUM, pos, c->xyzq.ptr, c->stream));
   return MI355NB_OK;
 }
 

@@ -243,20 +243,44 @@ __global__ void sumPartialsKernel(int n, const double* __restrict__ partials, do
 __global__ void __launch_bounds__(INT_THREADS)
 packSlotsKernel(int nslots, const int* __restrict__ atomIndex, const int* __restrict__ slotPatch,
                 const double* __restrict__ centre, const int* __restrict__ patchSystem,
-                const double* __restrict__ sysBox, double quantum, const double* __restrict__ pos,
-                F4* __restrict__ xyzq) {
+                const double* __restrict__ sysBox, const double* __restrict__ sysCell, int general, double quantum,
+                const double* __restrict__ pos, F4* __restrict__ xyzq) {
   const int s = blockIdx.x * INT_THREADS + threadIdx.x;
   if (s >= nslots) return;
   const int a = atomIndex[s];
   if (a < 0) return;
   const int p = slotPatch[s];
-  const double* L = sysBox + 3 * patchSystem[p];
   double q3[3];
+  if (general) {
+    // general cell: the image nearest the patch centre in fractional
+    // coordinates, with the host packing's operation order and no fused
+    // multiply-add in the fractional dot products, so that the result is
+    // bit-identical to it (PatchLayout::packPatches); the subtracted image is
+    // an integer combination of the lattice rows (exact)
+    const double* C = sysCell + 18 * patchSystem[p];
+    double x[3], nv[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double x = pos[3 * a + d] - centre[3 * p + d];
-    x -= L[d] * rint(x / L[d]);
-    q3[d] = rint(x / quantum) * quantum;
+    for (int d = 0; d < 3; ++d) x[d] = pos[3 * a + d] - centre[3 * p + d];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double t = __dmul_rn(C[9 + 3 * j], x[0]);
+      t = __dadd_rn(t, __dmul_rn(C[9 + 3 * j + 1], x[1]));
+      t = __dadd_rn(t, __dmul_rn(C[9 + 3 * j + 2], x[2]));
+      nv[j] = rint(t);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double img = nv[0] * C[d] + nv[1] * C[3 + d] + nv[2] * C[6 + d];
+      q3[d] = rint((x[d] - img) / quantum) * quantum;
+    }
+  } else {
+    const double* L = sysBox + 3 * patchSystem[p];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double x = pos[3 * a + d] - centre[3 * p + d];
+      x -= L[d] * rint(x / L[d]);
+      q3[d] = rint(x / quantum) * quantum;
+    }
   }
   F4 v = xyzq[s];
   v.x = (float)q3[0];
@@ -266,11 +290,11 @@ packSlotsKernel(int nslots, const int* __restrict__ atomIndex, const int* __rest
 }
 
 hipError_t launchPackSlots(int nslots, const int* atomIndex, const int* slotPatch, const double* centre,
-                           const int* patchSystem, const double* sysBox, double quantum, const double* pos, F4* xyzq,
-                           hipStream_t s) {
+                           const int* patchSystem, const double* sysBox, const double* sysCell, int general,
+                           double quantum, const double* pos, F4* xyzq, hipStream_t s) {
   if (nslots <= 0) return hipSuccess;
   hipLaunchKernelGGL(packSlotsKernel, dim3((nslots + INT_THREADS - 1) / INT_THREADS), dim3(INT_THREADS), 0, s,
-                     nslots, atomIndex, slotPatch, centre, patchSystem, sysBox, quantum, pos, xyzq);
+                     nslots, atomIndex, slotPatch, centre, patchSystem, sysBox, sysCell, general, quantum, pos, xyzq);
   return hipGetLastError();
 }
 

@@ -124,8 +124,8 @@ hipError_t launchModifiedExclusionCorrection(int numBlocks, int numSystems, cons
 
 // FP64 positions (3 per atom) -> patch-centred slot coordinates (integrate.hip)
 hipError_t launchPackSlots(int nslots, const int* atomIndex, const int* slotPatch, const double* centre,
-                           const int* patchSystem, const double* sysBox, double quantum, const double* pos, F4* xyzq,
-                           hipStream_t s);
+                           const int* patchSystem, const double* sysBox, const double* sysCell, int general,
+                           double quantum, const double* pos, F4* xyzq, hipStream_t s);
 
 // Force-matching adjoint (adjoint.hip)
 struct AdjointArgsHost {

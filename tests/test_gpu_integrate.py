@@ -96,14 +96,17 @@ class _DevArray:
         self.__cuda_array_interface__ = {"shape": shape, "typestr": typestr, "data": (ptr, False), "version": 3}
 
 
-def test_device_packing_equals_host_packing(torch_cuda):
+@pytest.mark.parametrize("cell", [False, True])
+def test_device_packing_equals_host_packing(torch_cuda, cell):
     torch = torch_cuda
     s = systems.water_box_liquid()
+    if cell:   # a skewed cell: fractional image choice on host and device
+        s = systems.sheared(s, np.array([[s.box[0], 0, 0], [6.0, s.box[1], 0], [-4.0, 5.0, s.box[2]]]))
     c = mc.ComputeNonbonded.from_system(s, stream=torch.cuda.current_stream().cuda_stream)
     n_slots = c.info()["atomStorageSize"]
     rng = np.random.default_rng(7)
     moved = s.pos + rng.normal(scale=0.3, size=s.pos.shape)
-    moved[:50] += s.box        # periodic images must pack the same way
+    moved[:50] += s.cell[0] + s.cell[2] if cell else s.box   # periodic images must pack the same way
     c.update_positions(moved)
     torch.cuda.synchronize()
     view = torch.as_tensor(_DevArray(c.device_buffers()["xyzq"], (n_slots, 4), "<f4"), device="cuda")
