@@ -200,6 +200,12 @@ int mi355nb_compute_assign_patches(mi355nb_compute* c, const double box[3], cons
  * update_positions and update_positions_device serve general cells too. */
 int mi355nb_compute_assign_patches_lattice(mi355nb_compute* c, const double lattice[9], const double* pos);
 
+/* A batch (see mi355nb_compute_assign_patches_batch) in which every system
+ * has its own general cell: lattices holds 9 doubles (rows a, b, c) per
+ * system, each laid out as mi355nb_compute_assign_patches_lattice does. */
+int mi355nb_compute_assign_patches_batch_lattice(mi355nb_compute* c, int numSystems, const int32_t* sysAtomStart,
+                                                 const double* lattices, const double* pos);
+
 /* Lattice rows a, b, c the kernels use, 9 doubles per system (orthorhombic
  * systems: the diagonal of their box). */
 int mi355nb_compute_get_system_cells(const mi355nb_compute* c, double* cells);

@@ -454,19 +454,22 @@ static int uploadModifiedSlots(mi355nb_compute* c) {
 static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomStart, const double* boxes,
                              const double* pos, const double* cell = nullptr) {
   if (cell) {
-    // every perpendicular width at least twice pairlistdist (one image per pair)
-    const double* a = cell;
-    const double* b = cell + 3;
-    const double* cc = cell + 6;
-    const double cr[3][3] = {{b[1] * cc[2] - b[2] * cc[1], b[2] * cc[0] - b[0] * cc[2], b[0] * cc[1] - b[1] * cc[0]},
-                             {cc[1] * a[2] - cc[2] * a[1], cc[2] * a[0] - cc[0] * a[2], cc[0] * a[1] - cc[1] * a[0]},
-                             {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]}};
-    const double V = a[0] * cr[0][0] + a[1] * cr[0][1] + a[2] * cr[0][2];
-    CHECK_ARG(V > 0, "lattice vectors a, b, c must be right-handed with positive volume");
-    for (int j = 0; j < 3; ++j)
-      CHECK_ARG(V / std::sqrt(cr[j][0] * cr[j][0] + cr[j][1] * cr[j][1] + cr[j][2] * cr[j][2]) >=
-                    2.0 * c->p.pairlistdist,
-                "every perpendicular width of the cell must be at least twice pairlistdist");
+    // every system: right-handed, every perpendicular width at least twice
+    // pairlistdist (one image per pair)
+    for (int sy = 0; sy < nsys; ++sy) {
+      const double* a = cell + 9 * (size_t)sy;
+      const double* b = a + 3;
+      const double* cc = a + 6;
+      const double cr[3][3] = {{b[1] * cc[2] - b[2] * cc[1], b[2] * cc[0] - b[0] * cc[2], b[0] * cc[1] - b[1] * cc[0]},
+                               {cc[1] * a[2] - cc[2] * a[1], cc[2] * a[0] - cc[0] * a[2], cc[0] * a[1] - cc[1] * a[0]},
+                               {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]}};
+      const double V = a[0] * cr[0][0] + a[1] * cr[0][1] + a[2] * cr[0][2];
+      CHECK_ARG(V > 0, "lattice vectors a, b, c must be right-handed with positive volume");
+      for (int j = 0; j < 3; ++j)
+        CHECK_ARG(V / std::sqrt(cr[j][0] * cr[j][0] + cr[j][1] * cr[j][1] + cr[j][2] * cr[j][2]) >=
+                      2.0 * c->p.pairlistdist,
+                  "every perpendicular width of the cell must be at least twice pairlistdist");
+    }
   } else {
     for (int s = 0; s < nsys; ++s)
       for (int d = 0; d < 3; ++d)
@@ -475,9 +478,7 @@ static int assignPatchesImpl(mi355nb_compute* c, int nsys, const int32_t* atomSt
   for (int i = 0; i < 3 * c->natoms; ++i) CHECK_ARG(std::isfinite(pos[i]), "non-finite position");
   std::string err;
   std::vector<int> start(atomStart, atomStart + nsys + 1);
-  const bool ok = cell ? c->L.buildCell(c->natoms, pos, cell, c->p.pairlistdist + c->p.margin, err)
-                       : c->L.buildBatch(nsys, start.data(), pos, boxes, c->p.pairlistdist + c->p.margin, err);
-  if (!ok) {
+  if (!c->L.buildBatch(nsys, start.data(), pos, boxes, c->p.pairlistdist + c->p.margin, err, cell)) {
     setLastError(err);
     return MI355NB_ERR_ARG;
   }
@@ -617,6 +618,14 @@ int mi355nb_compute_assign_patches_lattice(mi355nb_compute* c, const double latt
   CHECK_STATE(c->haveMolecule, "set_molecule must precede assign_patches_lattice");
   const int32_t start[2] = {0, c->natoms};
   return assignPatchesImpl(c, 1, start, nullptr, pos, lattice);
+}
+
+int mi355nb_compute_assign_patches_batch_lattice(mi355nb_compute* c, int numSystems, const int32_t* sysAtomStart,
+                                                 const double* lattices, const double* pos) {
+  CHECK_ARG(c && sysAtomStart && lattices && pos && numSystems >= 1, "bad batch arguments");
+  CHECK_STATE(c->haveMolecule, "set_molecule must precede assign_patches_batch_lattice");
+  CHECK_ARG(sysAtomStart[0] == 0 && sysAtomStart[numSystems] == c->natoms, "system ranges must cover the molecule");
+  return assignPatchesImpl(c, numSystems, sysAtomStart, nullptr, pos, lattices);
 }
 
 int mi355nb_compute_get_system_cells(const mi355nb_compute* c, double* cells) {

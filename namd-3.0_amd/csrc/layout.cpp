@@ -279,7 +279,7 @@ bool PatchLayout::buildCell(int n, const double* pos, const double cellIn[9], do
 }
 
 bool PatchLayout::buildBatch(int nsys, const int* atomStart, const double* pos, const double* boxes,
-                             double patchDim, std::string& err) {
+                             double patchDim, std::string& err, const double* cells) {
   if (nsys < 1) { err = "no systems"; return false; }
   for (int s = 0; s < nsys; ++s)
     if (atomStart[s + 1] <= atomStart[s]) { err = "every system needs at least one atom"; return false; }
@@ -294,7 +294,7 @@ bool PatchLayout::buildBatch(int nsys, const int* atomStart, const double* pos, 
   if (nsys == 1) {
     // one system: its layout is the whole layout (buildOne numbers patches,
     // lists and slots from zero, as the general path below would)
-    if (!buildOne(natoms, pos, boxes, patchDim, err)) return false;
+    if (!buildOne(natoms, pos, boxes, patchDim, err, cells)) return false;
     for (int d = 0; d < 3; ++d) sysBox[d] = box[d];
     patchSystem.assign(patches.size(), 0);
     sysSlotStart[1] = atomStorageSize;
@@ -308,16 +308,26 @@ bool PatchLayout::buildBatch(int nsys, const int* atomStart, const double* pos, 
   atomIndex.clear();
   jtileCapacity = 0;
   int slotOff = 0;
+  std::vector<double> cellsOut(cells ? 18 * (size_t)nsys : 0);
   for (int s = 0; s < nsys; ++s) {
     const int a0 = atomStart[s];
     PatchLayout sub;
-    if (!sub.buildOne(atomStart[s + 1] - a0, pos + 3 * (size_t)a0, boxes + 3 * s, patchDim, err)) {
+    if (!sub.buildOne(atomStart[s + 1] - a0, pos + 3 * (size_t)a0, cells ? nullptr : boxes + 3 * s, patchDim, err,
+                      cells ? cells + 9 * (size_t)s : nullptr)) {
       if (nsys > 1) err = "system " + std::to_string(s) + ": " + err;
       return false;
     }
     if (s == 0)
       for (int d = 0; d < 3; ++d) { dims[d] = sub.dims[d]; box[d] = sub.box[d]; patchSize[d] = sub.patchSize[d]; }
     for (int d = 0; d < 3; ++d) sysBox[3 * s + d] = sub.box[d];
+    if (cells) {
+      for (int k = 0; k < 9; ++k) {
+        cellsOut[18 * (size_t)s + k] = sub.cell[k];
+        cellsOut[18 * (size_t)s + 9 + k] = sub.recip[k];
+      }
+      if (s == 0)
+        for (int k = 0; k < 9; ++k) { cell[k] = sub.cell[k]; recip[k] = sub.recip[k]; }
+    }
     const int patchOff = (int)patches.size();
     const int computeOff = (int)computes.size();
     centre.insert(centre.end(), sub.centre.begin(), sub.centre.end());
@@ -358,7 +368,11 @@ bool PatchLayout::buildBatch(int nsys, const int* atomStart, const double* pos, 
   vdwTypes.assign(atomStorageSize, 0);
   exclStartMaxdiff.assign(atomStorageSize, I2{0, 0});
   effPos.assign(3 * (size_t)natoms, 0.0);
-  fillSysCell();
+  triclinic = cells != nullptr;
+  if (triclinic)
+    sysCell = cellsOut;
+  else
+    fillSysCell();
   return true;
 }
 
@@ -627,13 +641,16 @@ void PatchLayout::packPatches(const double* pos, int p0, int p1) {
       if (a < 0) continue;
       double q3[3];
       if (triclinic) {
-        // the image nearest the patch centre in fractional coordinates
+        // the image nearest the patch centre in fractional coordinates (the
+        // patch's system's lattice rows and reciprocal rows)
+        const double* C = &sysCell[18 * (size_t)patchSystem[p]];
+        const double* Rr = C + 9;
         double x[3], nv[3];
         for (int d = 0; d < 3; ++d) x[d] = pos[3 * a + d] - centre[3 * p + d];
         for (int j = 0; j < 3; ++j)
-          nv[j] = roundEven(recip[3 * j] * x[0] + recip[3 * j + 1] * x[1] + recip[3 * j + 2] * x[2]);
+          nv[j] = roundEven(Rr[3 * j] * x[0] + Rr[3 * j + 1] * x[1] + Rr[3 * j + 2] * x[2]);
         for (int d = 0; d < 3; ++d)
-          q3[d] = quantize(x[d] - (nv[0] * cell[d] + nv[1] * cell[3 + d] + nv[2] * cell[6 + d]), POS_QUANTUM);
+          q3[d] = quantize(x[d] - (nv[0] * C[d] + nv[1] * C[3 + d] + nv[2] * C[6 + d]), POS_QUANTUM);
       } else
       for (int d = 0; d < 3; ++d) {
         // image of the atom closest to its patch centre

@@ -118,9 +118,11 @@ struct PatchLayout {
   // combination of the quantized lattice vectors (exact in FP32).
   bool buildCell(int n, const double* pos, const double cellIn[9], double patchDim, std::string& err);
 
-  // The same for numSystems systems; boxes holds 3 edge lengths per system.
+  // The same for numSystems systems; boxes holds 3 edge lengths per system,
+  // or (cells not null) every system has a general cell, 9 doubles (rows a,
+  // b, c) per system in cells, laid out as buildCell does.
   bool buildBatch(int numSystems, const int* atomStart, const double* pos, const double* boxes,
-                  double patchDim, std::string& err);
+                  double patchDim, std::string& err, const double* cells = nullptr);
 
   // Fill slot arrays for the current assignment (the xyzq staging of
   // HomePatch::positionsReady, src/HomePatch.C:1965-1995).  chargeScaled is q*sqrt(COULOMB*scaling/eps).

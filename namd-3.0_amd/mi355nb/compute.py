@@ -78,8 +78,13 @@ class ComputeNonbonded:
         c.set_molecule(np.concatenate([s.vdw_type for s in systems]), np.concatenate([s.charge for s in systems]),
                        A, B, np.concatenate(ptr), np.concatenate(idx) if idx else np.zeros(0, np.int32))
         c.sys_atom_start = starts
-        c.assign_patches_batch(starts, np.array([s.box for s in systems], np.float64),
-                               np.concatenate([s.pos for s in systems]))
+        if any(s.cell is not None for s in systems):
+            # general cells (an orthorhombic member as the diagonal of its box)
+            cells = np.array([s.cell if s.cell is not None else np.diag(s.box) for s in systems], np.float64)
+            c.assign_patches_batch_lattice(starts, cells, np.concatenate([s.pos for s in systems]))
+        else:
+            c.assign_patches_batch(starts, np.array([s.box for s in systems], np.float64),
+                                   np.concatenate([s.pos for s in systems]))
         return c
 
     def close(self):
@@ -135,6 +140,15 @@ class ComputeNonbonded:
         L.check(L.lib().mi355nb_compute_assign_patches_batch(
             self._h, starts.size - 1, starts, np.ascontiguousarray(boxes, np.float64).reshape(-1),
             np.ascontiguousarray(pos, np.float64).reshape(-1)), "assign_patches_batch")
+
+    def assign_patches_batch_lattice(self, sys_atom_start, cells, pos):
+        """A batch whose systems each have a general cell (cells: (S, 3, 3) lattice rows)."""
+        starts = np.ascontiguousarray(sys_atom_start, np.int32)
+        self.sys_atom_start = starts
+        cl = np.ascontiguousarray(cells, np.float64).reshape(-1)
+        L.check(L.lib().mi355nb_compute_assign_patches_batch_lattice(
+            self._h, starts.size - 1, starts.ctypes.data, cl.ctypes.data,
+            np.ascontiguousarray(pos, np.float64).reshape(-1).ctypes.data), "assign_patches_batch_lattice")
 
     def set_alch(self, partition, alch, method="fep"):
         """partition: int8 per atom (0, 1, 2) or None to switch off; alch: one

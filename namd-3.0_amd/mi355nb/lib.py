@@ -72,7 +72,7 @@ EXPORTED = [
     "mi355nb_pme_frac_coords", "mi355nb_compute_assign_patches_batch", "mi355nb_alch_data_fill",
     "mi355nb_compute_set_alch", "mi355nb_compute_set_alch_method", "mi355nb_compute_finish_batch", "mi355nb_compute_launch_graph",
     "mi355nb_compute_update_parameters", "mi355nb_compute_force_matching_grad",
-    "mi355nb_build_force_energy_tables", "mi355nb_compute_set_force_table", "mi355nb_compute_set_vdw_force_switching", "mi355nb_compute_set_modified_exclusions", "mi355nb_compute_assign_patches_lattice", "mi355nb_compute_get_system_cells", "mi355nb_compute_modified_exclusion_correction", "mi355nb_compute_pme_atoms",
+    "mi355nb_build_force_energy_tables", "mi355nb_compute_set_force_table", "mi355nb_compute_set_vdw_force_switching", "mi355nb_compute_set_modified_exclusions", "mi355nb_compute_assign_patches_lattice", "mi355nb_compute_assign_patches_batch_lattice", "mi355nb_compute_get_system_cells", "mi355nb_compute_modified_exclusion_correction", "mi355nb_compute_pme_atoms",
     "mi355nb_compute_get_system_boxes", "mi355nb_pme_b_moduli", "mi355nb_pme_recip_create",
     "mi355nb_pme_recip_destroy", "mi355nb_pme_recip_fft_r2c", "mi355nb_pme_recip_fft_c2r",
     "mi355nb_pme_recip_scalar_sum", "mi355nb_pme_recip_solve", "mi355nb_compute_exclusion_correction",
@@ -137,6 +137,7 @@ def lib():
     L.mi355nb_compute_set_force_table.argtypes = [vp, C.c_int]
     L.mi355nb_compute_set_vdw_force_switching.argtypes = [vp, C.c_int]
     L.mi355nb_compute_assign_patches_lattice.argtypes = [vp, vp, vp]
+    L.mi355nb_compute_assign_patches_batch_lattice.argtypes = [vp, C.c_int, vp, vp, vp]
     L.mi355nb_compute_get_system_cells.argtypes = [vp, vp]
     L.mi355nb_compute_set_modified_exclusions.argtypes = [vp, vp, vp, C.c_double, vp, vp]
     L.mi355nb_compute_modified_exclusion_correction.argtypes = [vp, vp]
