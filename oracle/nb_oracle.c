@@ -26,7 +26,8 @@
  *                          :900-919; alchemical: ComputeNonbondedFEP.C:136-159,
  *                          ComputeNonbondedTI.C:113-128).
  *  oracle_nb_fp64_cell     oracle_nb_fp64 in a general (triclinic) cell, by
- *                          brute force over pairs and images.
+ *                          brute force over pairs and images (energy- or
+ *                          force-switched LJ).
  *  oracle_mod_excl_fp64    the modified (scaled 1-4) exclusion correction
  *                          of modifiedExclusionForce
  *                          (ComputeBondedCUDAKernel.cu:316-632) in FP64.
@@ -1404,10 +1405,12 @@ int oracle_mod_excl_alch_fp64(int n, const double* pos, const double* q, const i
  * layout it checks. */
 int oracle_nb_fp64_cell(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
                         const double* A, const double* B, const int64_t* eptr, const int32_t* eidx,
-                        const double cell[9], double cutoff, double switchdist, double beta, double* f, double* fs,
-                        double* energies, double* virial, double* virialSlow, int64_t* npairs, int nthreads) {
+                        const double cell[9], double cutoff, double switchdist, double beta, int forceSwitch,
+                        double* f, double* fs, double* energies, double* virial, double* virialSlow,
+                        int64_t* npairs, int nthreads) {
   PairConst c;
   memset(&c, 0, sizeof(c));
+  if (forceSwitch) set_force_switch(&c, cutoff, switchdist);
   c.rc = cutoff;
   c.rc2 = cutoff * cutoff;
   c.rs2 = switchdist * switchdist;

@@ -60,7 +60,7 @@ def lib():
         L.oracle_mod_excl_fp64.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, dp, dp, i64p, i32p, dp,
                                            C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp, dp, i64p]
         L.oracle_nb_fp64_cell.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, i64p, i32p, dp, C.c_double,
-                                          C.c_double, C.c_double, dp, dp, dp, dp, dp, i64p, C.c_int]
+                                          C.c_double, C.c_double, C.c_int, dp, dp, dp, dp, dp, i64p, C.c_int]
         L.oracle_mod_excl_alch_fp64.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, dp, dp, i64p, i32p, dp,
                                                 C.c_double, C.c_double, C.c_double, i8p, dp, C.c_int, dp, dp, dp,
                                                 dp, dp, i64p]
@@ -130,7 +130,8 @@ def nb_fp64_alch(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, box, cutoff,
                 virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]))
 
 
-def nb_fp64_cell(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, cell, cutoff, switchdist, beta, nthreads=0):
+def nb_fp64_cell(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, cell, cutoff, switchdist, beta, nthreads=0,
+                 vdw_force_switching=False):
     """nb_fp64 in a general periodic cell (cell: 3 x 3 lattice rows a, b, c),
     by brute force over all pairs and the 27 images around the minimum image."""
     n = len(q_scaled)
@@ -144,7 +145,8 @@ def nb_fp64_cell(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, cell, cutoff
     lib().oracle_nb_fp64_cell(n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32),
                               ntypes, _c(A, np.float64).reshape(-1), _c(B, np.float64).reshape(-1),
                               _c(excl_ptr, np.int64), _c(excl_idx, np.int32), _c(cell, np.float64).reshape(-1),
-                              cutoff, switchdist, beta, f, fs, en, vir, virs, npairs, int(nthreads))
+                              cutoff, switchdist, beta, int(bool(vdw_force_switching)), f, fs, en, vir, virs,
+                              npairs, int(nthreads))
     return dict(f=f.reshape(n, 3), fs=fs.reshape(n, 3), e_vdw=en[0], e_elec=en[1], e_slow=en[2],
                 virial=vir.reshape(3, 3), virial_slow=virs.reshape(3, 3), npairs=int(npairs[0]))
 

@@ -90,3 +90,18 @@ def test_forces_are_energy_gradient(tri):
             g = sum(ep[k] - em[k] for k in ("e_vdw", "e_elec", "e_slow")) / (2 * h)
             f = o["f"][a, d] + o["fs"][a, d]
             assert abs(f + g) <= 2e-5 * max(1.0, abs(f)), (a, d, f, -g)
+
+
+def test_orthorhombic_cell_matches_cell_list_oracle_force_switched(tri):
+    base, _ = tri
+    p = base.params
+    A, B = systems.lj_tables(base)
+    beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+    q = base.charge * math.sqrt(COULOMB)
+    o = orc.nb_fp64(base.pos, q, base.vdw_type, A, B, base.excl_ptr, base.excl_idx, base.box, p.cutoff,
+                    p.switchdist, beta, vdw_force_switching=True)
+    c = orc.nb_fp64_cell(base.pos, q, base.vdw_type, A, B, base.excl_ptr, base.excl_idx, np.diag(base.box), p.cutoff,
+                         p.switchdist, beta, vdw_force_switching=True)
+    for k in ("e_vdw", "e_elec", "e_slow"):
+        assert abs(c[k] - o[k]) <= 1e-11 * abs(o[k]), k
+    assert np.abs(c["f"] - o["f"]).max() <= 1e-9 * np.abs(o["f"]).max()

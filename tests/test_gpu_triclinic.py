@@ -23,12 +23,15 @@ TOL = 1e-5
 CELL = np.array([[34.0, 0.0, 0.0], [8.0, 33.0, 0.0], [-6.0, 7.0, 34.0]])
 
 
-@pytest.mark.parametrize("cell", ["skewed", "orthorhombic-as-lattice"])
-def test_triclinic_matches_oracle(torch_cuda, cell):
+@pytest.mark.parametrize("cell,force_switch", [("skewed", False), ("orthorhombic-as-lattice", False),
+                                               ("skewed", True)])
+def test_triclinic_matches_oracle(torch_cuda, cell, force_switch):
     torch = torch_cuda
     base = systems.small_box(n_waters=1000, n_chains=3, chain_len=20, box=34.0, seed=3)
     s = systems.sheared(base, CELL if cell == "skewed" else np.diag(base.box))
     c = ComputeNonbonded.from_system(s, stream=torch.cuda.current_stream().cuda_stream)
+    if force_switch:
+        c.set_vdw_force_switching(True)
     print(f"[triclinic {cell}] {s.natoms} atoms", flush=True)
     r = c.compute(True, True, True, do_pairlist=True)
     pos = c.effective_positions()
@@ -37,7 +40,7 @@ def test_triclinic_matches_oracle(torch_cuda, cell):
     A, B = systems.lj_tables(s)
     beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
     o = orc.nb_fp64_cell(pos, s.charge * math.sqrt(COULOMB), s.vdw_type, A, B, s.excl_ptr, s.excl_idx, lat,
-                         p.cutoff, p.switchdist, beta)
+                         p.cutoff, p.switchdist, beta, vdw_force_switching=force_switch)
     print(f"    pairs within the cutoff gpu {r['stat']['numPairsCutoff']} oracle {o['npairs']}", flush=True)
     assert r["stat"]["numPairsCutoff"] == o["npairs"]
     ef, es = rel_rms(r["f"], o["f"]), rel_rms(r["fs"], o["fs"])
