@@ -1257,7 +1257,20 @@ static int mod_excl_core(int n, const double* pos, const double* q, const int32_
                          const int64_t* mptr, const int32_t* midx, const double L[3], double cutoff,
                          double switchdist, double scale14, int forceSwitch, const int8_t* part, const AlchF* al,
                          double* f, double* fs, double* energies, int nenergies, double* virial,
-                         double* virialSlow, int64_t* npairs) {
+                         double* virialSlow, int64_t* npairs, const double* cell) {
+  /* cell (rows a, b, c; NULL: the orthorhombic L): minimum image by
+   * rounding fractional coordinates */
+  double R[9] = {0};
+  if (cell) {
+    const double* a = cell;
+    const double* b = cell + 3;
+    const double* cc = cell + 6;
+    const double bxc[3] = {b[1] * cc[2] - b[2] * cc[1], b[2] * cc[0] - b[0] * cc[2], b[0] * cc[1] - b[1] * cc[0]};
+    const double cxa[3] = {cc[1] * a[2] - cc[2] * a[1], cc[2] * a[0] - cc[0] * a[2], cc[0] * a[1] - cc[1] * a[0]};
+    const double axb[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double V = a[0] * bxc[0] + a[1] * bxc[1] + a[2] * bxc[2];
+    for (int d = 0; d < 3; ++d) { R[d] = bxc[d] / V; R[3 + d] = cxa[d] / V; R[6 + d] = axb[d] / V; }
+  }
   PairConst c;
   memset(&c, 0, sizeof(c));
   if (forceSwitch) set_force_switch(&c, cutoff, switchdist);
@@ -1288,7 +1301,14 @@ static int mod_excl_core(int n, const double* pos, const double* q, const int32_
         if (midx[m] == j) rep = 1;
       if (rep) continue;
       double d[3];
-      for (int u = 0; u < 3; ++u) d[u] = min_image(pos[3 * j + u] - pos[3 * i + u], L[u]);
+      if (cell) {
+        for (int u = 0; u < 3; ++u) d[u] = pos[3 * j + u] - pos[3 * i + u];
+        double sv[3];
+        for (int k = 0; k < 3; ++k) sv[k] = nearbyint(R[3 * k] * d[0] + R[3 * k + 1] * d[1] + R[3 * k + 2] * d[2]);
+        for (int u = 0; u < 3; ++u) d[u] -= sv[0] * cell[u] + sv[1] * cell[3 + u] + sv[2] * cell[6 + u];
+      } else {
+        for (int u = 0; u < 3; ++u) d[u] = min_image(pos[3 * j + u] - pos[3 * i + u], L[u]);
+      }
       const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
       if (!(r2 < c.rc2)) continue;
       const int ab = type[i] * ntypes + type[j];
@@ -1376,7 +1396,18 @@ int oracle_mod_excl_fp64(int n, const double* pos, const double* q, const int32_
                          double switchdist, double scale14, int forceSwitch, double* f, double* fs,
                          double* energies, double* virial, double* virialSlow, int64_t* npairs) {
   return mod_excl_core(n, pos, q, type, ntypes, A, B, A14, B14, mptr, midx, L, cutoff, switchdist, scale14,
-                       forceSwitch, NULL, NULL, f, fs, energies, 3, virial, virialSlow, npairs);
+                       forceSwitch, NULL, NULL, f, fs, energies, 3, virial, virialSlow, npairs, NULL);
+}
+
+/* The same in a general cell (rows a, b, c). */
+int oracle_mod_excl_cell_fp64(int n, const double* pos, const double* q, const int32_t* type, int ntypes,
+                              const double* A, const double* B, const double* A14, const double* B14,
+                              const int64_t* mptr, const int32_t* midx, const double cell[9], double cutoff,
+                              double switchdist, double scale14, int forceSwitch, double* f, double* fs,
+                              double* energies, double* virial, double* virialSlow, int64_t* npairs) {
+  const double L[3] = {0, 0, 0};
+  return mod_excl_core(n, pos, q, type, ntypes, A, B, A14, B14, mptr, midx, L, cutoff, switchdist, scale14,
+                       forceSwitch, NULL, NULL, f, fs, energies, 3, virial, virialSlow, npairs, cell);
 }
 
 /* With alchemical partitions (lam[13] and energies[12] as oracle_nb_fp64_alch;
@@ -1394,7 +1425,7 @@ int oracle_mod_excl_alch_fp64(int n, const double* pos, const double* q, const i
   al.decouple = decouple;
   al.coeff = lam[12];
   return mod_excl_core(n, pos, q, type, ntypes, A, B, A14, B14, mptr, midx, L, cutoff, switchdist, scale14, 0,
-                       part, &al, f, fs, energies, 12, virial, virialSlow, npairs);
+                       part, &al, f, fs, energies, 12, virial, virialSlow, npairs, NULL);
 }
 
 /* ---- general (triclinic) cell ----------------------------------------------

@@ -61,6 +61,7 @@ def lib():
                                            C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp, dp, i64p]
         L.oracle_nb_fp64_cell.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, i64p, i32p, dp, C.c_double,
                                           C.c_double, C.c_double, C.c_int, dp, dp, dp, dp, dp, i64p, C.c_int]
+        L.oracle_mod_excl_cell_fp64.argtypes = L.oracle_mod_excl_fp64.argtypes
         L.oracle_mod_excl_alch_fp64.argtypes = [C.c_int, dp, dp, i32p, C.c_int, dp, dp, dp, dp, i64p, i32p, dp,
                                                 C.c_double, C.c_double, C.c_double, i8p, dp, C.c_int, dp, dp, dp,
                                                 dp, dp, i64p]
@@ -152,12 +153,13 @@ def nb_fp64_cell(pos, q_scaled, vdw_type, A, B, excl_ptr, excl_idx, cell, cutoff
 
 
 def mod_excl_fp64(pos, q_scaled, vdw_type, A, B, A14, B14, mod_ptr, mod_idx, box, cutoff, switchdist, scale14,
-                  vdw_force_switching=False, partition=None, alch=None):
+                  vdw_force_switching=False, partition=None, alch=None, cell=None):
     """FP64 modified (scaled 1-4) exclusion correction over the listed pairs
     (atom CSR mod_ptr / mod_idx): returns f, fs (n x 3), e_vdw, e_elec, e_slow,
     virial, virial_slow, npairs (pairs within the cutoff).  With partition and
     alch (as nb_fp64_alch takes them) the alchemical correction, and also the
-    lambda2 sums e_*_s and the TI sums e_*_ti_1/2."""
+    lambda2 sums e_*_s and the TI sums e_*_ti_1/2.  cell (3 x 3 lattice rows):
+    a general cell instead of the orthorhombic box."""
     n = len(q_scaled)
     ntypes = int(round(np.sqrt(np.asarray(A).size)))
     f = np.zeros(3 * n)
@@ -168,10 +170,16 @@ def mod_excl_fp64(pos, q_scaled, vdw_type, A, B, A14, B14, mod_ptr, mod_idx, box
     npairs = np.zeros(1, np.int64)
     args = (n, _c(pos, np.float64).reshape(-1), _c(q_scaled, np.float64), _c(vdw_type, np.int32), ntypes,
             _c(A, np.float64).reshape(-1), _c(B, np.float64).reshape(-1), _c(A14, np.float64).reshape(-1),
-            _c(B14, np.float64).reshape(-1), _c(mod_ptr, np.int64), _c(mod_idx, np.int32), _c(box, np.float64),
+            _c(B14, np.float64).reshape(-1), _c(mod_ptr, np.int64), _c(mod_idx, np.int32),
+            _c(box if box is not None else np.zeros(3), np.float64),
             cutoff, switchdist, scale14)
     out = dict()
-    if partition is None:
+    if cell is not None:
+        if partition is not None:
+            raise ValueError("the general-cell modified-exclusion oracle has no alchemical route")
+        args = args[:11] + (_c(cell, np.float64).reshape(-1),) + args[12:]
+        lib().oracle_mod_excl_cell_fp64(*args, int(bool(vdw_force_switching)), f, fs, en, vir, virs, npairs)
+    elif partition is None:
         lib().oracle_mod_excl_fp64(*args, int(bool(vdw_force_switching)), f, fs, en, vir, virs, npairs)
     else:
         if vdw_force_switching:

@@ -228,3 +228,19 @@ def test_alch_forces_are_energy_gradient():
             g_slow = (ep["e_slow"] - em["e_slow"]) / (2 * h)
             assert abs(o["f"][a, d] + g_fast) <= 1e-5 * max(1.0, abs(g_fast)), (a, d)
             assert abs(o["fs"][a, d] + g_slow) <= 1e-5 * max(1.0, abs(g_slow)), (a, d)
+
+
+def test_general_cell_variant_matches_the_box_in_an_orthorhombic_cell():
+    s = systems.small_box(n_waters=60, n_chains=3, chain_len=20, box=20.0, seed=5)
+    p = s.params
+    A, B = systems.lj_tables(s)
+    A14, B14 = systems.lj14_tables(s)
+    mptr, midx = systems.chain_pairs_14(s)
+    q = s.charge * math.sqrt(COULOMB)
+    o = orc.mod_excl_fp64(s.pos, q, s.vdw_type, A, B, A14, B14, mptr, midx, s.box, p.cutoff, p.switchdist, 0.8333)
+    c = orc.mod_excl_fp64(s.pos, q, s.vdw_type, A, B, A14, B14, mptr, midx, s.box, p.cutoff, p.switchdist, 0.8333,
+                          cell=np.diag(s.box))
+    assert c["npairs"] == o["npairs"]
+    for k in ("e_vdw", "e_elec", "e_slow"):
+        assert abs(c[k] - o[k]) <= 1e-12 * abs(o[k]), k
+    assert np.abs(c["f"] - o["f"]).max() <= 1e-10 * np.abs(o["f"]).max()

@@ -186,6 +186,42 @@ def test_batch_of_general_cells(torch_cuda):
     c.close()
 
 
+def test_modified_exclusions_in_a_skewed_cell(torch_cuda):
+    """Scaled 1-4 exclusions (the chains' 1-4 pairs, scale14 0.8333, 1-4 LJ
+    0.5 A / 0.7 B) in a skewed cell: the correction's energies within 1e-9 of
+    the general-cell modified-exclusion oracle, pair kernel + correction
+    forces within 1e-5 of the general-cell oracles' sum."""
+    torch = torch_cuda
+    base = systems.small_box(n_waters=1000, n_chains=3, chain_len=20, box=34.0, seed=3)
+    s = systems.sheared(base, CELL)
+    c = ComputeNonbonded.from_system(s, stream=torch.cuda.current_stream().cuda_stream)
+    A14, B14 = systems.lj14_tables(s)
+    mp, mi = systems.chain_pairs_14(s)
+    c.set_modified_exclusions(mp, mi, 0.8333, A14, B14)
+    c.launch_work(do_energy=False, do_virial=False, do_slow=True, do_pairlist=True)
+    ev = c.modified_exclusion_correction()
+    r = c.finish_batch()
+    torch.cuda.synchronize()
+    ev = ev.cpu().numpy()[0]
+    pos, lat = c.effective_positions(), c.system_cells()[0]
+    p = s.params
+    A, B = systems.lj_tables(s)
+    q32 = (s.charge * math.sqrt(COULOMB)).astype(np.float32).astype(np.float64)
+    m = orc.mod_excl_fp64(pos, q32, s.vdw_type, A, B, A14, B14, mp, mi, None, p.cutoff, p.switchdist, 0.8333,
+                          cell=lat)
+    assert m["npairs"] == mi.size
+    for k, key in enumerate(("e_vdw", "e_elec", "e_slow")):
+        print(f"    {key} gpu {ev[k]:.12e} oracle {m[key]:.12e}", flush=True)
+        assert abs(ev[k] - m[key]) <= 1e-9 * abs(m[key]), key
+    beta = systems.ewald_coefficient(p.cutoff, p.ewald_tol)
+    o = orc.nb_fp64_cell(pos, s.charge * math.sqrt(COULOMB), s.vdw_type, A, B, s.excl_ptr, s.excl_idx, lat,
+                         p.cutoff, p.switchdist, beta)
+    ef, es = rel_rms(r[2], o["f"] + m["f"]), rel_rms(r[3], o["fs"] + m["fs"])
+    print(f"    totals fast {ef:.2e} slow {es:.2e}", flush=True)
+    assert ef < TOL and es < TOL
+    c.close()
+
+
 def test_lattice_argument_errors(torch_cuda):
     base = systems.small_box(n_waters=300, n_chains=1, chain_len=20, box=28.0, seed=3)
     c = ComputeNonbonded.from_system(base)
