@@ -91,7 +91,8 @@ typedef struct {
   int32_t numCandidateLists;    /* (compute, i-tile) lists before refinement */
   int32_t patchDims[3];
   int64_t jtileCapacity;
-  double box[3];                /* edge lengths after quantisation to 2^-17 A (system 0) */
+  double box[3];                /* edge lengths after quantisation to 2^-17 A (system 0; a general
+                                   cell: the lengths of its rows, which get_system_cells gives) */
   int32_t numSystems;           /* independent systems of a batch (1 otherwise) */
 } mi355nb_layout_info;
 
